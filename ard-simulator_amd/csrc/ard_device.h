@@ -1,0 +1,87 @@
+// Device-side descriptors shared by the ARD kernels and the host context.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace ard {
+
+// One problem of a grouped, strided GEMM  C = alpha * A * B  (A: M x K row-major).
+// B(k, n) = B[k*bK + (n % N1)*b1 + (n / N1)*b2], C(m, n) likewise with cM, c1, c2.
+// Every per-axis DCT pass is one such problem: A is the cosine matrix of the
+// axis and B the volume (or face planes) viewed along that axis.
+struct GemmDesc {
+  const float* A;
+  const float* B;
+  float* C;
+  int M, N, K, N1;
+  long long bK, b1, b2;
+  long long cM, c1, c2;
+  float alpha;
+  int tiles_n;  // ceil(N / 64)
+  int tile0;    // first tile of this problem in the launch
+  // Modal epilogue (used by the last forward pass of the plain form):
+  // C holds the previous modes on entry and receives the next ones,
+  //   C = a*cur - b*C + g*(alpha*AB)   (ARD/dct_partition.cpp:66-70).
+  const float* ep_cur;
+  const float* ep_a;
+  const float* ep_g;
+  float ep_b;
+};
+
+// A PML slab (ARD/pml_partition.h). Buffers are double-buffered by step parity:
+// p[par] is the current pressure, p[par^1] the previous one and receives the
+// next; phi[par] current, phi[par^1] next.
+struct PmlDesc {
+  int W, H, D;
+  int axis;  // axis along which zeta is nonzero (x = 0, y = 1, z = 2)
+  float* p[2];
+  float* phi[2][3];
+  float* f;
+  const float* zprof;  // zeta along that axis (length = W, H or D)
+  long long block0;    // first block of this slab in a grouped launch
+};
+
+// Pressure / force access to one partition as seen by the interface kernel.
+// kind 0: dense DCT fields (plain form); 1: PML (p double-buffered);
+// 2: DCT face shells (batched form).
+struct PartAccess {
+  int kind;
+  int W, H, D;
+  float* p[2];
+  float* f;
+  // batched shells: x-shell [6][D][H], y-shell [6][D][W], z-shell [6][H][W]
+  float* ps[3];
+  float* fs[3];
+};
+
+// One interface (ARD/boundary.cpp:21-173) with its sides resolved: side 0 is
+// the left/top/front partition, side 1 the right/bottom/back one.
+struct BndDesc {
+  int type;        // 0 X, 1 Y, 2 Z
+  int side[2];     // partition index of each side
+  int t0s, t0n;    // first tangential coordinate range (global): X: y, Y: x, Z: x
+  int t1s, t1n;    // second tangential range (global): X: z, Y: z, Z: y
+  int org[2][2];   // per side: origin of the two tangential axes (global)
+  float k[2];      // absorption * c0^2 / (180 h^2), per side (same value)
+  int self[2];     // include_self_terms_ per side
+  const uint8_t* mask;  // per element, bit m+3 set: write output m (last-writer rule); null = all
+  long long elem0; // first element of this boundary in a grouped launch
+};
+
+struct SourceDesc {
+  int part;
+  long long cell;   // flat cell index in the DCT partition
+  int kind;         // 0 Gaussian, 1 waveform
+  const float* wave;
+  int n;
+  float* fptr;      // where the forcing value is written (plain: F + cell)
+};
+
+struct RecDesc {
+  const float* p;   // plain: pressure field of the owner
+  long long cell;
+};
+
+}  // namespace ard

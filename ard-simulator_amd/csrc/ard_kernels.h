@@ -1,0 +1,18 @@
+// Launch helpers of the ARD kernels (ard_kernels.hip). Each returns 0 on
+// success, negative on a launch failure.
+#pragma once
+
+#include "ard_device.h"
+
+namespace ard {
+
+int launch_gemm(const GemmDesc* d_descs, int ndesc, int total_tiles, bool b_k_contig, bool c_m_contig, int ep,
+                hipStream_t s);
+int launch_pml(const PmlDesc* d_descs, int ndesc, long long total_blocks, int par, float dh, float dt, float c0,
+               hipStream_t s);
+int launch_interface(const BndDesc* d_descs, int ndesc, const PartAccess* d_parts, int par, long long total,
+                     hipStream_t s);
+int launch_sources(const SourceDesc* d_src, int n, const int* d_step, float courant, hipStream_t s);
+int launch_record(const RecDesc* d_rec, int n, float* trace, int cap, int* d_step, hipStream_t s);
+
+}  // namespace ard

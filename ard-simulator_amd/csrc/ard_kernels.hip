@@ -1,0 +1,387 @@
+// HIP kernels of the ARD time step for gfx950 (MI355X).
+//
+// Plain form (one partition per launch, the reference's structure):
+//   k_gemm         per-axis DCT-II / DCT-III passes as strided GEMMs against the
+//                  cosine matrix of the axis, LDS-tiled, 64x64 tiles per 256 threads;
+//                  the last forward pass fuses the modal update
+//                  (ARD/dct_volume.cpp:38-68, ARD/dct_partition.cpp:48-73).
+//   k_pml          Grote-Sim FDTD update of PML slabs (ARD/pml_partition.cpp:113-200).
+//   k_interface    6th-order interface forcing (ARD/boundary.cpp:21-173).
+//   k_sources      source forcing (ARD/partition.cpp:187-197, ARD/gaussian_source.cpp:16-20).
+//   k_record       recorder point samples (ARD/recorder.cpp:40-57) into a device trace.
+// All kernels accept a table of problems so that one launch can also cover
+// every partition of a phase (the batched form).
+#include "ard_kernels.h"
+
+namespace ard {
+
+// ---------------------------------------------------------------------------
+// Grouped strided GEMM: 64x64 output tile per 256-thread block, K in chunks
+// of 16 staged in LDS, 4x4 outputs per thread.
+// ---------------------------------------------------------------------------
+template <bool B_K_CONTIG, bool C_M_CONTIG, int EP>
+__global__ __launch_bounds__(256) void k_gemm(const GemmDesc* __restrict__ descs, int ndesc) {
+  __shared__ float As[16][68];
+  __shared__ float Bs[16][68];
+  const int tile = blockIdx.x;
+  int lo = 0, hi = ndesc - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (descs[mid].tile0 <= tile) lo = mid;
+    else hi = mid - 1;
+  }
+  const GemmDesc& g = descs[lo];
+  const int M = g.M, N = g.N, K = g.K, N1 = g.N1;
+  const int t = tile - g.tile0;
+  const int m0 = (t / g.tiles_n) * 64;
+  const int n0 = (t % g.tiles_n) * 64;
+  const int tid = threadIdx.x;
+
+  // B element offsets of the columns this thread loads (fixed over k)
+  long long boff[4];
+  int bn[4];
+  if (B_K_CONTIG) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int n = n0 + tid / 16 + 16 * r;
+      bn[r] = n;
+      boff[r] = n < N ? (long long)(n % N1) * g.b1 + (long long)(n / N1) * g.b2 : 0;
+    }
+  } else {
+    const int n = n0 + (tid & 63);
+    bn[0] = n;
+    boff[0] = n < N ? (long long)(n % N1) * g.b1 + (long long)(n / N1) * g.b2 : 0;
+  }
+  const int tm = C_M_CONTIG ? (tid & 15) : (tid >> 4);
+  const int tn = C_M_CONTIG ? (tid >> 4) : (tid & 15);
+
+  float acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = 0.f;
+
+  const float* __restrict__ A = g.A;
+  const float* __restrict__ B = g.B;
+  for (int k0 = 0; k0 < K; k0 += 16) {
+    // A tile: As[k][m] = A[m0+m][k0+k]
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int k = tid & 15, m = (tid >> 4) + 16 * r;
+      const int gm = m0 + m, gk = k0 + k;
+      As[k][m] = (gm < M && gk < K) ? A[(long long)gm * K + gk] : 0.f;
+    }
+    if (B_K_CONTIG) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int k = tid & 15, n = (tid >> 4) + 16 * r;
+        const int gk = k0 + k;
+        Bs[k][n] = (bn[r] < N && gk < K) ? B[(long long)gk * g.bK + boff[r]] : 0.f;
+      }
+    } else {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int k = (tid >> 6) + 4 * r, n = tid & 63;
+        const int gk = k0 + k;
+        Bs[k][n] = (bn[0] < N && gk < K) ? B[(long long)gk * g.bK + boff[0]] : 0.f;
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < 16; ++kk) {
+      const float4 a = *reinterpret_cast<const float4*>(&As[kk][4 * tm]);
+      const float4 b = *reinterpret_cast<const float4*>(&Bs[kk][4 * tn]);
+      const float av[4] = {a.x, a.y, a.z, a.w};
+      const float bv[4] = {b.x, b.y, b.z, b.w};
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = fmaf(av[i], bv[j], acc[i][j]);
+    }
+    __syncthreads();
+  }
+
+  long long coff[4];
+  bool nok[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int n = n0 + 4 * tn + j;
+    nok[j] = n < N;
+    coff[j] = nok[j] ? (long long)(n % N1) * g.c1 + (long long)(n / N1) * g.c2 : 0;
+  }
+  float* __restrict__ C = g.C;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int m = m0 + 4 * tm + i;
+    if (m >= M) continue;
+    const long long mo = (long long)m * g.cM;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      if (!nok[j]) continue;
+      const long long o = mo + coff[j];
+      const float v = g.alpha * acc[i][j];
+      if (EP == 0) {
+        C[o] = v;
+      } else {
+        C[o] = g.ep_a[o] * g.ep_cur[o] - g.ep_b * C[o] + g.ep_g[o] * v;
+      }
+    }
+  }
+}
+
+int launch_gemm(const GemmDesc* d_descs, int ndesc, int total_tiles, bool b_k_contig, bool c_m_contig, int ep,
+                hipStream_t s) {
+  if (total_tiles <= 0) return 0;
+  dim3 grid(total_tiles), block(256);
+  if (b_k_contig && c_m_contig && ep == 0)
+    hipLaunchKernelGGL((k_gemm<true, true, 0>), grid, block, 0, s, d_descs, ndesc);
+  else if (!b_k_contig && !c_m_contig && ep == 0)
+    hipLaunchKernelGGL((k_gemm<false, false, 0>), grid, block, 0, s, d_descs, ndesc);
+  else if (!b_k_contig && !c_m_contig && ep == 1)
+    hipLaunchKernelGGL((k_gemm<false, false, 1>), grid, block, 0, s, d_descs, ndesc);
+  else if (b_k_contig && !c_m_contig && ep == 0)
+    hipLaunchKernelGGL((k_gemm<true, false, 0>), grid, block, 0, s, d_descs, ndesc);
+  else if (!b_k_contig && c_m_contig && ep == 0)
+    hipLaunchKernelGGL((k_gemm<false, true, 0>), grid, block, 0, s, d_descs, ndesc);
+  else
+    return -1;
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+// ---------------------------------------------------------------------------
+// PML slab update, one thread per cell (ARD/pml_partition.cpp:113-200).
+// Out-of-slab neighbours read 0 (the reference's sentinel element, :6-15).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_pml(const PmlDesc* __restrict__ descs, int ndesc, int par, float dh,
+                                             float dt, float c0) {
+  const long long b = blockIdx.x;
+  int lo = 0, hi = ndesc - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (descs[mid].block0 <= b) lo = mid;
+    else hi = mid - 1;
+  }
+  const PmlDesc& P = descs[lo];
+  const int W = P.W, H = P.H, D = P.D;
+  const long long cell = (b - P.block0) * 256 + threadIdx.x;
+  if (cell >= (long long)W * H * D) return;
+  const int i = int(cell % W);
+  const int j = int((cell / W) % H);
+  const int k = int(cell / ((long long)W * H));
+  const float* __restrict__ p = P.p[par];
+  float* __restrict__ pn = P.p[par ^ 1];  // holds p_old on entry
+  const float* __restrict__ fx = P.phi[par][0];
+  const float* __restrict__ fy = P.phi[par][1];
+  const float* __restrict__ fz = P.phi[par][2];
+  const long long sx = 1, sy = W, sz = (long long)W * H;
+  auto at = [&](const float* a, int x, int y, int z) -> float {
+    return (x < 0 || x >= W || y < 0 || y >= H || z < 0 || z >= D) ? 0.f : a[z * sz + y * sy + x * sx];
+  };
+  const float c6[7] = {2.f, -27.f, 270.f, -490.f, 270.f, -27.f, 2.f};
+  const float c4[5] = {1.f, -8.f, 0.f, 8.f, -1.f};
+  float dxx = 0.f, dyy = 0.f, dzz = 0.f;
+#pragma unroll
+  for (int m = 0; m < 7; ++m) {
+    dxx += c6[m] * at(p, i + m - 3, j, k);
+    dyy += c6[m] * at(p, i, j + m - 3, k);
+    dzz += c6[m] * at(p, i, j, k + m - 3);
+  }
+  const float inv180 = 180.f * dh * dh;
+  dxx /= inv180;
+  dyy /= inv180;
+  dzz /= inv180;
+  const float z = P.zprof[P.axis == 0 ? i : (P.axis == 1 ? j : k)];
+  const float zx = P.axis == 0 ? z : 0.f, zy = P.axis == 1 ? z : 0.f, zz = P.axis == 2 ? z : 0.f;
+  const float pc = p[cell];
+  const float po = pn[cell];
+  const float term1 = 2.f * pc;
+  const float term2 = -po;
+  const float term3 = c0 * c0 * (dxx + dyy + dzz);
+  const float term4 = -(zx + zy + zz) * (pc - po) / dt;
+  const float term5 = -(zx * zy + zy * zz + zx * zz) * pc;
+  float dpx = 0.f, dpy = 0.f, dpz = 0.f, dux = 0.f, duy = 0.f, duz = 0.f;
+#pragma unroll
+  for (int m = 0; m < 5; ++m) {
+    if (m == 2) continue;  // zero coefficient
+    dpx += c4[m] * at(fx, i + m - 2, j, k);
+    dpy += c4[m] * at(fy, i, j + m - 2, k);
+    dpz += c4[m] * at(fz, i, j, k + m - 2);
+    dux += c4[m] * at(p, i + m - 2, j, k);
+    duy += c4[m] * at(p, i, j + m - 2, k);
+    duz += c4[m] * at(p, i, j, k + m - 2);
+  }
+  const float d12 = 12.f * dh;
+  dpx /= d12; dpy /= d12; dpz /= d12;
+  dux /= d12; duy /= d12; duz /= d12;
+  const float term6 = dpx + dpy + dpz;
+  float* __restrict__ f = P.f;
+  const float fc = f[cell];
+  pn[cell] = term1 + term2 + dt * dt * (term3 + term4 + term5 + term6 + fc);
+  const float phx = fx[cell], phy = fy[cell], phz = fz[cell];
+  P.phi[par ^ 1][0][cell] = phx - dt * zx * phx + dt * (zy + zz - zx) * dux;
+  P.phi[par ^ 1][1][cell] = phy - dt * zy * phy + dt * (zx + zz - zy) * duy;
+  P.phi[par ^ 1][2][cell] = phz - dt * zz * phz + dt * (zx + zy - zz) * duz;
+  if (fc != 0.f) f[cell] = 0.f;  // force cleared after the update (:199)
+}
+
+int launch_pml(const PmlDesc* d_descs, int ndesc, long long total_blocks, int par, float dh, float dt, float c0,
+               hipStream_t s) {
+  if (total_blocks <= 0) return 0;
+  hipLaunchKernelGGL(k_pml, dim3(unsigned(total_blocks)), dim3(256), 0, s, d_descs, ndesc, par, dh, dt, c0);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+// ---------------------------------------------------------------------------
+// Interface forcing (ARD/boundary.cpp:21-173). One thread per face element
+// computes the 6 outputs m = -3..2 from the 3 cells on each side.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ long long shell_index(const PartAccess& A, int x, int y, int z, int* which) {
+  // face-shell storage of the batched form: a cell belongs to the x-shell if
+  // it is within 3 of an x face, else to the y-shell, else to the z-shell.
+  const int W = A.W, H = A.H, D = A.D;
+  if (x < 3 || x >= W - 3) {
+    const int q = x < 3 ? x : x - W + 6;
+    *which = 0;
+    return ((long long)q * D + z) * H + y;
+  }
+  if (y < 3 || y >= H - 3) {
+    const int q = y < 3 ? y : y - H + 6;
+    *which = 1;
+    return ((long long)q * D + z) * W + x;
+  }
+  const int q = z < 3 ? z : z - D + 6;
+  *which = 2;
+  return ((long long)q * H + y) * W + x;
+}
+
+__device__ __forceinline__ float read_p(const PartAccess& A, int par, int x, int y, int z) {
+  if (A.kind == 2) {
+    int w;
+    const long long o = shell_index(A, x, y, z, &w);
+    return A.ps[w][o];
+  }
+  if (x < 0 || x >= A.W || y < 0 || y >= A.H || z < 0 || z >= A.D) return 0.f;
+  return A.p[par][((long long)z * A.H + y) * A.W + x];
+}
+
+__device__ __forceinline__ void write_f(const PartAccess& A, int x, int y, int z, float v) {
+  if (A.kind == 2) {
+    int w;
+    const long long o = shell_index(A, x, y, z, &w);
+    A.fs[w][o] = v;
+    return;
+  }
+  A.f[((long long)z * A.H + y) * A.W + x] = v;
+}
+
+__global__ __launch_bounds__(256) void k_interface(const BndDesc* __restrict__ descs, int ndesc,
+                                                   const PartAccess* __restrict__ parts, int par,
+                                                   long long total) {
+  const long long e = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (e >= total) return;
+  int lo = 0, hi = ndesc - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (descs[mid].elem0 <= e) lo = mid;
+    else hi = mid - 1;
+  }
+  const BndDesc& B = descs[lo];
+  const long long le = e - B.elem0;
+  const int u = B.t0s + int(le / B.t1n);  // outer tangential (reference loop order)
+  const int v = B.t1s + int(le % B.t1n);
+  const PartAccess& S0 = parts[B.side[0]];
+  const PartAccess& S1 = parts[B.side[1]];
+  const int u0 = u - B.org[0][0], v0 = v - B.org[0][1];
+  const int u1 = u - B.org[1][0], v1 = v - B.org[1][1];
+  const int n0 = B.type == 0 ? S0.W : (B.type == 1 ? S0.H : S0.D);
+  float pl[3], pr[3];
+#pragma unroll
+  for (int n = 0; n < 3; ++n) {
+    if (B.type == 0) {
+      pl[n] = read_p(S0, par, n0 - 3 + n, u0, v0);
+      pr[n] = read_p(S1, par, n, u1, v1);
+    } else if (B.type == 1) {
+      pl[n] = read_p(S0, par, u0, n0 - 3 + n, v0);
+      pr[n] = read_p(S1, par, u1, n, v1);
+    } else {
+      pl[n] = read_p(S0, par, u0, v0, n0 - 3 + n);
+      pr[n] = read_p(S1, par, u1, v1, n);
+    }
+  }
+  const float C[6][6] = {{0, 0, -2, 2, 0, 0},       {0, -2, 27, -27, 2, 0},  {-2, 27, -270, 270, -27, 2},
+                         {2, -27, 270, -270, 27, -2}, {0, 2, -27, 27, -2, 0}, {0, 0, 2, -2, 0, 0}};
+  const unsigned mask = B.mask ? B.mask[le] : 0x3fu;
+#pragma unroll
+  for (int r = 0; r < 6; ++r) {
+    if (!(mask & (1u << r))) continue;
+    const int m = r - 3;
+    float s1 = 0.f, s2 = 0.f;
+#pragma unroll
+    for (int n = 0; n < 3; ++n) s1 += C[r][n] * pl[n];
+#pragma unroll
+    for (int n = 0; n < 3; ++n) s2 += C[r][n + 3] * pr[n];
+    if (m < 0) {
+      const float sip = B.self[0] ? (s1 + s2) : s2;
+      const float val = sip * B.k[0];
+      if (B.type == 0) write_f(S0, n0 + m, u0, v0, val);
+      else if (B.type == 1) write_f(S0, u0, n0 + m, v0, val);
+      else write_f(S0, u0, v0, n0 + m, val);
+    } else {
+      const float sip = B.self[1] ? (s1 + s2) : s1;
+      const float val = sip * B.k[1];
+      if (B.type == 0) write_f(S1, m, u1, v1, val);
+      else if (B.type == 1) write_f(S1, u1, m, v1, val);
+      else write_f(S1, u1, v1, m, val);
+    }
+  }
+}
+
+int launch_interface(const BndDesc* d_descs, int ndesc, const PartAccess* d_parts, int par, long long total,
+                     hipStream_t s) {
+  if (total <= 0) return 0;
+  const long long blocks = (total + 255) / 256;
+  hipLaunchKernelGGL(k_interface, dim3(unsigned(blocks)), dim3(256), 0, s, d_descs, ndesc, d_parts, par, total);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+// ---------------------------------------------------------------------------
+// Sources and recorders. The step index lives in device memory (d_step) so
+// that a captured graph replays with the right source time.
+// ---------------------------------------------------------------------------
+__global__ void k_sources(const SourceDesc* __restrict__ s, int n, const int* __restrict__ d_step, float courant) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int t = *d_step;
+  float v;
+  if (s[i].kind == 0) {
+    // GaussianSource::SampleValue (ARD/gaussian_source.cpp:16-20), FP32
+    const float x = 3.14159265358979f * ((2.f * courant * float(t)) / 6.f - 2.f);
+    v = 1e9f * expf(-(x * x));
+  } else {
+    v = t < s[i].n ? s[i].wave[t] : 0.f;
+  }
+  *s[i].fptr = v;
+}
+
+int launch_sources(const SourceDesc* d_src, int n, const int* d_step, float courant, hipStream_t s) {
+  if (n <= 0) return 0;
+  hipLaunchKernelGGL(k_sources, dim3((n + 63) / 64), dim3(64), 0, s, d_src, n, d_step, courant);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+__global__ void k_record(const RecDesc* __restrict__ r, int n, float* __restrict__ trace, int cap,
+                         int* __restrict__ d_step) {
+  const int t = *d_step;
+  __syncthreads();
+  for (int i = threadIdx.x; i < n; i += blockDim.x)
+    if (t < cap) trace[(long long)t * n + i] = r[i].p[r[i].cell];
+  __syncthreads();
+  if (threadIdx.x == 0) *d_step = t + 1;
+}
+
+int launch_record(const RecDesc* d_rec, int n, float* trace, int cap, int* d_step, hipStream_t s) {
+  hipLaunchKernelGGL(k_record, dim3(1), dim3(256), 0, s, d_rec, n, trace, cap, d_step);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+}  // namespace ard

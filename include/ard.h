@@ -1,0 +1,136 @@
+/*
+ * ard.h -- C-ABI of the MI355X ARD time-stepping library (libard.so).
+ *
+ * The reference (pjx3/ARD-simulator) exposes this path as C++ classes, not as
+ * an FFI. Each entry point below names the reference interface it replaces
+ * (paths relative to the reference root, ARD = ARD-simulator-190113/). A
+ * context holds one scene: DCT partitions, PML partitions, boundaries,
+ * sources and recorders, all device-resident. Plain pointers and sizes only.
+ *
+ * Errors: every function returns ARD_OK (0) or a negative ARD_ERR_* code;
+ * ard_last_error() gives a thread-local message. The reference asserts or
+ * crashes instead (ARD/dct_volume.cpp:23,25; ARD/recorder.cpp:50).
+ *
+ * Threading: a context is externally synchronised (one host thread at a
+ * time). Host pointers are borrowed for the duration of the call.
+ *
+ * Coordinates are integer cells. Volumes are z-major with x fastest,
+ * idx = z*H*W + y*W + x (ARD/dct_volume.cpp:70-88).
+ */
+#ifndef ARD_H_
+#define ARD_H_
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ARD_OK 0
+#define ARD_ERR_INVALID (-1)     /* bad argument */
+#define ARD_ERR_STATE (-2)       /* call not allowed in the context's state */
+#define ARD_ERR_HIP (-3)         /* HIP runtime failure (message has the HIP error) */
+#define ARD_ERR_RANGE (-4)       /* coordinate or index out of range */
+#define ARD_ERR_UNSUPPORTED (-5) /* valid input this form cannot run */
+
+typedef struct ard_ctx ard_ctx;
+
+/* Stepping forms. */
+#define ARD_FORM_PLAIN 0   /* per-partition launches, dense fields: the reference's structure */
+#define ARD_FORM_BATCHED 1 /* all partitions per phase in one launch, face-shell transforms, hipGraph */
+
+/* Recorder addressing. REFERENCE: the recorder's global cell coordinates
+ * index the owning partition's array (ARD/recorder.cpp:44-55 passes global
+ * coordinates to get_pressure, ARD/dct_volume.cpp:70-73 does not subtract the
+ * origin). LOCAL: the cell at those global coordinates. */
+#define ARD_REC_REFERENCE 0
+#define ARD_REC_LOCAL 1
+
+/* PML face types (ARD/pml_partition.h:39-46) and boundary types (ARD/boundary.h:30-34). */
+#define ARD_P_LEFT 0
+#define ARD_P_RIGHT 1
+#define ARD_P_TOP 2
+#define ARD_P_BOTTOM 3
+#define ARD_P_FRONT 4
+#define ARD_P_BACK 5
+#define ARD_X_BOUNDARY 0
+#define ARD_Y_BOUNDARY 1
+#define ARD_Z_BOUNDARY 2
+
+typedef struct ard_params {
+  float h;          /* Simulation::m_dh (ARD/simulation.h:43) */
+  float dt;         /* Simulation::m_dt */
+  float c0;         /* Simulation::m_c0 */
+  int pml_layers;   /* Simulation::m_pml_layers */
+  float absorption; /* Partition::m_absorption (ARD/partition.h:45) */
+  int form;         /* ARD_FORM_* */
+  int recorder_mode;/* ARD_REC_* */
+  int reserved[9];  /* zero */
+} ard_params;
+
+/* Replaces the statics set in ARD/main.cpp:29-45 plus the GPU set-up of
+ * initVkGPU (ARD/utils_VkFFT.cpp:564-610). */
+int ard_ctx_create(int device, const ard_params* params, ard_ctx** out);
+int ard_ctx_destroy(ard_ctx* ctx);
+const char* ard_last_error(void);
+
+/* Per-face scale of the PML damping profile zeta (ARD/pml_partition.cpp:61-92
+ * has scale 1). Before ard_finalize. */
+int ard_set_pml_scale(ard_ctx* ctx, const float scale[6]);
+
+/* DctPartition(xs,ys,zs,w,h,d) (ARD/dct_partition.cpp:7-37). Returns the id (>= 0). */
+int ard_add_dct_partition(ard_ctx* ctx, int x0, int y0, int z0, int w, int h, int d);
+/* PmlPartition(neighbor, type, xs,ys,zs,w,h,d) (ARD/pml_partition.cpp:17-93). Returns the id. */
+int ard_add_pml_partition(ard_ctx* ctx, int neighbor, int type, int x0, int y0, int z0, int w, int h, int d);
+/* Boundary(type, absorption, a, b, xs,xe,ys,ye,zs,ze) (ARD/boundary.cpp:7-15). Registration
+ * order is write order: a cell forced by two boundaries keeps the later one's value. */
+int ard_add_boundary(ard_ctx* ctx, int type, int a, int b, int xs, int xe, int ys, int ye, int zs, int ze,
+                     float absorption);
+/* The topology part of Simulation::Simulation (ARD/simulation.cpp:19-266): boundaries between
+ * the DCT partitions added so far (Boundary::FindBoundary, ARD/boundary.cpp:289-334), border
+ * masks (Partition::AddBoundary, ARD/partition.cpp:96-143) and the PML slabs with their boundaries. */
+int ard_build_topology(ard_ctx* ctx);
+
+/* A source at global cell (x,y,z). wave == NULL: GaussianSource (ARD/gaussian_source.cpp:16-20);
+ * otherwise sample t of the step t is wave[t] (0 past n). Assigned like ARD/simulation.cpp:37-48. */
+int ard_add_source(ard_ctx* ctx, int x, int y, int z, const float* wave, int n);
+/* Recorder at global cell (x,y,z); owner found as Recorder::FindPartition (ARD/recorder.cpp:26-38). */
+int ard_add_recorder(ard_ctx* ctx, int x, int y, int z);
+
+/* Allocates device state and tables. After it, topology calls fail with ARD_ERR_STATE. */
+int ard_finalize(ard_ctx* ctx);
+
+/* Topology queries. part_info[9] = is_pml, pml_type, parent, x0,y0,z0, w,h,d.
+ * bnd_info[9] = type, a, b, xs,xe,ys,ye,zs,ze. */
+int ard_num_partitions(ard_ctx* ctx);
+int ard_num_boundaries(ard_ctx* ctx);
+int ard_num_recorders(ard_ctx* ctx);
+int ard_partition_info(ard_ctx* ctx, int part, int* info9);
+int ard_boundary_info(ard_ctx* ctx, int bnd, int* info9);
+int ard_recorder_partition(ard_ctx* ctx, int rec);
+
+/* n calls of Simulation::Update (ARD/simulation.cpp:300-317) followed by the recorder point
+ * sample of Recorder::RecordField (ARD/recorder.cpp:40-57), all on the device. Asynchronous
+ * with respect to the host. */
+int ard_step(ard_ctx* ctx, int n);
+/* Simulation::time_step_ (ARD/simulation.h:48): number of steps taken. */
+int ard_time_step(ard_ctx* ctx);
+/* Recorder samples of steps [t0, t0+nt) into out[nt][num_recorders] (the response_<id>.txt
+ * column of ARD/recorder.cpp:55). Synchronises. */
+int ard_fetch_recorders(ard_ctx* ctx, float* out, int t0, int nt);
+
+/* Partition::get_pressure_field (ARD/partition.h:52): the whole pressure volume, w*h*d floats.
+ * Synchronises. The batched form evaluates DCT partitions from their modes on demand. */
+int ard_copy_pressure_field(ard_ctx* ctx, int part, float* out);
+/* Partition::get_pressure(x,y,z) (ARD/partition.h:56), partition-local cell. */
+int ard_get_pressure(ard_ctx* ctx, int part, int x, int y, int z, float* out);
+/* Partition::set_force(x,y,z,f) (ARD/partition.h:57), partition-local cell. DCT forces
+ * persist across steps; PML forces are cleared by the next update (ARD/pml_partition.cpp:199). */
+int ard_set_force(ard_ctx* ctx, int part, int x, int y, int z, float f);
+
+/* Blocks until all work queued on the context's stream is done. */
+int ard_synchronize(ard_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ARD_H_ */

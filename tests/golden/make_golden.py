@@ -1,0 +1,40 @@
+"""Writes the golden recorder traces and topology fixtures used by the parity tests.
+
+The traces come from the CPU oracle (oracle/ard_oracle.cpp) in FP32 (the
+reference's arithmetic) and FP64; the oracle is pinned by
+tests/test_oracle_pins.py. Run from the repository root:
+
+    python tests/golden/make_golden.py
+"""
+import json
+import os
+import sys
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path[:0] = [os.path.join(ROOT, "oracle"), os.path.join(ROOT, "ard-simulator_amd")]
+import ard_oracle as O  # noqa: E402
+import ard_scenes as S  # noqa: E402
+
+OUT = os.path.dirname(os.path.abspath(__file__))
+
+
+def main():
+    for sc in (S.two_rooms(), S.scene3()):
+        topo = None
+        for double in (False, True):
+            o = O.from_scene(sc, double=double)
+            if topo is None:
+                topo = {"partitions": o.partitions(), "boundaries": o.boundaries(),
+                        "recorders": sc.recorders, "recorder_partitions": [o.recorder_partition(i) for i in range(len(sc.recorders))]}
+            tr = o.step(sc.steps)
+            name = "%s_%s.npy" % (sc.name, "fp64" if double else "fp32")
+            np.save(os.path.join(OUT, name), tr.astype(np.float64))
+            print(name, tr.shape, float(np.abs(tr).max()), flush=True)
+        with open(os.path.join(OUT, "%s_topology.json" % sc.name), "w") as f:
+            json.dump(topo, f)
+
+
+if __name__ == "__main__":
+    main()
