@@ -1,0 +1,186 @@
+"""Python mirror of the reference's host API over libard.so (ctypes).
+
+The reference drives a scene with ``Simulation(partitions, sources)``,
+``Simulation::Update()`` and ``Recorder::RecordField`` (ARD/main.cpp:62-150).
+:class:`Simulation` below keeps those names and meanings on top of the C-ABI
+declared in include/ard.h; every call goes to the HIP library. There is no CPU
+fallback: constructing a GPU simulation without libard.so built raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libard.so")
+
+ARD_OK = 0
+FORM_PLAIN, FORM_BATCHED = 0, 1
+REC_REFERENCE, REC_LOCAL = 0, 1
+
+EXPORTED = [
+    "ard_ctx_create", "ard_ctx_destroy", "ard_last_error", "ard_set_pml_scale", "ard_add_dct_partition",
+    "ard_add_pml_partition", "ard_add_boundary", "ard_build_topology", "ard_add_source", "ard_add_recorder",
+    "ard_finalize", "ard_num_partitions", "ard_num_boundaries", "ard_num_recorders", "ard_partition_info",
+    "ard_boundary_info", "ard_recorder_partition", "ard_step", "ard_time_step", "ard_fetch_recorders",
+    "ard_copy_pressure_field", "ard_get_pressure", "ard_set_force", "ard_synchronize",
+]
+
+
+class ArdError(RuntimeError):
+    pass
+
+
+class Params(ctypes.Structure):
+    _fields_ = [
+        ("h", ctypes.c_float), ("dt", ctypes.c_float), ("c0", ctypes.c_float), ("pml_layers", ctypes.c_int),
+        ("absorption", ctypes.c_float), ("form", ctypes.c_int), ("recorder_mode", ctypes.c_int),
+        ("reserved", ctypes.c_int * 9),
+    ]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ArdError("libard.so is not built (run __graft_entry__.build())")
+        L = ctypes.CDLL(LIB_PATH)
+        vp, i, f = ctypes.c_void_p, ctypes.c_int, ctypes.c_float
+        fp = ctypes.POINTER(ctypes.c_float)
+        ip = ctypes.POINTER(ctypes.c_int)
+        L.ard_ctx_create.argtypes = [i, ctypes.POINTER(Params), ctypes.POINTER(vp)]
+        L.ard_ctx_destroy.argtypes = [vp]
+        L.ard_last_error.restype = ctypes.c_char_p
+        L.ard_set_pml_scale.argtypes = [vp, fp]
+        L.ard_add_dct_partition.argtypes = [vp, i, i, i, i, i, i]
+        L.ard_add_pml_partition.argtypes = [vp, i, i, i, i, i, i, i, i]
+        L.ard_add_boundary.argtypes = [vp, i, i, i, i, i, i, i, i, i, f]
+        L.ard_build_topology.argtypes = [vp]
+        L.ard_add_source.argtypes = [vp, i, i, i, fp, i]
+        L.ard_add_recorder.argtypes = [vp, i, i, i]
+        L.ard_finalize.argtypes = [vp]
+        for n in ("ard_num_partitions", "ard_num_boundaries", "ard_num_recorders", "ard_time_step",
+                  "ard_synchronize"):
+            getattr(L, n).argtypes = [vp]
+        L.ard_partition_info.argtypes = [vp, i, ip]
+        L.ard_boundary_info.argtypes = [vp, i, ip]
+        L.ard_recorder_partition.argtypes = [vp, i]
+        L.ard_step.argtypes = [vp, i]
+        L.ard_fetch_recorders.argtypes = [vp, fp, i, i]
+        L.ard_copy_pressure_field.argtypes = [vp, i, fp]
+        L.ard_get_pressure.argtypes = [vp, i, i, i, i, fp]
+        L.ard_set_force.argtypes = [vp, i, i, i, i, f]
+        _lib = L
+    return _lib
+
+
+def _check(r):
+    if r < 0:
+        raise ArdError("libard error %d: %s" % (r, lib().ard_last_error().decode()))
+    return r
+
+
+def _fp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+class Simulation:
+    """A scene on one MI355X: the reference's Simulation with Update() on the GPU.
+
+    ``scene`` is an ard_scenes.Scene. ``device=-1`` builds a host-only context
+    whose topology can be inspected without a GPU.
+    """
+
+    def __init__(self, scene, form=FORM_PLAIN, device=0, pml_scale=None, finalize=True):
+        self.L = lib()
+        self.scene = scene
+        self.form = form
+        p = Params(scene.h, scene.dt, scene.c0, scene.pml_layers, scene.absorption, form, scene.recorder_mode)
+        h = ctypes.c_void_p()
+        _check(self.L.ard_ctx_create(device, ctypes.byref(p), ctypes.byref(h)))
+        self.ctx = h
+        self._keep = []
+        if pml_scale is not None:
+            a = np.ascontiguousarray(pml_scale, dtype=np.float32)
+            _check(self.L.ard_set_pml_scale(self.ctx, _fp(a)))
+        for b in scene.boxes:
+            _check(self.L.ard_add_dct_partition(self.ctx, *b))
+        _check(self.L.ard_build_topology(self.ctx))
+        for s in scene.sources:
+            if s.wave is None:
+                _check(self.L.ard_add_source(self.ctx, s.x, s.y, s.z, None, 0))
+            else:
+                w = np.ascontiguousarray(s.wave, dtype=np.float32)
+                self._keep.append(w)
+                _check(self.L.ard_add_source(self.ctx, s.x, s.y, s.z, _fp(w), len(w)))
+        for r in scene.recorders:
+            _check(self.L.ard_add_recorder(self.ctx, *r))
+        self.nrec = len(scene.recorders)
+        if finalize:
+            _check(self.L.ard_finalize(self.ctx))
+
+    def close(self):
+        if getattr(self, "ctx", None):
+            self.L.ard_ctx_destroy(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        self.close()
+
+    # --- topology -------------------------------------------------------
+    def partitions(self):
+        out, buf = [], (ctypes.c_int * 9)()
+        for i in range(_check(self.L.ard_num_partitions(self.ctx))):
+            _check(self.L.ard_partition_info(self.ctx, i, buf))
+            out.append(tuple(buf))
+        return out
+
+    def boundaries(self):
+        out, buf = [], (ctypes.c_int * 9)()
+        for i in range(_check(self.L.ard_num_boundaries(self.ctx))):
+            _check(self.L.ard_boundary_info(self.ctx, i, buf))
+            out.append(tuple(buf))
+        return out
+
+    def recorder_partition(self, r):
+        return _check(self.L.ard_recorder_partition(self.ctx, r))
+
+    # --- stepping ---------------------------------------------------------
+    def update(self, n=1):
+        """n calls of Simulation::Update (asynchronous)."""
+        _check(self.L.ard_step(self.ctx, n))
+
+    @property
+    def time_step(self):
+        return _check(self.L.ard_time_step(self.ctx))
+
+    def synchronize(self):
+        _check(self.L.ard_synchronize(self.ctx))
+
+    def recorders(self, t0=0, nt=None):
+        """Recorder samples [nt][num_recorders] of steps t0..t0+nt-1."""
+        if nt is None:
+            nt = self.time_step - t0
+        out = np.zeros((nt, max(self.nrec, 1)), dtype=np.float32)
+        _check(self.L.ard_fetch_recorders(self.ctx, _fp(out), t0, nt))
+        return out[:, : self.nrec]
+
+    def pressure_field(self, part):
+        info = self.partitions()[part]
+        w, h, d = info[6:9]
+        out = np.zeros((d, h, w), dtype=np.float32)
+        _check(self.L.ard_copy_pressure_field(self.ctx, part, _fp(out)))
+        return out
+
+    def get_pressure(self, part, x, y, z):
+        v = ctypes.c_float()
+        _check(self.L.ard_get_pressure(self.ctx, part, x, y, z, ctypes.byref(v)))
+        return v.value
+
+    def set_force(self, part, x, y, z, f):
+        _check(self.L.ard_set_force(self.ctx, part, x, y, z, f))

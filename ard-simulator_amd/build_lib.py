@@ -1,0 +1,54 @@
+"""Builds libard.so (HIP kernels for gfx950 + the C-ABI) in this directory.
+
+Usage: python build_lib.py   (called by __graft_entry__.build()).
+"""
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+OUT = os.path.join(HERE, "libard.so")
+OBJ = os.path.join(HERE, "_build")
+
+HIP_SOURCES = ["ard_kernels.hip", "ard_batched.hip", "ard_ctx.cpp", "ard_plain.cpp"]
+CXX_SOURCES = ["ard_topology.cpp"]
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = "gfx950"
+# Packed FP32 instructions are disabled for all device code.
+COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-parameter", "-Wno-unused-variable"]
+DEVICE = ["--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+          "-munsafe-fp-atomics"]
+
+
+def run(cmd):
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    out = "\n".join(l for l in r.stdout.splitlines() if "packed-fp32-ops" not in l and l.strip())
+    if r.returncode != 0:
+        sys.stderr.write(out + "\n")
+        raise SystemExit("build failed: " + " ".join(cmd))
+    return out
+
+
+def build(verbose=False):
+    os.makedirs(OBJ, exist_ok=True)
+    objs = []
+    for src in HIP_SOURCES:
+        obj = os.path.join(OBJ, src + ".o")
+        run([HIPCC, "-x", "hip", "-c", os.path.join(CSRC, src), "-o", obj] + COMMON + DEVICE)
+        objs.append(obj)
+    for src in CXX_SOURCES:
+        obj = os.path.join(OBJ, src + ".o")
+        run(["g++", "-c", os.path.join(CSRC, src), "-o", obj] + COMMON)
+        objs.append(obj)
+    tmp = OUT + ".tmp"
+    run([HIPCC, "-shared", "-o", tmp] + objs + ["--offload-arch=" + ARCH, "-fPIC"])
+    os.replace(tmp, OUT)
+    if verbose:
+        print("built", OUT)
+    return OUT
+
+
+if __name__ == "__main__":
+    build(verbose=True)

@@ -136,21 +136,26 @@ int ard_ctx_create(int device, const ard_params* params, ard_ctx** out) {
     return fail(ARD_ERR_INVALID, "unknown form");
   if (params->recorder_mode != ARD_REC_REFERENCE && params->recorder_mode != ARD_REC_LOCAL)
     return fail(ARD_ERR_INVALID, "unknown recorder mode");
-  int ndev = 0;
-  hipError_t e = hipGetDeviceCount(&ndev);
-  if (e != hipSuccess) return hip_fail(e, "hipGetDeviceCount");
-  if (device < 0 || device >= ndev) return fail(ARD_ERR_INVALID, "no such device");
-  e = hipSetDevice(device);
-  if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+  // device -1: a host-only context (topology queries only; ard_finalize fails)
+  if (device != -1) {
+    int ndev = 0;
+    hipError_t e = hipGetDeviceCount(&ndev);
+    if (e != hipSuccess) return hip_fail(e, "hipGetDeviceCount");
+    if (device < 0 || device >= ndev) return fail(ARD_ERR_INVALID, "no such device");
+    e = hipSetDevice(device);
+    if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+  }
   auto* c = new Ctx();
   c->device = device;
   c->prm = *params;
   c->topo.pml_layers = params->pml_layers;
   c->topo.absorption = params->absorption;
-  e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
-  if (e != hipSuccess) {
-    delete c;
-    return hip_fail(e, "hipStreamCreate");
+  if (device != -1) {
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+      delete c;
+      return hip_fail(e, "hipStreamCreate");
+    }
   }
   *out = reinterpret_cast<ard_ctx*>(c);
   return ARD_OK;
@@ -159,8 +164,10 @@ int ard_ctx_create(int device, const ard_params* params, ard_ctx** out) {
 int ard_ctx_destroy(ard_ctx* ctx) {
   if (!ctx) return fail(ARD_ERR_INVALID, "null context");
   Ctx* c = reinterpret_cast<Ctx*>(ctx);
-  hipSetDevice(c->device);
-  if (c->stream) hipStreamSynchronize(c->stream);
+  if (c->device >= 0) {
+    hipSetDevice(c->device);
+    if (c->stream) hipStreamSynchronize(c->stream);
+  }
   delete c;
   return ARD_OK;
 }
@@ -245,6 +252,7 @@ int ard_finalize(ard_ctx* ctx) {
   if (c->finalized) return fail(ARD_ERR_STATE, "already finalized");
   std::string err = c->topo.resolve(c->prm.recorder_mode == ARD_REC_LOCAL);
   if (!err.empty()) return fail(ARD_ERR_RANGE, err);
+  if (c->device < 0) return fail(ARD_ERR_STATE, "host-only context (device -1) cannot be finalized");
   hipSetDevice(c->device);
   int r = c->prm.form == ARD_FORM_PLAIN ? finalize_plain(*c) : finalize_batched(*c);
   if (r != ARD_OK) return r;

@@ -1,0 +1,113 @@
+// Internal state of an ARD context (the object behind the C-ABI handle).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../../include/ard.h"
+#include "ard_device.h"
+#include "ard_topology.h"
+
+namespace ard {
+
+struct BatchedState;  // ard_batched.hip
+
+// Dense fields of one DCT partition in the plain form (ARD/dct_partition.h,
+// ARD/dct_volume.h): force F and pressure p in the value domain, modes M[2]
+// (current / previous by step parity, stored multiplied by the inverse
+// transform scale), modal coefficients a and g.
+struct DctPlain {
+  float* F = nullptr;
+  float* p = nullptr;
+  float* M[2] = {nullptr, nullptr};
+  float* a = nullptr;
+  float* g = nullptr;
+  int gemm0 = 0;  // first of its 12 GEMM problems (2 parities x 6 passes)
+  int tiles[6] = {0, 0, 0, 0, 0, 0};
+};
+
+// PML slab buffers (ARD/pml_partition.h:8-25); shared by both forms.
+struct PmlBuffers {
+  float* p[2] = {nullptr, nullptr};
+  float* phi[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
+  float* f = nullptr;
+  float* zprof = nullptr;
+  int desc = -1;  // index into the PML descriptor table
+};
+
+struct Ctx {
+  int device = -1;
+  hipStream_t stream = nullptr;
+  ard_params prm{};
+  float pml_scale[6] = {1.f, 1.f, 1.f, 1.f, 1.f, 1.f};
+  Topology topo;
+  bool finalized = false;
+  bool topology_built = false;
+  int time_step = 0;
+  float courant = 0.f;  // c0 * dt / h in FP32, as ARD/gaussian_source.cpp:18 forms it
+
+  std::vector<void*> allocs;
+  std::map<int, std::pair<float*, float*>> tables;  // axis length -> (REDFT10, REDFT01) matrices
+
+  // step counter and recorder trace, device-resident so captured graphs replay correctly
+  int* d_step = nullptr;
+  float* d_trace = nullptr;
+  int trace_cap = 0;
+
+  // plain form
+  std::vector<DctPlain> dct;       // by partition id
+  std::vector<PmlBuffers> pml;     // by partition id
+  float* tmp[2] = {nullptr, nullptr};
+  std::vector<GemmDesc> h_gemm;
+  GemmDesc* d_gemm = nullptr;
+  std::vector<PmlDesc> h_pml;      // block0 = 0 each (one launch per slab)
+  PmlDesc* d_pml = nullptr;
+  std::vector<long long> pml_blocks;
+  std::vector<BndDesc> h_bnd;      // elem0 = 0 each (one launch per boundary)
+  BndDesc* d_bnd = nullptr;
+  std::vector<long long> bnd_elems;
+  std::vector<PartAccess> h_parts;
+  PartAccess* d_parts = nullptr;
+  SourceDesc* d_src = nullptr;
+  int nsrc = 0;
+  RecDesc* d_rec = nullptr;
+
+  // batched form
+  BatchedState* bs = nullptr;
+  hipGraphExec_t graph_exec[2] = {nullptr, nullptr};
+
+  ~Ctx();
+  int dalloc(void** out, size_t bytes);
+  int dzero(void* p, size_t bytes);
+  int upload(void* dst, const void* src, size_t bytes);
+  int cos_tables(int n, const float** fwd, const float** inv);
+};
+
+int hip_fail(hipError_t e, const char* what);
+int set_error(int code, const std::string& msg);
+void modal_tables(int W, int H, int D, double h, double dt, double c0, std::vector<float>& a, std::vector<float>& g);
+
+// shared set-up (ard_plain.cpp)
+int setup_common(Ctx& c);               // step counter, PML slabs, sources/recorders tables
+int build_boundary_descs(Ctx& c, std::vector<BndDesc>& out, std::vector<long long>& elems);
+int ensure_trace(Ctx& c, int steps_needed);
+
+// plain form (ard_plain.cpp)
+int finalize_plain(Ctx& c);
+int step_plain(Ctx& c, int n);
+
+// batched form (ard_batched.hip)
+int finalize_batched(Ctx& c);
+int step_batched(Ctx& c, int n);
+int batched_pressure_field(Ctx& c, int part, const float** out);
+int batched_set_force(Ctx& c, int part, int x, int y, int z, float f);
+
+// field access (either form)
+int pressure_field_ptr(Ctx& c, int part, const float** out);
+int set_force_cell(Ctx& c, int part, int x, int y, int z, float f);
+
+}  // namespace ard

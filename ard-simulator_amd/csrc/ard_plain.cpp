@@ -1,0 +1,357 @@
+// Plain form of the ARD step: the reference's structure (one set of launches
+// per partition and per boundary, dense pressure and force volumes) on the
+// device. Also the set-up shared with the batched form (PML slabs, sources,
+// recorders, boundary descriptors).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <string>
+#include <vector>
+
+#include "ard_ctx_impl.h"
+#include "ard_kernels.h"
+
+namespace ard {
+
+namespace {
+long long ceil_div(long long a, long long b) { return (a + b - 1) / b; }
+
+// zeta along the normal of a PML slab (ARD/pml_partition.cpp:28-92): the
+// profile zeta0 (x/L - sin(2 pi x/L)/(2 pi)), x = q h, growing outward, with
+// zeta0 = c0/L log10(1/R), R = 1e-5, L = layers*h.
+std::vector<float> pml_profile(const PartGeom& p, int layers, double h, double c0, double scale) {
+  const int axis = p.pml_type / 2;
+  const int n = axis == 0 ? p.w : (axis == 1 ? p.h : p.d);
+  const double L = layers * h;
+  const double z0 = c0 / L * std::log10(1.0 / double(0.00001f));
+  const double pi = 3.14159265358979323846;
+  std::vector<float> prof(n);
+  const bool grows = (p.pml_type == P_RIGHT || p.pml_type == P_BOTTOM || p.pml_type == P_BACK);
+  for (int i = 0; i < n; ++i) {
+    const int q = grows ? i + 1 : n - i;
+    const double x = q * h;
+    prof[i] = float(scale * z0 * (x / L - std::sin(2.0 * pi * x / L) / (2.0 * pi)));
+  }
+  return prof;
+}
+}  // namespace
+
+int ensure_trace(Ctx& c, int steps_needed) {
+  const int nr = int(c.topo.recorders.size());
+  if (steps_needed <= c.trace_cap) return ARD_OK;
+  const int cap = std::max({steps_needed, 2 * c.trace_cap, 1024});
+  float* nt = nullptr;
+  int r = c.dalloc((void**)&nt, size_t(cap) * std::max(nr, 1) * 4);
+  if (r) return r;
+  if (c.d_trace && nr > 0) {
+    hipError_t e = hipMemcpyAsync(nt, c.d_trace, size_t(c.trace_cap) * nr * 4, hipMemcpyDeviceToDevice, c.stream);
+    if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync D2D");
+    e = hipStreamSynchronize(c.stream);
+    if (e != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
+    hipFree(c.d_trace);
+    c.allocs.erase(std::find(c.allocs.begin(), c.allocs.end(), (void*)c.d_trace));
+  }
+  c.d_trace = nt;
+  c.trace_cap = cap;
+  return ARD_OK;
+}
+
+// Resolves each boundary's sides the way Boundary::ComputeForcingTerms does
+// (ARD/boundary.cpp:30-34, 78-82, 126-130) and lays out its face elements in
+// the reference's loop order.
+int build_boundary_descs(Ctx& c, std::vector<BndDesc>& out, std::vector<long long>& elems) {
+  const float h = c.prm.h, c0 = c.prm.c0;
+  out.clear();
+  elems.clear();
+  for (const BoundaryGeom& b : c.topo.bnds) {
+    const PartGeom& A = c.topo.parts[b.a];
+    BndDesc d{};
+    d.type = b.type;
+    bool a_first;
+    if (b.type == X_BOUNDARY) a_first = (b.xs <= A.x1() && b.xe >= A.x1());
+    else if (b.type == Y_BOUNDARY) a_first = (b.ys <= A.y1() && b.ye >= A.y1());
+    else a_first = (b.zs <= A.z1() && b.ze >= A.z1());
+    d.side[0] = a_first ? b.a : b.b;
+    d.side[1] = a_first ? b.b : b.a;
+    if (b.type == X_BOUNDARY) {
+      d.t0s = b.ys; d.t0n = b.ye - b.ys; d.t1s = b.zs; d.t1n = b.ze - b.zs;
+    } else if (b.type == Y_BOUNDARY) {
+      d.t0s = b.xs; d.t0n = b.xe - b.xs; d.t1s = b.zs; d.t1n = b.ze - b.zs;
+    } else {
+      d.t0s = b.xs; d.t0n = b.xe - b.xs; d.t1s = b.ys; d.t1n = b.ye - b.ys;
+    }
+    if (d.t0n <= 0 || d.t1n <= 0) return set_error(ARD_ERR_INVALID, "empty boundary");
+    for (int s = 0; s < 2; ++s) {
+      const PartGeom& P = c.topo.parts[d.side[s]];
+      if (b.type == X_BOUNDARY) { d.org[s][0] = P.y0; d.org[s][1] = P.z0; }
+      else if (b.type == Y_BOUNDARY) { d.org[s][0] = P.x0; d.org[s][1] = P.z0; }
+      else { d.org[s][0] = P.x0; d.org[s][1] = P.y0; }
+      // fi = sip * c0^2 / (180 h^2), scaled by the boundary's absorption (ARD/boundary.cpp:64-65)
+      d.k[s] = b.absorption * (c0 * c0 / (180.0f * h * h));
+      d.self[s] = P.is_pml ? 0 : 1;  // include_self_terms_ (ARD/partition.h:38, ARD/pml_partition.cpp:20)
+      // tangential ranges must lie inside each side
+      const int n0 = b.type == X_BOUNDARY ? P.h : P.w;
+      const int n1 = b.type == Z_BOUNDARY ? P.h : P.d;
+      if (d.t0s - d.org[s][0] < 0 || d.t0s + d.t0n - d.org[s][0] > n0 || d.t1s - d.org[s][1] < 0 ||
+          d.t1s + d.t1n - d.org[s][1] > n1)
+        return set_error(ARD_ERR_RANGE, "boundary face extends past a partition");
+      const int nn = b.type == X_BOUNDARY ? P.w : (b.type == Y_BOUNDARY ? P.h : P.d);
+      if (nn < 3) return set_error(ARD_ERR_RANGE, "partition thinner than the 3-cell interface stencil");
+    }
+    d.mask = nullptr;
+    d.elem0 = 0;
+    out.push_back(d);
+    elems.push_back((long long)d.t0n * d.t1n);
+  }
+  return ARD_OK;
+}
+
+int setup_common(Ctx& c) {
+  int r;
+  if ((r = c.dalloc((void**)&c.d_step, sizeof(int)))) return r;
+  if ((r = c.dzero(c.d_step, sizeof(int)))) return r;
+  c.courant = c.prm.c0 * c.prm.dt / c.prm.h;
+  // PML slabs
+  const int np = int(c.topo.parts.size());
+  c.pml.assign(np, PmlBuffers());
+  c.h_pml.clear();
+  c.pml_blocks.clear();
+  for (int i = 0; i < np; ++i) {
+    const PartGeom& p = c.topo.parts[i];
+    if (!p.is_pml) continue;
+    PmlBuffers& B = c.pml[i];
+    const size_t n = size_t(p.cells());
+    float* blk = nullptr;
+    if ((r = c.dalloc((void**)&blk, n * 9 * 4))) return r;
+    if ((r = c.dzero(blk, n * 9 * 4))) return r;
+    B.p[0] = blk; B.p[1] = blk + n;
+    for (int q = 0; q < 3; ++q) { B.phi[0][q] = blk + (2 + q) * n; B.phi[1][q] = blk + (5 + q) * n; }
+    B.f = blk + 8 * n;
+    std::vector<float> prof = pml_profile(p, c.prm.pml_layers, c.prm.h, c.prm.c0, c.pml_scale[p.pml_type]);
+    if ((r = c.dalloc((void**)&B.zprof, prof.size() * 4))) return r;
+    if ((r = c.upload(B.zprof, prof.data(), prof.size() * 4))) return r;
+    PmlDesc d{};
+    d.W = p.w; d.H = p.h; d.D = p.d; d.axis = p.pml_type / 2;
+    d.p[0] = B.p[0]; d.p[1] = B.p[1];
+    for (int s = 0; s < 2; ++s)
+      for (int q = 0; q < 3; ++q) d.phi[s][q] = B.phi[s][q];
+    d.f = B.f;
+    d.zprof = B.zprof;
+    d.block0 = 0;
+    B.desc = int(c.h_pml.size());
+    c.h_pml.push_back(d);
+    c.pml_blocks.push_back(ceil_div(n, 256));
+  }
+  if (!c.h_pml.empty()) {
+    if ((r = c.dalloc((void**)&c.d_pml, c.h_pml.size() * sizeof(PmlDesc)))) return r;
+    if ((r = c.upload(c.d_pml, c.h_pml.data(), c.h_pml.size() * sizeof(PmlDesc)))) return r;
+  }
+  return ensure_trace(c, 1024);
+}
+
+int finalize_plain(Ctx& c) {
+  int r;
+  if ((r = setup_common(c))) return r;
+  const int np = int(c.topo.parts.size());
+  c.dct.assign(np, DctPlain());
+  size_t maxn = 0;
+  for (int i = 0; i < np; ++i)
+    if (!c.topo.parts[i].is_pml) maxn = std::max(maxn, size_t(c.topo.parts[i].cells()));
+  if ((r = c.dalloc((void**)&c.tmp[0], maxn * 4)) || (r = c.dalloc((void**)&c.tmp[1], maxn * 4))) return r;
+  c.h_gemm.clear();
+  for (int i = 0; i < np; ++i) {
+    const PartGeom& p = c.topo.parts[i];
+    if (p.is_pml) continue;
+    DctPlain& D = c.dct[i];
+    const size_t n = size_t(p.cells());
+    float* blk = nullptr;
+    if ((r = c.dalloc((void**)&blk, n * 6 * 4))) return r;
+    if ((r = c.dzero(blk, n * 4 * 4))) return r;
+    D.F = blk; D.p = blk + n; D.M[0] = blk + 2 * n; D.M[1] = blk + 3 * n; D.a = blk + 4 * n; D.g = blk + 5 * n;
+    std::vector<float> a, g;
+    modal_tables(p.w, p.h, p.d, c.prm.h, c.prm.dt, c.prm.c0, a, g);
+    if ((r = c.upload(D.a, a.data(), n * 4)) || (r = c.upload(D.g, g.data(), n * 4))) return r;
+    const float *fw, *iw, *fh, *ih, *fd, *id;
+    if ((r = c.cos_tables(p.w, &fw, &iw)) || (r = c.cos_tables(p.h, &fh, &ih)) || (r = c.cos_tables(p.d, &fd, &id)))
+      return r;
+    const long long W = p.w, H = p.h, Dd = p.d;
+    D.gemm0 = int(c.h_gemm.size());
+    for (int par = 0; par < 2; ++par) {
+      float* cur = D.M[par];
+      float* nxt = D.M[par ^ 1];
+      // pass along x: B(k=x, n=row) contiguous in k; C(m=kx, n=row) contiguous in m
+      auto xpass = [&](const float* A, const float* in, float* out) {
+        GemmDesc g{};
+        g.A = A; g.B = in; g.C = out;
+        g.M = p.w; g.K = p.w; g.N = int(H * Dd); g.N1 = g.N;
+        g.bK = 1; g.b1 = W; g.b2 = 0;
+        g.cM = 1; g.c1 = W; g.c2 = 0;
+        g.alpha = 1.f;
+        return g;
+      };
+      // pass along y: per z-plane; n = (x, z)
+      auto ypass = [&](const float* A, const float* in, float* out) {
+        GemmDesc g{};
+        g.A = A; g.B = in; g.C = out;
+        g.M = p.h; g.K = p.h; g.N = int(W * Dd); g.N1 = p.w;
+        g.bK = W; g.b1 = 1; g.b2 = W * H;
+        g.cM = W; g.c1 = 1; g.c2 = W * H;
+        g.alpha = 1.f;
+        return g;
+      };
+      // pass along z: n = (x, y) flattened
+      auto zpass = [&](const float* A, const float* in, float* out) {
+        GemmDesc g{};
+        g.A = A; g.B = in; g.C = out;
+        g.M = p.d; g.K = p.d; g.N = int(W * H); g.N1 = g.N;
+        g.bK = W * H; g.b1 = 1; g.b2 = 0;
+        g.cM = W * H; g.c1 = 1; g.c2 = 0;
+        g.alpha = 1.f;
+        return g;
+      };
+      GemmDesc f0 = xpass(fw, D.F, c.tmp[0]);
+      GemmDesc f1 = ypass(fh, c.tmp[0], c.tmp[1]);
+      GemmDesc f2 = zpass(fd, c.tmp[1], nxt);  // modal update into the previous-modes buffer
+      f2.ep_cur = cur; f2.ep_a = D.a; f2.ep_g = D.g; f2.ep_b = 0.999f;
+      GemmDesc i0 = xpass(iw, nxt, c.tmp[0]);
+      GemmDesc i1 = ypass(ih, c.tmp[0], c.tmp[1]);
+      GemmDesc i2 = zpass(id, c.tmp[1], D.p);
+      for (GemmDesc* g : {&f0, &f1, &f2, &i0, &i1, &i2}) {
+        g->tiles_n = int(ceil_div(g->N, 64));
+        g->tile0 = 0;
+        c.h_gemm.push_back(*g);
+      }
+      if (par == 0) {
+        const GemmDesc* gs[6] = {&f0, &f1, &f2, &i0, &i1, &i2};
+        for (int j = 0; j < 6; ++j) D.tiles[j] = int(ceil_div(gs[j]->M, 64) * gs[j]->tiles_n);
+      }
+    }
+  }
+  if ((r = c.dalloc((void**)&c.d_gemm, c.h_gemm.size() * sizeof(GemmDesc)))) return r;
+  if ((r = c.upload(c.d_gemm, c.h_gemm.data(), c.h_gemm.size() * sizeof(GemmDesc)))) return r;
+
+  // partition access table for the interface kernel
+  c.h_parts.assign(np, PartAccess());
+  for (int i = 0; i < np; ++i) {
+    const PartGeom& p = c.topo.parts[i];
+    PartAccess& A = c.h_parts[i];
+    A.W = p.w; A.H = p.h; A.D = p.d;
+    if (p.is_pml) {
+      A.kind = 1;
+      A.p[0] = c.pml[i].p[0]; A.p[1] = c.pml[i].p[1]; A.f = c.pml[i].f;
+    } else {
+      A.kind = 0;
+      A.p[0] = A.p[1] = c.dct[i].p; A.f = c.dct[i].F;
+    }
+  }
+  if ((r = c.dalloc((void**)&c.d_parts, np * sizeof(PartAccess)))) return r;
+  if ((r = c.upload(c.d_parts, c.h_parts.data(), np * sizeof(PartAccess)))) return r;
+  if ((r = build_boundary_descs(c, c.h_bnd, c.bnd_elems))) return r;
+  if (!c.h_bnd.empty()) {
+    if ((r = c.dalloc((void**)&c.d_bnd, c.h_bnd.size() * sizeof(BndDesc)))) return r;
+    if ((r = c.upload(c.d_bnd, c.h_bnd.data(), c.h_bnd.size() * sizeof(BndDesc)))) return r;
+  }
+
+  // sources: ARD/partition.cpp:187-197 writes the sample into the owner's force volume
+  std::vector<SourceDesc> src;
+  for (const SourceDef& s : c.topo.sources) {
+    if (s.part < 0) continue;  // outside every DCT partition: never assigned (ARD/simulation.cpp:37-48)
+    const PartGeom& p = c.topo.parts[s.part];
+    SourceDesc d{};
+    d.part = s.part;
+    d.cell = ((long long)(s.z - p.z0) * p.h + (s.y - p.y0)) * p.w + (s.x - p.x0);
+    d.kind = s.wave.empty() ? 0 : 1;
+    d.n = int(s.wave.size());
+    if (d.kind == 1) {
+      float* w;
+      if ((r = c.dalloc((void**)&w, s.wave.size() * 4))) return r;
+      if ((r = c.upload(w, s.wave.data(), s.wave.size() * 4))) return r;
+      d.wave = w;
+    }
+    d.fptr = c.dct[s.part].F + d.cell;
+    src.push_back(d);
+  }
+  c.nsrc = int(src.size());
+  if (c.nsrc) {
+    if ((r = c.dalloc((void**)&c.d_src, src.size() * sizeof(SourceDesc)))) return r;
+    if ((r = c.upload(c.d_src, src.data(), src.size() * sizeof(SourceDesc)))) return r;
+  }
+  std::vector<RecDesc> rec;
+  for (const RecorderDef& rd : c.topo.recorders) {
+    const PartGeom& p = c.topo.parts[rd.part];
+    RecDesc d{};
+    d.p = c.dct[rd.part].p;
+    d.cell = ((long long)rd.lz * p.h + rd.ly) * p.w + rd.lx;
+    rec.push_back(d);
+  }
+  if (!rec.empty()) {
+    if ((r = c.dalloc((void**)&c.d_rec, rec.size() * sizeof(RecDesc)))) return r;
+    if ((r = c.upload(c.d_rec, rec.data(), rec.size() * sizeof(RecDesc)))) return r;
+  }
+  hipError_t e = hipStreamSynchronize(c.stream);
+  return e == hipSuccess ? ARD_OK : hip_fail(e, "hipStreamSynchronize");
+}
+
+// One Simulation::Update per iteration (ARD/simulation.cpp:300-317): sources,
+// every partition's update, then the boundaries in list order (the later
+// writer wins on shared cells), then the recorder sample.
+int step_plain(Ctx& c, int n) {
+  const int nr = int(c.topo.recorders.size());
+  for (int s = 0; s < n; ++s) {
+    const int par = c.time_step & 1;
+    if (launch_sources(c.d_src, c.nsrc, c.d_step, c.courant, c.stream)) return set_error(ARD_ERR_HIP, "sources");
+    for (size_t i = 0; i < c.dct.size(); ++i) {
+      const DctPlain& D = c.dct[i];
+      if (!D.F) continue;
+      const GemmDesc* g = c.d_gemm + D.gemm0 + par * 6;
+      int r = 0;
+      r |= launch_gemm(g + 0, 1, D.tiles[0], true, true, 0, c.stream);
+      r |= launch_gemm(g + 1, 1, D.tiles[1], false, false, 0, c.stream);
+      r |= launch_gemm(g + 2, 1, D.tiles[2], false, false, 1, c.stream);
+      r |= launch_gemm(g + 3, 1, D.tiles[3], true, true, 0, c.stream);
+      r |= launch_gemm(g + 4, 1, D.tiles[4], false, false, 0, c.stream);
+      r |= launch_gemm(g + 5, 1, D.tiles[5], false, false, 0, c.stream);
+      if (r) return set_error(ARD_ERR_HIP, "DCT pass launch");
+    }
+    for (size_t i = 0; i < c.h_pml.size(); ++i)
+      if (launch_pml(c.d_pml + i, 1, c.pml_blocks[i], par, c.prm.h, c.prm.dt, c.prm.c0, c.stream))
+        return set_error(ARD_ERR_HIP, "PML launch");
+    for (size_t b = 0; b < c.h_bnd.size(); ++b)
+      if (launch_interface(c.d_bnd + b, 1, c.d_parts, par ^ 1, c.bnd_elems[b], c.stream))
+        return set_error(ARD_ERR_HIP, "interface launch");
+    if (launch_record(c.d_rec, nr, c.d_trace, c.trace_cap, c.d_step, c.stream))
+      return set_error(ARD_ERR_HIP, "recorder launch");
+    c.time_step++;
+  }
+  return ARD_OK;
+}
+
+int pressure_field_ptr(Ctx& c, int part, const float** out) {
+  const PartGeom& p = c.topo.parts[part];
+  if (p.is_pml) {
+    *out = c.pml[part].p[c.time_step & 1];
+    return ARD_OK;
+  }
+  if (c.prm.form == ARD_FORM_PLAIN) {
+    *out = c.dct[part].p;
+    return ARD_OK;
+  }
+  return batched_pressure_field(c, part, out);
+}
+
+int set_force_cell(Ctx& c, int part, int x, int y, int z, float f) {
+  const PartGeom& p = c.topo.parts[part];
+  float* dst = nullptr;
+  const size_t idx = (size_t(z) * p.h + y) * p.w + x;
+  if (p.is_pml) {
+    dst = c.pml[part].f + idx;
+  } else if (c.prm.form == ARD_FORM_PLAIN) {
+    dst = c.dct[part].F + idx;
+  } else {
+    return batched_set_force(c, part, x, y, z, f);
+  }
+  return c.upload(dst, &f, 4);
+}
+
+}  // namespace ard

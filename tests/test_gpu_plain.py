@@ -1,0 +1,97 @@
+"""GPU parity of the plain form (per-partition launches) against the oracle.
+
+Tolerance: recorder traces within rel-L2 1e-4 of the FP32 oracle's golden
+traces (the bar stated for this path); fields within 1e-4 relative to their
+maximum after a few dozen steps."""
+import os
+
+import numpy as np
+import pytest
+
+import ard_oracle as O
+import ard_scenes as S
+import ard_sim as A
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+TOL = 1e-4
+
+
+def rel_l2(a, b):
+    a = np.asarray(a, np.float64).ravel()
+    b = np.asarray(b, np.float64).ravel()
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def test_two_rooms_trace_matches_golden():
+    sc = S.two_rooms()
+    print("case two_rooms plain", flush=True)
+    sim = A.Simulation(sc, form=A.FORM_PLAIN)
+    sim.update(sc.steps)
+    tr = sim.recorders()
+    g32 = np.load(os.path.join(GOLD, "two_rooms_fp32.npy"))
+    g64 = np.load(os.path.join(GOLD, "two_rooms_fp64.npy"))
+    e32, e64 = rel_l2(tr, g32), rel_l2(tr, g64)
+    print("two_rooms rel-L2 vs fp32 %.3e vs fp64 %.3e" % (e32, e64))
+    assert e32 <= TOL and e64 <= TOL
+
+
+def test_scene3_trace_matches_golden():
+    sc = S.scene3()
+    print("case scene3 plain", flush=True)
+    sim = A.Simulation(sc, form=A.FORM_PLAIN)
+    sim.update(sc.steps)
+    tr = sim.recorders()
+    g32 = np.load(os.path.join(GOLD, "scene3_fp32.npy"))
+    g64 = np.load(os.path.join(GOLD, "scene3_fp64.npy"))
+    e32, e64 = rel_l2(tr, g32), rel_l2(tr, g64)
+    print("scene3 rel-L2 vs fp32 %.3e vs fp64 %.3e" % (e32, e64))
+    assert e32 <= TOL and e64 <= TOL
+
+
+def small_scene():
+    """Three boxes with partial faces (PML runs, edge cells written twice)."""
+    sc = S.two_rooms()
+    sc.boxes = [(0, 0, 0, 16, 12, 10), (16, 0, 0, 12, 20, 10), (0, 12, 0, 16, 8, 10)]
+    sc.sources = [S.Source(5, 5, 4), S.Source(20, 15, 6)]
+    sc.recorders = [(22, 10, 5)]
+    sc.recorder_mode = S.REC_LOCAL
+    return sc
+
+
+@pytest.mark.parametrize("form", [A.FORM_PLAIN])
+def test_all_fields_match_oracle(form):
+    sc = small_scene()
+    print("case small_scene fields form", form, flush=True)
+    sim = A.Simulation(sc, form=form)
+    o = O.from_scene(sc)
+    assert sim.partitions() == o.partitions() and sim.boundaries() == o.boundaries()
+    n = 40
+    sim.update(n)
+    ref_tr = o.step(n)
+    np.testing.assert_array_less(rel_l2(sim.recorders(), ref_tr), TOL)
+    for i, info in enumerate(sim.partitions()):
+        got = sim.pressure_field(i)
+        ref = o.field(i, 0)
+        scale = np.abs(ref).max()
+        assert scale > 0
+        err = np.abs(got - ref).max() / scale
+        assert err < TOL, (i, info, err)
+
+
+def test_set_force_and_get_pressure():
+    sc = small_scene()
+    sc.sources = []
+    sim = A.Simulation(sc, form=A.FORM_PLAIN)
+    o = O.from_scene(sc)
+    # a persistent force at an interior cell of box 1 (ARD/dct_partition.cpp: never cleared)
+    sim.set_force(1, 6, 10, 5, 1e6)
+    f = o.field(1, 1)
+    f[5, 10, 6] = 1e6
+    o.set_field(1, 1, f)
+    sim.update(15)
+    o.step(15)
+    ref = o.field(1, 0)
+    got = sim.get_pressure(1, 6, 10, 5)
+    assert abs(got - ref[5, 10, 6]) <= TOL * np.abs(ref).max()
+    assert sim.time_step == 15

@@ -1,0 +1,78 @@
+"""CPU checks of libard.so: it loads, exports the C-ABI of include/ard.h, and its
+host-side topology (the Simulation constructor's rules) matches the oracle and
+the committed topology fixtures. No compute calls: these run without a GPU."""
+import json
+import os
+import re
+
+import pytest
+
+import ard_oracle as O
+import ard_scenes as S
+import ard_sim as A
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLD = os.path.join(ROOT, "tests", "golden")
+
+
+def header_symbols():
+    with open(os.path.join(ROOT, "include", "ard.h")) as f:
+        text = f.read()
+    return sorted(set(re.findall(r"\b(ard_[a-z_0-9]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    L = A.lib()
+    syms = header_symbols()
+    assert len(syms) >= 20
+    for s in syms:
+        assert hasattr(L, s), s
+    assert sorted(A.EXPORTED) == syms
+
+
+@pytest.mark.parametrize("name", ["two_rooms", "scene3"])
+def test_topology_matches_fixture(name):
+    sc = getattr(S, name)()
+    sim = A.Simulation(sc, device=-1, finalize=False)
+    with open(os.path.join(GOLD, name + "_topology.json")) as f:
+        topo = json.load(f)
+    assert [list(p) for p in sim.partitions()] == topo["partitions"]
+    assert [list(b) for b in sim.boundaries()] == topo["boundaries"]
+
+
+@pytest.mark.parametrize("seed", [7, 8, 9])
+def test_hall_topology_matches_oracle(seed):
+    sc = S.hall(seed=seed)
+    sim = A.Simulation(sc, device=-1, finalize=False)
+    o = O.from_scene(sc)
+    assert sim.partitions() == o.partitions()
+    assert sim.boundaries() == o.boundaries()
+
+
+def test_recorder_outside_margin_is_an_error():
+    sc = S.two_rooms()
+    sc.recorders = [(2, 20, 15)]  # 2 cells from the x = 0 face: no owner (ARD/recorder.cpp:26-38)
+    sim = A.Simulation(sc, device=-1, finalize=False)
+    with pytest.raises(A.ArdError, match="not 5 cells inside"):
+        A._check(sim.L.ard_finalize(sim.ctx))
+
+
+def test_reference_addressing_out_of_array_is_an_error():
+    sc = S.two_rooms()
+    # a 20^3 box at y = 200: global coordinates (10, 210, 14) address
+    # 14*400 + 210*20 + 10 = 9810 >= 8000 cells of its array
+    sc.boxes = [(0, 200, 0, 20, 20, 20)]
+    sc.sources = [S.Source(10, 205, 10)]
+    sc.recorders = [(10, 210, 14)]
+    sim = A.Simulation(sc, device=-1, finalize=False)
+    with pytest.raises(A.ArdError, match="outside its partition"):
+        A._check(sim.L.ard_finalize(sim.ctx))
+
+
+def test_bad_arguments_rejected():
+    sc = S.two_rooms()
+    sim = A.Simulation(sc, device=-1, finalize=False)
+    with pytest.raises(A.ArdError):
+        A._check(sim.L.ard_add_dct_partition(sim.ctx, 0, 0, 0, 30, 30, 30))  # topology already built
+    with pytest.raises(A.ArdError):
+        A._check(sim.L.ard_step(sim.ctx, 1))  # not finalized
