@@ -11,7 +11,7 @@ CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libard.so")
 OBJ = os.path.join(HERE, "_build")
 
-HIP_SOURCES = ["ard_kernels.hip", "ard_batched.hip", "ard_ctx.cpp", "ard_plain.cpp"]
+HIP_SOURCES = ["ard_kernels.hip", "ard_batched_kernels.hip", "ard_batched.hip", "ard_ctx.cpp", "ard_plain.cpp"]
 CXX_SOURCES = ["ard_topology.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -1,0 +1,84 @@
+// Descriptors and launchers of the batched face-shell form (ard_batched_kernels.hip).
+#pragma once
+
+#include "ard_device.h"
+
+#define ARD_MAX_REC_PER_PART 8
+
+namespace ard {
+
+// sum / difference of the face pair planes (x, n-1-x), x = 0..2
+struct FoldDesc {
+  const float* src;  // [6][plane] raw face slab, plane q = x for q < 3, n-6+q for q >= 3
+  float* dst;        // [6][plane]: q < 3 sum, q >= 3 difference
+  long long plane;
+};
+
+struct UnfoldDesc {
+  const float* part;  // [nchunk][6][plane] partial reductions: q < 3 even modes, q >= 3 odd
+  float* dst;         // [6][plane] in face-pair order
+  long long plane;
+  int nchunk;
+};
+
+// 1D factors of a point in mode space: fx[W], fy[H], fz[D]; for a source the
+// product fx*fy*fz*val is its forcing spectrum, for a recorder the weights
+// of its pressure sample.
+struct SrcTab {
+  const float* fx;
+  const float* fy;
+  const float* fz;
+  const float* val;
+};
+struct RecTab {
+  const float* fx;
+  const float* fy;
+  const float* fz;
+};
+
+struct BSourceDesc {
+  int kind;  // 0 Gaussian, 1 waveform, 2 constant
+  const float* wave;
+  int n;
+  float value;
+  float* dst;  // face-slab cell or value slot
+};
+
+struct ModalDesc {
+  int W, H, D;
+  int nbx, nby, nbz, kz_chunk;
+  const float* a;
+  const float* g;
+  float* M[2];
+  const float* Ex;  // [W][3]  2 c_W(kx, x)
+  const float* Rx;  // [W][3]  e(kx) c_W(kx, x)
+  const float* Ey;
+  const float* Ry;
+  const float* Ez;
+  const float* Rz;
+  const float* Gx;  // [6][D][H] folded face spectra (q < 3 even, q >= 3 odd)
+  const float* Gy;  // [6][D][W]
+  const float* Gz;  // [6][H][W]
+  float* Tx;        // [nbx][6][D][H] partial reductions
+  float* Ty;        // [nby][6][D][W]
+  float* Tz;        // [nbz][6][H][W]
+  int nsrc;
+  const SrcTab* src;
+  int nrec;
+  const RecTab* rec;
+  float* rec_part;  // [nbx*nby*nbz][ARD_MAX_REC_PER_PART]
+};
+
+struct BRecDesc {
+  const float* part;  // first block partial of the owner, stride ARD_MAX_REC_PER_PART
+  int nblocks;
+};
+
+int launch_fold(const FoldDesc* d, const long long* starts, int n, long long total, hipStream_t s);
+int launch_unfold(const UnfoldDesc* d, const long long* starts, int n, long long total, hipStream_t s);
+int launch_bsources(const BSourceDesc* d, int n, const int* d_step, float courant, hipStream_t s);
+int launch_modal(int rpw, const ModalDesc* d, const long long* starts, int n, long long blocks, int par,
+                 hipStream_t s);
+int launch_brecord(const BRecDesc* d, int n, float* trace, int cap, int* d_step, hipStream_t s);
+
+}  // namespace ard

@@ -1,15 +1,653 @@
-// Batched form of the ARD step (placeholder until the face-shell kernels land).
+// Batched face-shell form of the ARD step: every phase of Simulation::Update
+// (ARD/simulation.cpp:300-317) for all partitions in one launch, and one step
+// captured in a hipGraph (one graph per step parity, replayed alternately).
+//
+// Per step: sources -> PML update -> fold face slabs -> 2D DCT-II of the face
+// slabs (2 grouped GEMM launches) -> modal pass (force spectrum expansion,
+// modal update, reduction onto face planes and recorders) -> unfold ->
+// 2D DCT-III to face-slab pressure (2 grouped GEMM launches) -> interface
+// forcing with last-writer masks -> recorder samples.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "ard_batched.h"
 #include "ard_ctx_impl.h"
+#include "ard_kernels.h"
 
 namespace ard {
 
-int finalize_batched(Ctx& c) { return set_error(ARD_ERR_UNSUPPORTED, "batched form not built yet"); }
-int step_batched(Ctx& c, int n) { return set_error(ARD_ERR_UNSUPPORTED, "batched form not built yet"); }
-int batched_pressure_field(Ctx& c, int part, const float** out) {
-  return set_error(ARD_ERR_UNSUPPORTED, "batched form not built yet");
+namespace {
+long long ceil_div(long long a, long long b) { return (a + b - 1) / b; }
+const double kPi = 3.14159265358979323846;
+
+// Host twin of shell_index (ard_kernels.hip): which face slab stores cell
+// (x,y,z) of a W x H x D partition, and where.
+int shell_of(int W, int H, int D, int x, int y, int z, long long* off) {
+  if (x < 3 || x >= W - 3) {
+    const int q = x < 3 ? x : x - W + 6;
+    *off = ((long long)q * D + z) * H + y;
+    return 0;
+  }
+  if (y < 3 || y >= H - 3) {
+    const int q = y < 3 ? y : y - H + 6;
+    *off = ((long long)q * D + z) * W + x;
+    return 1;
+  }
+  if (z < 3 || z >= D - 3) {
+    const int q = z < 3 ? z : z - D + 6;
+    *off = ((long long)q * H + y) * W + x;
+    return 2;
+  }
+  return -1;
 }
+}  // namespace
+
+struct BPart {
+  // sizes of the three face-slab families: [6][D][H], [6][D][W], [6][H][W]
+  long long plane[3] = {0, 0, 0};
+  float* F[3] = {};   // raw face-slab force (written by the interface kernel and sources)
+  float* S[3] = {};   // folded force; after the modal pass, unfolded mode planes
+  float* U[3] = {};   // GEMM intermediate
+  float* G[3] = {};   // folded force spectra
+  float* P[3] = {};   // face-slab pressure
+  float* T[3] = {};   // partial reductions [nchunk][6][plane]
+  float* M[2] = {};
+  float* a = nullptr;
+  float* g = nullptr;
+  float* E[3] = {};   // Ex, Ey, Ez
+  float* R[3] = {};   // Rx, Ry, Rz
+  float* rec_part = nullptr;
+  int rpw = 4, nbx = 0, nby = 0, nbz = 0, kz_chunk = 0;
+  int nrec = 0;
+  std::vector<int> recs;  // recorder ids owned
+};
+
+struct PointSrc {
+  int part;
+  int x, y, z;      // local cell
+  int kind;         // 0 Gaussian, 1 waveform, 2 constant
+  const float* wave = nullptr;
+  int n = 0;
+  float value = 0.f;
+  float* slot = nullptr;   // value slot (interior) or face-slab cell
+  bool in_shell = false;
+  float* tab = nullptr;    // fx, fy, fz factor tables (interior points)
+};
+
+struct BatchedState {
+  std::vector<BPart> parts;  // by partition id (DCT entries used)
+  std::vector<PointSrc> srcs;
+  // device descriptor tables
+  FoldDesc* d_fold = nullptr; long long* d_fold_start = nullptr; int nfold = 0; long long fold_total = 0;
+  UnfoldDesc* d_unfold = nullptr; long long* d_unfold_start = nullptr; int nunfold = 0; long long unfold_total = 0;
+  GemmDesc* d_gemm[4] = {};   // FT1, FT2, IT1, IT2
+  int ngemm[4] = {}, gemm_tiles[4] = {};
+  struct ModalGroup {
+    int rpw;
+    ModalDesc* d_desc[2] = {};  // same table for both parities (the kernel takes the parity)
+    long long* d_start = nullptr;
+    int n = 0;
+    long long blocks = 0;
+  };
+  std::vector<ModalGroup> modal;
+  PmlDesc* d_pml = nullptr; long long pml_blocks = 0; int npml = 0;
+  BndDesc* d_bnd = nullptr; long long bnd_total = 0; int nbnd = 0;
+  PartAccess* d_parts = nullptr;
+  BSourceDesc* d_src = nullptr; int nsrc = 0;
+  BRecDesc* d_rec = nullptr;
+  float* d_slots = nullptr;  // value slots of interior point sources
+  int nslots = 0;
+  // dense pressure evaluation on demand
+  float* dense[3] = {};
+  size_t dense_n = 0;
+  GemmDesc* d_dense = nullptr;
+  const float* graph_trace = nullptr;
+  bool graphs_valid = false;
+};
+
+namespace {
+
+int upload_vec(Ctx& c, void** dptr, const void* src, size_t bytes) {
+  int r = c.dalloc(dptr, bytes);
+  if (r) return r;
+  return bytes ? c.upload(*dptr, src, bytes) : ARD_OK;
+}
+
+// 1D factor tables along one axis of length n for points at x0..x2 = 0,1,2:
+// E[k][x] = 2 c_n(k, x) (force expansion, REDFT10 weight), R[k][x] = e(k) c_n(k, x)
+// (pressure reduction, REDFT01 weight).
+void axis_tables(int n, std::vector<float>& E, std::vector<float>& R) {
+  E.resize(size_t(n) * 3);
+  R.resize(size_t(n) * 3);
+  for (int k = 0; k < n; ++k)
+    for (int x = 0; x < 3; ++x) {
+      const double cs = std::cos(kPi * k * (x + 0.5) / n);
+      E[size_t(k) * 3 + x] = float(2.0 * cs);
+      R[size_t(k) * 3 + x] = float((k == 0 ? 1.0 : 2.0) * cs);
+    }
+}
+
+void point_table(int n, int x, bool forward, float* out) {
+  for (int k = 0; k < n; ++k) {
+    const double cs = std::cos(kPi * k * (x + 0.5) / n);
+    out[k] = float(forward ? 2.0 * cs : (k == 0 ? 1.0 : 2.0) * cs);
+  }
+}
+
+GemmDesc gemm_k_contig(const float* A, int L, const float* B, float* C, int rows, long long ld) {
+  GemmDesc g{};
+  g.A = A; g.B = B; g.C = C;
+  g.M = L; g.K = L; g.N = rows; g.N1 = rows;
+  g.bK = 1; g.b1 = ld; g.b2 = 0;
+  g.cM = 1; g.c1 = ld; g.c2 = 0;
+  g.alpha = 1.f;
+  g.tiles_n = int(ceil_div(g.N, 64));
+  return g;
+}
+
+// transform along the middle axis of a [6][L][inner] stack
+GemmDesc gemm_n_contig(const float* A, int L, const float* B, float* C, int inner) {
+  GemmDesc g{};
+  g.A = A; g.B = B; g.C = C;
+  g.M = L; g.K = L; g.N = 6 * inner; g.N1 = inner;
+  g.bK = inner; g.b1 = 1; g.b2 = (long long)L * inner;
+  g.cM = inner; g.c1 = 1; g.c2 = (long long)L * inner;
+  g.alpha = 1.f;
+  g.tiles_n = int(ceil_div(g.N, 64));
+  return g;
+}
+
+int upload_gemms(Ctx& c, std::vector<GemmDesc>& v, GemmDesc** d, int* tiles) {
+  int t = 0;
+  for (auto& g : v) {
+    g.tile0 = t;
+    t += int(ceil_div(g.M, 64)) * g.tiles_n;
+  }
+  *tiles = t;
+  return upload_vec(c, (void**)d, v.data(), v.size() * sizeof(GemmDesc));
+}
+
+int build_sources(Ctx& c) {
+  BatchedState& B = *c.bs;
+  int r;
+  std::vector<BSourceDesc> sd;
+  B.nslots = 0;
+  for (auto& s : B.srcs) {
+    const PartGeom& p = c.topo.parts[s.part];
+    long long off;
+    if (shell_of(p.w, p.h, p.d, s.x, s.y, s.z, &off) < 0) B.nslots++;
+  }
+  if (B.nslots) {
+    if ((r = c.dalloc((void**)&B.d_slots, B.nslots * 4))) return r;
+    if ((r = c.dzero(B.d_slots, B.nslots * 4))) return r;
+  }
+  int slot = 0;
+  for (auto& s : B.srcs) {
+    const PartGeom& p = c.topo.parts[s.part];
+    BPart& P = B.parts[s.part];
+    long long off;
+    const int w = shell_of(p.w, p.h, p.d, s.x, s.y, s.z, &off);
+    if (w >= 0) {
+      s.in_shell = true;
+      s.slot = P.F[w] + off;
+    } else {
+      s.in_shell = false;
+      s.slot = B.d_slots + slot++;
+      if (!s.tab) {
+        std::vector<float> t(size_t(p.w) + p.h + p.d);
+        point_table(p.w, s.x, true, t.data());
+        point_table(p.h, s.y, true, t.data() + p.w);
+        point_table(p.d, s.z, true, t.data() + p.w + p.h);
+        if ((r = upload_vec(c, (void**)&s.tab, t.data(), t.size() * 4))) return r;
+      }
+    }
+    BSourceDesc d{};
+    d.kind = s.kind; d.wave = s.wave; d.n = s.n; d.value = s.value; d.dst = s.slot;
+    sd.push_back(d);
+  }
+  B.nsrc = int(sd.size());
+  if (B.nsrc && (r = upload_vec(c, (void**)&B.d_src, sd.data(), sd.size() * sizeof(BSourceDesc)))) return r;
+  return ARD_OK;
+}
+
+int build_modal(Ctx& c) {
+  BatchedState& B = *c.bs;
+  int r;
+  std::map<int, std::vector<ModalDesc>> groups;
+  for (int i = 0; i < int(c.topo.parts.size()); ++i) {
+    const PartGeom& p = c.topo.parts[i];
+    if (p.is_pml) continue;
+    BPart& P = B.parts[i];
+    ModalDesc d{};
+    d.W = p.w; d.H = p.h; d.D = p.d;
+    d.nbx = P.nbx; d.nby = P.nby; d.nbz = P.nbz; d.kz_chunk = P.kz_chunk;
+    d.a = P.a; d.g = P.g; d.M[0] = P.M[0]; d.M[1] = P.M[1];
+    d.Ex = P.E[0]; d.Rx = P.R[0]; d.Ey = P.E[1]; d.Ry = P.R[1]; d.Ez = P.E[2]; d.Rz = P.R[2];
+    d.Gx = P.G[0]; d.Gy = P.G[1]; d.Gz = P.G[2];
+    d.Tx = P.T[0]; d.Ty = P.T[1]; d.Tz = P.T[2];
+    std::vector<SrcTab> st;
+    for (auto& s : B.srcs)
+      if (s.part == i && !s.in_shell) st.push_back(SrcTab{s.tab, s.tab + p.w, s.tab + p.w + p.h, s.slot});
+    d.nsrc = int(st.size());
+    if (d.nsrc) {
+      SrcTab* ds;
+      if ((r = upload_vec(c, (void**)&ds, st.data(), st.size() * sizeof(SrcTab)))) return r;
+      d.src = ds;
+    }
+    std::vector<RecTab> rt;
+    for (int rid : P.recs) {
+      const RecorderDef& rd = c.topo.recorders[rid];
+      std::vector<float> t(size_t(p.w) + p.h + p.d);
+      point_table(p.w, rd.lx, false, t.data());
+      point_table(p.h, rd.ly, false, t.data() + p.w);
+      point_table(p.d, rd.lz, false, t.data() + p.w + p.h);
+      float* dt;
+      if ((r = upload_vec(c, (void**)&dt, t.data(), t.size() * 4))) return r;
+      rt.push_back(RecTab{dt, dt + p.w, dt + p.w + p.h});
+    }
+    d.nrec = int(rt.size());
+    if (d.nrec) {
+      RecTab* dr;
+      if ((r = upload_vec(c, (void**)&dr, rt.data(), rt.size() * sizeof(RecTab)))) return r;
+      d.rec = dr;
+    }
+    d.rec_part = P.rec_part;
+    groups[P.rpw].push_back(d);
+  }
+  B.modal.clear();
+  for (auto& kv : groups) {
+    BatchedState::ModalGroup g;
+    g.rpw = kv.first;
+    std::vector<long long> starts;
+    long long blocks = 0;
+    for (auto& d : kv.second) {
+      starts.push_back(blocks);
+      blocks += (long long)d.nbx * d.nby * d.nbz;
+    }
+    g.n = int(kv.second.size());
+    g.blocks = blocks;
+    if ((r = upload_vec(c, (void**)&g.d_desc[0], kv.second.data(), kv.second.size() * sizeof(ModalDesc)))) return r;
+    g.d_desc[1] = g.d_desc[0];
+    if ((r = upload_vec(c, (void**)&g.d_start, starts.data(), starts.size() * 8))) return r;
+    B.modal.push_back(g);
+  }
+  return ARD_OK;
+}
+
+// Last-writer masks: replays the boundaries in list order over the written
+// cells (ARD/simulation.cpp:313-317 run serially) and keeps, for each face
+// element, the outputs that no later boundary overwrites.
+int build_masks(Ctx& c, std::vector<BndDesc>& descs, const std::vector<long long>& elems) {
+  const int np = int(c.topo.parts.size());
+  std::vector<std::vector<int>> owner[3];
+  for (int w = 0; w < 3; ++w) owner[w].resize(np);
+  std::vector<std::vector<int>> powner(np);
+  for (int i = 0; i < np; ++i) {
+    const PartGeom& p = c.topo.parts[i];
+    if (p.is_pml) powner[i].assign(size_t(p.cells()), -1);
+    else {
+      owner[0][i].assign(size_t(6) * p.d * p.h, -1);
+      owner[1][i].assign(size_t(6) * p.d * p.w, -1);
+      owner[2][i].assign(size_t(6) * p.h * p.w, -1);
+    }
+  }
+  auto target = [&](const BndDesc& d, int side, int u, int v, int m, int** slot) {
+    const PartGeom& p = c.topo.parts[d.side[side]];
+    const int uu = u - d.org[side][0], vv = v - d.org[side][1];
+    const int n0 = d.type == 0 ? p.w : (d.type == 1 ? p.h : p.d);
+    const int nrm = side == 0 ? n0 + m : m;
+    int x, y, z;
+    if (d.type == 0) { x = nrm; y = uu; z = vv; }
+    else if (d.type == 1) { x = uu; y = nrm; z = vv; }
+    else { x = uu; y = vv; z = nrm; }
+    if (p.is_pml) {
+      *slot = &powner[d.side[side]][(size_t(z) * p.h + y) * p.w + x];
+    } else {
+      long long off;
+      const int w = shell_of(p.w, p.h, p.d, x, y, z, &off);
+      *slot = w >= 0 ? &owner[w][d.side[side]][size_t(off)] : nullptr;
+    }
+  };
+  for (int b = 0; b < int(descs.size()); ++b) {
+    const BndDesc& d = descs[b];
+    for (long long e = 0; e < elems[b]; ++e) {
+      const int u = d.t0s + int(e / d.t1n), v = d.t1s + int(e % d.t1n);
+      for (int r = 0; r < 6; ++r) {
+        int* s;
+        target(d, r < 3 ? 0 : 1, u, v, r - 3, &s);
+        if (s) *s = b;
+      }
+    }
+  }
+  BatchedState& B = *c.bs;
+  std::vector<uint8_t> mask;
+  std::vector<long long> mstart(descs.size());
+  for (int b = 0; b < int(descs.size()); ++b) {
+    const BndDesc& d = descs[b];
+    mstart[b] = (long long)mask.size();
+    for (long long e = 0; e < elems[b]; ++e) {
+      const int u = d.t0s + int(e / d.t1n), v = d.t1s + int(e % d.t1n);
+      uint8_t m = 0;
+      for (int r = 0; r < 6; ++r) {
+        int* s;
+        target(d, r < 3 ? 0 : 1, u, v, r - 3, &s);
+        if (s && *s == b) m |= uint8_t(1u << r);
+      }
+      mask.push_back(m);
+    }
+  }
+  uint8_t* dm = nullptr;
+  int r = upload_vec(c, (void**)&dm, mask.data(), mask.size());
+  if (r) return r;
+  long long e0 = 0;
+  for (int b = 0; b < int(descs.size()); ++b) {
+    descs[b].mask = dm + mstart[b];
+    descs[b].elem0 = e0;
+    e0 += elems[b];
+  }
+  B.bnd_total = e0;
+  return ARD_OK;
+}
+
+}  // namespace
+
+int finalize_batched(Ctx& c) {
+  int r;
+  if ((r = setup_common(c))) return r;
+  c.bs = new BatchedState();
+  BatchedState& B = *c.bs;
+  const int np = int(c.topo.parts.size());
+  B.parts.assign(np, BPart());
+  for (size_t i = 0; i < c.topo.recorders.size(); ++i) B.parts[c.topo.recorders[i].part].recs.push_back(int(i));
+  std::vector<FoldDesc> folds;
+  std::vector<UnfoldDesc> unfolds;
+  std::vector<GemmDesc> gm[4];
+  for (int i = 0; i < np; ++i) {
+    const PartGeom& p = c.topo.parts[i];
+    if (p.is_pml) continue;
+    BPart& P = B.parts[i];
+    if (int(P.recs.size()) > ARD_MAX_REC_PER_PART)
+      return set_error(ARD_ERR_UNSUPPORTED, "more than 8 recorders in one partition");
+    const long long W = p.w, H = p.h, D = p.d, N = W * H * D;
+    P.plane[0] = D * H; P.plane[1] = D * W; P.plane[2] = H * W;
+    // tiling of the modal pass: rows per wave with the least padding, kz chunks of ~48
+    int best = 4, bestw = 1 << 30;
+    for (int rpw : {8, 5, 4}) {
+      const int waste = int(ceil_div(H, 4 * rpw) * 4 * rpw - H);
+      if (waste < bestw) { bestw = waste; best = rpw; }
+    }
+    P.rpw = best;
+    P.nbx = int(ceil_div(W, 64));
+    P.nby = int(ceil_div(H, 4 * P.rpw));
+    P.nbz = int(ceil_div(D, 48));
+    P.kz_chunk = int(ceil_div(ceil_div(D, P.nbz), 2) * 2);
+    P.nbz = int(ceil_div(D, P.kz_chunk));
+    const long long shell = 6 * (P.plane[0] + P.plane[1] + P.plane[2]);
+    const long long tsz = 6 * (P.nbx * P.plane[0] + P.nby * P.plane[1] + P.nbz * P.plane[2]);
+    float* blk;
+    if ((r = c.dalloc((void**)&blk, size_t(5 * shell + tsz + 4 * N) * 4))) return r;
+    if ((r = c.dzero(blk, size_t(5 * shell + tsz + 4 * N) * 4))) return r;
+    float* q = blk;
+    for (float** arr : {P.F, P.S, P.U, P.G, P.P})
+      for (int w = 0; w < 3; ++w) { arr[w] = q; q += 6 * P.plane[w]; }
+    P.T[0] = q; q += 6 * P.nbx * P.plane[0];
+    P.T[1] = q; q += 6 * P.nby * P.plane[1];
+    P.T[2] = q; q += 6 * P.nbz * P.plane[2];
+    P.M[0] = q; q += N;
+    P.M[1] = q; q += N;
+    P.a = q; q += N;
+    P.g = q; q += N;
+    std::vector<float> a, g;
+    modal_tables(p.w, p.h, p.d, c.prm.h, c.prm.dt, c.prm.c0, a, g);
+    if ((r = c.upload(P.a, a.data(), N * 4)) || (r = c.upload(P.g, g.data(), N * 4))) return r;
+    const int dims[3] = {p.w, p.h, p.d};
+    for (int ax = 0; ax < 3; ++ax) {
+      std::vector<float> E, R;
+      axis_tables(dims[ax], E, R);
+      if ((r = upload_vec(c, (void**)&P.E[ax], E.data(), E.size() * 4))) return r;
+      if ((r = upload_vec(c, (void**)&P.R[ax], R.data(), R.size() * 4))) return r;
+    }
+    if ((r = c.dalloc((void**)&P.rec_part, size_t(P.nbx) * P.nby * P.nbz * ARD_MAX_REC_PER_PART * 4))) return r;
+    for (int w = 0; w < 3; ++w) {
+      folds.push_back(FoldDesc{P.F[w], P.S[w], P.plane[w]});
+      unfolds.push_back(UnfoldDesc{P.T[w], P.S[w], P.plane[w], w == 0 ? P.nbx : (w == 1 ? P.nby : P.nbz)});
+    }
+    const float *fw, *iw, *fh, *ih, *fd, *id;
+    if ((r = c.cos_tables(p.w, &fw, &iw)) || (r = c.cos_tables(p.h, &fh, &ih)) || (r = c.cos_tables(p.d, &fd, &id)))
+      return r;
+    // x faces [6][D][H]: along y (contiguous) then z; y faces [6][D][W]: x then z; z faces [6][H][W]: x then y
+    gm[0].push_back(gemm_k_contig(fh, p.h, P.S[0], P.U[0], int(6 * D), H));
+    gm[0].push_back(gemm_k_contig(fw, p.w, P.S[1], P.U[1], int(6 * D), W));
+    gm[0].push_back(gemm_k_contig(fw, p.w, P.S[2], P.U[2], int(6 * H), W));
+    gm[1].push_back(gemm_n_contig(fd, p.d, P.U[0], P.G[0], p.h));
+    gm[1].push_back(gemm_n_contig(fd, p.d, P.U[1], P.G[1], p.w));
+    gm[1].push_back(gemm_n_contig(fh, p.h, P.U[2], P.G[2], p.w));
+    gm[2].push_back(gemm_k_contig(ih, p.h, P.S[0], P.U[0], int(6 * D), H));
+    gm[2].push_back(gemm_k_contig(iw, p.w, P.S[1], P.U[1], int(6 * D), W));
+    gm[2].push_back(gemm_k_contig(iw, p.w, P.S[2], P.U[2], int(6 * H), W));
+    gm[3].push_back(gemm_n_contig(id, p.d, P.U[0], P.P[0], p.h));
+    gm[3].push_back(gemm_n_contig(id, p.d, P.U[1], P.P[1], p.w));
+    gm[3].push_back(gemm_n_contig(ih, p.h, P.U[2], P.P[2], p.w));
+  }
+  for (int k = 0; k < 4; ++k) {
+    B.ngemm[k] = int(gm[k].size());
+    if ((r = upload_gemms(c, gm[k], &B.d_gemm[k], &B.gemm_tiles[k]))) return r;
+  }
+  {
+    std::vector<long long> st;
+    long long t = 0;
+    for (auto& f : folds) { st.push_back(t); t += 3 * f.plane; }
+    B.nfold = int(folds.size());
+    B.fold_total = t;
+    if ((r = upload_vec(c, (void**)&B.d_fold, folds.data(), folds.size() * sizeof(FoldDesc)))) return r;
+    if ((r = upload_vec(c, (void**)&B.d_fold_start, st.data(), st.size() * 8))) return r;
+    std::vector<long long> su;
+    t = 0;
+    for (auto& u : unfolds) { su.push_back(t); t += 3 * u.plane; }
+    B.nunfold = int(unfolds.size());
+    B.unfold_total = t;
+    if ((r = upload_vec(c, (void**)&B.d_unfold, unfolds.data(), unfolds.size() * sizeof(UnfoldDesc)))) return r;
+    if ((r = upload_vec(c, (void**)&B.d_unfold_start, su.data(), su.size() * 8))) return r;
+  }
+  // PML slabs: one grouped launch
+  {
+    std::vector<PmlDesc> pd = c.h_pml;
+    long long b0 = 0;
+    for (size_t k = 0; k < pd.size(); ++k) {
+      pd[k].block0 = b0;
+      b0 += c.pml_blocks[k];
+    }
+    B.npml = int(pd.size());
+    B.pml_blocks = b0;
+    if (B.npml && (r = upload_vec(c, (void**)&B.d_pml, pd.data(), pd.size() * sizeof(PmlDesc)))) return r;
+  }
+  // partition access for the interface kernel
+  std::vector<PartAccess> pa(np);
+  for (int i = 0; i < np; ++i) {
+    const PartGeom& p = c.topo.parts[i];
+    PartAccess& A = pa[i];
+    A.W = p.w; A.H = p.h; A.D = p.d;
+    if (p.is_pml) {
+      A.kind = 1;
+      A.p[0] = c.pml[i].p[0]; A.p[1] = c.pml[i].p[1]; A.f = c.pml[i].f;
+    } else {
+      A.kind = 2;
+      for (int w = 0; w < 3; ++w) { A.ps[w] = B.parts[i].P[w]; A.fs[w] = B.parts[i].F[w]; }
+    }
+  }
+  if ((r = upload_vec(c, (void**)&B.d_parts, pa.data(), pa.size() * sizeof(PartAccess)))) return r;
+  std::vector<BndDesc> bd;
+  std::vector<long long> be;
+  if ((r = build_boundary_descs(c, bd, be))) return r;
+  if ((r = build_masks(c, bd, be))) return r;
+  B.nbnd = int(bd.size());
+  if (B.nbnd && (r = upload_vec(c, (void**)&B.d_bnd, bd.data(), bd.size() * sizeof(BndDesc)))) return r;
+  // sources
+  for (const SourceDef& s : c.topo.sources) {
+    if (s.part < 0) continue;
+    const PartGeom& p = c.topo.parts[s.part];
+    PointSrc ps;
+    ps.part = s.part;
+    ps.x = s.x - p.x0; ps.y = s.y - p.y0; ps.z = s.z - p.z0;
+    ps.kind = s.wave.empty() ? 0 : 1;
+    if (ps.kind == 1) {
+      float* w;
+      if ((r = upload_vec(c, (void**)&w, s.wave.data(), s.wave.size() * 4))) return r;
+      ps.wave = w;
+      ps.n = int(s.wave.size());
+    }
+    B.srcs.push_back(ps);
+  }
+  if ((r = build_sources(c)) || (r = build_modal(c))) return r;
+  // recorders
+  std::vector<BRecDesc> rd;
+  for (const RecorderDef& rc : c.topo.recorders) {
+    const BPart& P = B.parts[rc.part];
+    int slot = int(std::find(P.recs.begin(), P.recs.end(), int(&rc - &c.topo.recorders[0])) - P.recs.begin());
+    rd.push_back(BRecDesc{P.rec_part + slot, P.nbx * P.nby * P.nbz});
+  }
+  if (!rd.empty() && (r = upload_vec(c, (void**)&B.d_rec, rd.data(), rd.size() * sizeof(BRecDesc)))) return r;
+  hipError_t e = hipStreamSynchronize(c.stream);
+  return e == hipSuccess ? ARD_OK : hip_fail(e, "hipStreamSynchronize");
+}
+
+namespace {
+
+int enqueue_step(Ctx& c, int par) {
+  BatchedState& B = *c.bs;
+  int r = 0;
+  r |= launch_bsources(B.d_src, B.nsrc, c.d_step, c.courant, c.stream);
+  r |= launch_pml(B.d_pml, B.npml, B.pml_blocks, par, c.prm.h, c.prm.dt, c.prm.c0, c.stream);
+  r |= launch_fold(B.d_fold, B.d_fold_start, B.nfold, B.fold_total, c.stream);
+  r |= launch_gemm(B.d_gemm[0], B.ngemm[0], B.gemm_tiles[0], true, true, 0, c.stream);
+  r |= launch_gemm(B.d_gemm[1], B.ngemm[1], B.gemm_tiles[1], false, false, 0, c.stream);
+  for (auto& g : B.modal) r |= launch_modal(g.rpw, g.d_desc[par], g.d_start, g.n, g.blocks, par, c.stream);
+  r |= launch_unfold(B.d_unfold, B.d_unfold_start, B.nunfold, B.unfold_total, c.stream);
+  r |= launch_gemm(B.d_gemm[2], B.ngemm[2], B.gemm_tiles[2], true, true, 0, c.stream);
+  r |= launch_gemm(B.d_gemm[3], B.ngemm[3], B.gemm_tiles[3], false, false, 0, c.stream);
+  // interface reads the PML pressure just written: buffer par ^ 1
+  r |= launch_interface(B.d_bnd, B.nbnd, B.d_parts, par ^ 1, B.bnd_total, c.stream);
+  r |= launch_brecord(B.d_rec, int(c.topo.recorders.size()), c.d_trace, c.trace_cap, c.d_step, c.stream);
+  return r ? set_error(ARD_ERR_HIP, "batched step launch") : ARD_OK;
+}
+
+int build_graphs(Ctx& c) {
+  for (int par = 0; par < 2; ++par) {
+    if (c.graph_exec[par]) {
+      hipGraphExecDestroy(c.graph_exec[par]);
+      c.graph_exec[par] = nullptr;
+    }
+    hipGraph_t g;
+    hipError_t e = hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) return hip_fail(e, "hipStreamBeginCapture");
+    int r = enqueue_step(c, par);
+    e = hipStreamEndCapture(c.stream, &g);
+    if (r) return r;
+    if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+    e = hipGraphInstantiate(&c.graph_exec[par], g, nullptr, nullptr, 0);
+    hipGraphDestroy(g);
+    if (e != hipSuccess) return hip_fail(e, "hipGraphInstantiate");
+  }
+  c.bs->graph_trace = c.d_trace;
+  c.bs->graphs_valid = true;
+  return ARD_OK;
+}
+
+}  // namespace
+
+int step_batched(Ctx& c, int n) {
+  BatchedState& B = *c.bs;
+  if (!B.graphs_valid || B.graph_trace != c.d_trace) {
+    int r = build_graphs(c);
+    if (r) return r;
+  }
+  for (int s = 0; s < n; ++s) {
+    const int par = c.time_step & 1;
+    hipError_t e = hipGraphLaunch(c.graph_exec[par], c.stream);
+    if (e != hipSuccess) return hip_fail(e, "hipGraphLaunch");
+    c.time_step++;
+  }
+  return ARD_OK;
+}
+
+// Dense pressure of a DCT partition from its current modes: the three
+// DCT-III passes of the plain form, on demand.
+int batched_pressure_field(Ctx& c, int part, const float** out) {
+  BatchedState& B = *c.bs;
+  const PartGeom& p = c.topo.parts[part];
+  const size_t N = size_t(p.cells());
+  int r;
+  if (B.dense_n < N) {
+    for (int k = 0; k < 3; ++k)
+      if ((r = c.dalloc((void**)&B.dense[k], N * 4))) return r;
+    B.dense_n = N;
+    if (!B.d_dense && (r = c.dalloc((void**)&B.d_dense, 3 * sizeof(GemmDesc)))) return r;
+  }
+  const float *fw, *iw, *fh, *ih, *fd, *id;
+  if ((r = c.cos_tables(p.w, &fw, &iw)) || (r = c.cos_tables(p.h, &fh, &ih)) || (r = c.cos_tables(p.d, &fd, &id)))
+    return r;
+  const long long W = p.w, H = p.h, D = p.d;
+  const float* M = B.parts[part].M[c.time_step & 1];
+  GemmDesc g[3];
+  g[0] = gemm_k_contig(iw, p.w, M, B.dense[0], int(H * D), W);
+  // along y per z-plane: n = (x, z)
+  g[1] = GemmDesc{};
+  g[1].A = ih; g[1].B = B.dense[0]; g[1].C = B.dense[1];
+  g[1].M = p.h; g[1].K = p.h; g[1].N = int(W * D); g[1].N1 = p.w;
+  g[1].bK = W; g[1].b1 = 1; g[1].b2 = W * H; g[1].cM = W; g[1].c1 = 1; g[1].c2 = W * H; g[1].alpha = 1.f;
+  g[1].tiles_n = int(ceil_div(g[1].N, 64));
+  g[2] = GemmDesc{};
+  g[2].A = id; g[2].B = B.dense[1]; g[2].C = B.dense[2];
+  g[2].M = p.d; g[2].K = p.d; g[2].N = int(W * H); g[2].N1 = g[2].N;
+  g[2].bK = W * H; g[2].b1 = 1; g[2].cM = W * H; g[2].c1 = 1; g[2].alpha = 1.f;
+  g[2].tiles_n = int(ceil_div(g[2].N, 64));
+  for (auto& x : g) x.tile0 = 0;
+  if ((r = c.upload(B.d_dense, g, sizeof g))) return r;
+  r = launch_gemm(B.d_dense + 0, 1, int(ceil_div(g[0].M, 64)) * g[0].tiles_n, true, true, 0, c.stream);
+  r |= launch_gemm(B.d_dense + 1, 1, int(ceil_div(g[1].M, 64)) * g[1].tiles_n, false, false, 0, c.stream);
+  r |= launch_gemm(B.d_dense + 2, 1, int(ceil_div(g[2].M, 64)) * g[2].tiles_n, false, false, 0, c.stream);
+  if (r) return set_error(ARD_ERR_HIP, "dense evaluation launch");
+  *out = B.dense[2];
+  return ARD_OK;
+}
+
+// Partition::set_force on a DCT partition: a face-slab cell is written in
+// place; an interior cell becomes a constant point force (forces of DCT
+// partitions persist, ARD/dct_partition.cpp:48-73).
 int batched_set_force(Ctx& c, int part, int x, int y, int z, float f) {
-  return set_error(ARD_ERR_UNSUPPORTED, "batched form not built yet");
+  BatchedState& B = *c.bs;
+  const PartGeom& p = c.topo.parts[part];
+  long long off;
+  const int w = shell_of(p.w, p.h, p.d, x, y, z, &off);
+  if (w >= 0) return c.upload(B.parts[part].F[w] + off, &f, 4);
+  for (auto& s : B.srcs)
+    if (s.part == part && s.x == x && s.y == y && s.z == z) {
+      // a source cell: the source overwrites it before the next update anyway
+      if (s.kind != 2) return ARD_OK;
+      s.value = f;
+      int r = build_sources(c);
+      if (r) return r;
+      B.graphs_valid = false;
+      return build_modal(c);
+    }
+  PointSrc ps;
+  ps.part = part; ps.x = x; ps.y = y; ps.z = z; ps.kind = 2; ps.value = f;
+  B.srcs.push_back(ps);
+  int r = build_sources(c);
+  if (r) return r;
+  B.graphs_valid = false;
+  return build_modal(c);
+}
+
+void destroy_batched(Ctx& c) {
+  delete c.bs;
+  c.bs = nullptr;
 }
 
 }  // namespace ard

@@ -1,0 +1,336 @@
+// Kernels of the batched face-shell form of the ARD step.
+//
+// A DCT partition's force field is nonzero only within 3 cells of its faces
+// (interface forcing, ARD/boundary.cpp:30-173) and at source cells
+// (ARD/partition.cpp:187-197); its pressure is read only there and at the
+// recorder cells (ARD/recorder.cpp:40-57). The 3D DCT-II of the force and the
+// DCT-III to pressure (ARD/dct_volume.cpp:38-68) are therefore evaluated as
+//   * 2D transforms of the six 3-cell face slabs (small GEMMs), and
+//   * one streaming pass over the modes (k_modal) that expands the face
+//     transforms into the force spectrum, applies the modal update
+//     (ARD/dct_partition.cpp:66-70) and reduces the new modes back onto the
+//     face slabs and recorder points,
+// which computes the same quantities as the dense transforms with O(1) work
+// per cell instead of O(W+H+D).
+//
+// Notation for one partition (W x H x D cells, modes stored times the inverse
+// scale 1/sqrt(2N)): c_n(k, x) = cos(pi k (x + 1/2) / n), e(k) = 1 for k = 0
+// else 2. For x in {0,1,2} the mirror cell n-1-x has c_n(k, n-1-x) =
+// (-1)^k c_n(k, x), so each face pair is carried as a sum (even k) and a
+// difference (odd k) plane.
+#include <type_traits>
+
+#include "ard_batched.h"
+
+namespace ard {
+
+namespace {
+__device__ __forceinline__ int find_desc_ll(const long long* __restrict__ starts, int n, long long key) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (starts[mid] <= key) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Fold: face pair (x, n-1-x) -> sum / difference planes, x = 0..2.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_fold(const FoldDesc* __restrict__ d, const long long* __restrict__ starts,
+                                              int n) {
+  const long long e = (long long)blockIdx.x * 256 + threadIdx.x;
+  const int i = find_desc_ll(starts, n, e);
+  const FoldDesc& F = d[i];
+  const long long le = e - starts[i];
+  if (le >= 3 * F.plane) return;
+  const int q = int(le / F.plane);
+  const long long o = le % F.plane;
+  const float a = F.src[q * F.plane + o];
+  const float b = F.src[(5 - q) * F.plane + o];
+  F.dst[q * F.plane + o] = a + b;
+  F.dst[(3 + q) * F.plane + o] = a - b;
+}
+
+// ---------------------------------------------------------------------------
+// Unfold: sum the per-chunk partial reductions and turn the (even, odd) pair
+// back into the face pair: x -> even + odd, n-1-x -> even - odd.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_unfold(const UnfoldDesc* __restrict__ d,
+                                                const long long* __restrict__ starts, int n) {
+  const long long e = (long long)blockIdx.x * 256 + threadIdx.x;
+  const int i = find_desc_ll(starts, n, e);
+  const UnfoldDesc& U = d[i];
+  const long long le = e - starts[i];
+  if (le >= 3 * U.plane) return;
+  const int q = int(le / U.plane);
+  const long long o = le % U.plane;
+  float ev = 0.f, od = 0.f;
+  for (int c = 0; c < U.nchunk; ++c) {
+    const float* p = U.part + (long long)c * 6 * U.plane;
+    ev += p[q * U.plane + o];
+    od += p[(3 + q) * U.plane + o];
+  }
+  U.dst[q * U.plane + o] = ev + od;
+  U.dst[(5 - q) * U.plane + o] = ev - od;
+}
+
+// ---------------------------------------------------------------------------
+// Source samples: GaussianSource (ARD/gaussian_source.cpp:16-20), a waveform,
+// or a constant (a force set through set_force at an interior cell). Shell
+// cells receive the value in the face slab; interior cells in a value slot
+// read by k_modal.
+// ---------------------------------------------------------------------------
+__global__ void k_bsources(const BSourceDesc* __restrict__ s, int n, const int* __restrict__ d_step,
+                           float courant) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int t = *d_step;
+  float v;
+  if (s[i].kind == 0) {
+    const float x = 3.14159265358979f * ((2.f * courant * float(t)) / 6.f - 2.f);
+    v = 1e9f * expf(-(x * x));
+  } else if (s[i].kind == 1) {
+    v = t < s[i].n ? s[i].wave[t] : 0.f;
+  } else {
+    v = s[i].value;
+  }
+  *s[i].dst = v;
+}
+
+// ---------------------------------------------------------------------------
+// The modal pass. Block = 4 waves; lane <-> kx (64 per block), wave w owns
+// rows ky = ky0 + w + 4 r (r < RPW), the block walks kz over its chunk.
+// Per mode:   Fh = sum_i Ex[kx][i] Gx[par kx][i][kz][ky] + (y, z alike) + sources
+//             M+ = a M - b M- + g Fh           (stored over M-)
+// and reduces M+ onto the face planes and recorders:
+//   Tx[par kx][i][kz][ky] += Rx[kx][i] M+   (across lanes)
+//   Ty[par ky][j][kz][kx] += Ry[ky][j] M+   (across rows and waves)
+//   Tz[par kz][l][ky][kx] += Rz[kz][l] M+   (along the kz walk)
+// Partial sums go to per-chunk slots, summed by k_unfold.
+// ---------------------------------------------------------------------------
+template <int RPW>
+__global__ __launch_bounds__(256) void k_modal(const ModalDesc* __restrict__ descs,
+                                               const long long* __restrict__ starts, int ndesc, int par) {
+  __shared__ float s_ty[4][3][64];
+  __shared__ float s_rec[4][ARD_MAX_REC_PER_PART];
+  const int bid = blockIdx.x;
+  const int di = find_desc_ll(starts, ndesc, bid);
+  const ModalDesc& P = descs[di];
+  const int lb = bid - int(starts[di]);
+  const int bx = lb % P.nbx;
+  const int by = (lb / P.nbx) % P.nby;
+  const int bz = lb / (P.nbx * P.nby);
+  const int W = P.W, H = P.H, D = P.D;
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
+  const int kx = bx * 64 + lane;
+  const bool xok = kx < W;
+  const int kxc = xok ? kx : W - 1;
+  const int px = kx & 1;
+  const int ky0 = by * (4 * RPW) + wave;
+  const int py = ky0 & 1;  // every row of this wave has the parity of ky0
+  const int kz0 = bz * P.kz_chunk;  // kz_chunk is even, so kz0 is even
+  const int kz1 = min(D, kz0 + P.kz_chunk);
+  const long long HW = (long long)H * W;
+  const long long DH = (long long)D * H, DW = (long long)D * W;
+
+  float ex[3], rx[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    ex[i] = P.Ex[kxc * 3 + i];
+    rx[i] = xok ? P.Rx[kxc * 3 + i] : 0.f;
+  }
+  // per-row constants: force expansion along z (both kz parities) and the
+  // z-reduction accumulators (both parities)
+  float gz[RPW][6], tz[RPW][6];
+#pragma unroll
+  for (int r = 0; r < RPW; ++r) {
+    const int ky = ky0 + 4 * r;
+    const bool ok = ky < H && xok;
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+      gz[r][q] = ok ? P.Gz[q * HW + (long long)ky * W + kx] : 0.f;
+      tz[r][q] = 0.f;
+    }
+  }
+  float rec_acc[ARD_MAX_REC_PER_PART];
+#pragma unroll
+  for (int q = 0; q < ARD_MAX_REC_PER_PART; ++q) rec_acc[q] = 0.f;
+  const int nrec = P.nrec;
+  const int nsrc = P.nsrc;
+
+  const float* __restrict__ Mc = P.M[par];
+  float* __restrict__ Mp = P.M[par ^ 1];
+  const float bdamp = 0.999f;
+
+  auto body = [&](const int kz, auto pzc) {
+    constexpr int PZ = decltype(pzc)::value;
+    const float ez0 = P.Ez[kz * 3 + 0], ez1 = P.Ez[kz * 3 + 1], ez2 = P.Ez[kz * 3 + 2];
+    const float rz0 = P.Rz[kz * 3 + 0], rz1 = P.Rz[kz * 3 + 1], rz2 = P.Rz[kz * 3 + 2];
+    // y-face force spectrum at (kx, kz) for this wave's ky parity
+    float gy[3];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) gy[j] = xok ? P.Gy[(py * 3 + j) * DW + (long long)kz * W + kx] : 0.f;
+    float ty[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+    for (int r = 0; r < RPW; ++r) {
+      const int ky = ky0 + 4 * r;
+      if (ky >= H) break;  // uniform across the wave
+      const float ey0 = P.Ey[ky * 3 + 0], ey1 = P.Ey[ky * 3 + 1], ey2 = P.Ey[ky * 3 + 2];
+      const float ry0 = P.Ry[ky * 3 + 0], ry1 = P.Ry[ky * 3 + 1], ry2 = P.Ry[ky * 3 + 2];
+      // x-face force spectrum at (ky, kz): uniform across lanes, parity per lane
+      const long long gxo = (long long)kz * H + ky;
+      float fh = 0.f;
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        const float gs = P.Gx[i * DH + gxo];
+        const float gd = P.Gx[(3 + i) * DH + gxo];
+        fh = fmaf(ex[i], px ? gd : gs, fh);
+      }
+      fh = fmaf(ey0, gy[0], fh);
+      fh = fmaf(ey1, gy[1], fh);
+      fh = fmaf(ey2, gy[2], fh);
+      fh = fmaf(ez0, gz[r][PZ * 3 + 0], fh);
+      fh = fmaf(ez1, gz[r][PZ * 3 + 1], fh);
+      fh = fmaf(ez2, gz[r][PZ * 3 + 2], fh);
+      for (int s = 0; s < nsrc; ++s) {
+        const SrcTab& T = P.src[s];
+        fh = fmaf(*T.val * T.fy[ky] * T.fz[kz], T.fx[kxc], fh);
+      }
+      const long long o = (long long)kz * HW + (long long)ky * W + kx;
+      float mn = 0.f;
+      if (xok) {
+        mn = P.a[o] * Mc[o] - bdamp * Mp[o] + P.g[o] * fh;
+        Mp[o] = mn;
+      }
+      tz[r][PZ * 3 + 0] = fmaf(rz0, mn, tz[r][PZ * 3 + 0]);
+      tz[r][PZ * 3 + 1] = fmaf(rz1, mn, tz[r][PZ * 3 + 1]);
+      tz[r][PZ * 3 + 2] = fmaf(rz2, mn, tz[r][PZ * 3 + 2]);
+      ty[0] = fmaf(ry0, mn, ty[0]);
+      ty[1] = fmaf(ry1, mn, ty[1]);
+      ty[2] = fmaf(ry2, mn, ty[2]);
+#pragma unroll
+      for (int q = 0; q < ARD_MAX_REC_PER_PART; ++q) {
+        if (q < nrec) {
+          const RecTab& T = P.rec[q];
+          rec_acc[q] = fmaf(T.fx[kxc] * T.fy[ky] * T.fz[kz], mn, rec_acc[q]);
+        }
+      }
+      // x reduction across lanes, keeping even and odd kx apart
+      float v0 = rx[0] * mn, v1 = rx[1] * mn, v2 = rx[2] * mn;
+#pragma unroll
+      for (int m = 2; m < 64; m <<= 1) {
+        v0 += __shfl_xor(v0, m);
+        v1 += __shfl_xor(v1, m);
+        v2 += __shfl_xor(v2, m);
+      }
+      if (lane < 2) {
+        float* tx = P.Tx + (long long)bx * 6 * DH;
+        tx[(lane * 3 + 0) * DH + gxo] = v0;
+        tx[(lane * 3 + 1) * DH + gxo] = v1;
+        tx[(lane * 3 + 2) * DH + gxo] = v2;
+      }
+    }
+    // y reduction: waves 0,2 hold even ky, waves 1,3 odd ky
+#pragma unroll
+    for (int j = 0; j < 3; ++j) s_ty[wave][j][lane] = ty[j];
+    __syncthreads();
+    if (wave < 2 && xok) {
+      float* typ = P.Ty + (long long)by * 6 * DW;
+#pragma unroll
+      for (int j = 0; j < 3; ++j)
+        typ[(wave * 3 + j) * DW + (long long)kz * W + kx] = s_ty[wave][j][lane] + s_ty[wave + 2][j][lane];
+    }
+    __syncthreads();
+  };
+  for (int kz = kz0; kz < kz1; kz += 2) {
+    body(kz, std::integral_constant<int, 0>());
+    if (kz + 1 < kz1) body(kz + 1, std::integral_constant<int, 1>());
+  }
+  // z reduction partials of this kz chunk
+  float* tzp = P.Tz + (long long)bz * 6 * HW;
+#pragma unroll
+  for (int r = 0; r < RPW; ++r) {
+    const int ky = ky0 + 4 * r;
+    if (ky >= H || !xok) continue;
+#pragma unroll
+    for (int q = 0; q < 6; ++q) tzp[q * HW + (long long)ky * W + kx] = tz[r][q];
+  }
+  // recorder partials of this block
+  if (nrec > 0) {
+#pragma unroll
+    for (int q = 0; q < ARD_MAX_REC_PER_PART; ++q) {
+      if (q < nrec) {
+        float v = rec_acc[q];
+#pragma unroll
+        for (int m = 1; m < 64; m <<= 1) v += __shfl_xor(v, m);
+        if (lane == 0) s_rec[wave][q] = v;
+      }
+    }
+    __syncthreads();
+    if (threadIdx.x < nrec)
+      P.rec_part[(long long)lb * ARD_MAX_REC_PER_PART + threadIdx.x] =
+          s_rec[0][threadIdx.x] + s_rec[1][threadIdx.x] + s_rec[2][threadIdx.x] + s_rec[3][threadIdx.x];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Recorder samples from the block partials, then the step counter.
+// ---------------------------------------------------------------------------
+__global__ void k_brecord(const BRecDesc* __restrict__ r, int n, float* __restrict__ trace, int cap,
+                          int* __restrict__ d_step) {
+  const int t = *d_step;
+  __syncthreads();
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    float s = 0.f;
+    for (int b = 0; b < r[i].nblocks; ++b) s += r[i].part[(long long)b * ARD_MAX_REC_PER_PART];
+    if (t < cap) trace[(long long)t * n + i] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) *d_step = t + 1;
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+static int grid_of(long long total) { return int((total + 255) / 256); }
+
+int launch_fold(const FoldDesc* d, const long long* starts, int n, long long total, hipStream_t s) {
+  if (total <= 0) return 0;
+  hipLaunchKernelGGL(k_fold, dim3(grid_of(total)), dim3(256), 0, s, d, starts, n);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+int launch_unfold(const UnfoldDesc* d, const long long* starts, int n, long long total, hipStream_t s) {
+  if (total <= 0) return 0;
+  hipLaunchKernelGGL(k_unfold, dim3(grid_of(total)), dim3(256), 0, s, d, starts, n);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+int launch_bsources(const BSourceDesc* d, int n, const int* d_step, float courant, hipStream_t s) {
+  if (n <= 0) return 0;
+  hipLaunchKernelGGL(k_bsources, dim3((n + 63) / 64), dim3(64), 0, s, d, n, d_step, courant);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+int launch_modal(int rpw, const ModalDesc* d, const long long* starts, int n, long long blocks, int par,
+                 hipStream_t s) {
+  if (blocks <= 0) return 0;
+  switch (rpw) {
+    case 4: hipLaunchKernelGGL(k_modal<4>, dim3(unsigned(blocks)), dim3(256), 0, s, d, starts, n, par); break;
+    case 5: hipLaunchKernelGGL(k_modal<5>, dim3(unsigned(blocks)), dim3(256), 0, s, d, starts, n, par); break;
+    case 8: hipLaunchKernelGGL(k_modal<8>, dim3(unsigned(blocks)), dim3(256), 0, s, d, starts, n, par); break;
+    default: return -1;
+  }
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+int launch_brecord(const BRecDesc* d, int n, float* trace, int cap, int* d_step, hipStream_t s) {
+  hipLaunchKernelGGL(k_brecord, dim3(1), dim3(256), 0, s, d, n, trace, cap, d_step);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+}  // namespace ard

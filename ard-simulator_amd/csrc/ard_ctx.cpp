@@ -43,6 +43,7 @@ Ctx::~Ctx() {
   if (device >= 0) hipSetDevice(device);
   if (graph_exec[0]) hipGraphExecDestroy(graph_exec[0]);
   if (graph_exec[1]) hipGraphExecDestroy(graph_exec[1]);
+  destroy_batched(*this);
   for (void* p : allocs) hipFree(p);
   if (stream) hipStreamDestroy(stream);
 }

@@ -105,6 +105,7 @@ int finalize_batched(Ctx& c);
 int step_batched(Ctx& c, int n);
 int batched_pressure_field(Ctx& c, int part, const float** out);
 int batched_set_force(Ctx& c, int part, int x, int y, int z, float f);
+void destroy_batched(Ctx& c);
 
 // field access (either form)
 int pressure_field_ptr(Ctx& c, int part, const float** out);

@@ -1,4 +1,5 @@
-"""GPU parity of the plain form (per-partition launches) against the oracle.
+"""GPU parity of both forms (plain per-partition launches, batched face-shell
+graph) against the oracle.
 
 Tolerance: recorder traces within rel-L2 1e-4 of the FP32 oracle's golden
 traces (the bar stated for this path); fields within 1e-4 relative to their
@@ -23,10 +24,14 @@ def rel_l2(a, b):
     return float(np.linalg.norm(a - b) / np.linalg.norm(b))
 
 
-def test_two_rooms_trace_matches_golden():
+FORMS = [A.FORM_PLAIN, A.FORM_BATCHED]
+
+
+@pytest.mark.parametrize("form", FORMS)
+def test_two_rooms_trace_matches_golden(form):
     sc = S.two_rooms()
-    print("case two_rooms plain", flush=True)
-    sim = A.Simulation(sc, form=A.FORM_PLAIN)
+    print("case two_rooms form", form, flush=True)
+    sim = A.Simulation(sc, form=form)
     sim.update(sc.steps)
     tr = sim.recorders()
     g32 = np.load(os.path.join(GOLD, "two_rooms_fp32.npy"))
@@ -36,10 +41,11 @@ def test_two_rooms_trace_matches_golden():
     assert e32 <= TOL and e64 <= TOL
 
 
-def test_scene3_trace_matches_golden():
+@pytest.mark.parametrize("form", FORMS)
+def test_scene3_trace_matches_golden(form):
     sc = S.scene3()
-    print("case scene3 plain", flush=True)
-    sim = A.Simulation(sc, form=A.FORM_PLAIN)
+    print("case scene3 form", form, flush=True)
+    sim = A.Simulation(sc, form=form)
     sim.update(sc.steps)
     tr = sim.recorders()
     g32 = np.load(os.path.join(GOLD, "scene3_fp32.npy"))
@@ -59,7 +65,7 @@ def small_scene():
     return sc
 
 
-@pytest.mark.parametrize("form", [A.FORM_PLAIN])
+@pytest.mark.parametrize("form", FORMS)
 def test_all_fields_match_oracle(form):
     sc = small_scene()
     print("case small_scene fields form", form, flush=True)
@@ -79,10 +85,11 @@ def test_all_fields_match_oracle(form):
         assert err < TOL, (i, info, err)
 
 
-def test_set_force_and_get_pressure():
+@pytest.mark.parametrize("form", FORMS)
+def test_set_force_and_get_pressure(form):
     sc = small_scene()
     sc.sources = []
-    sim = A.Simulation(sc, form=A.FORM_PLAIN)
+    sim = A.Simulation(sc, form=form)
     o = O.from_scene(sc)
     # a persistent force at an interior cell of box 1 (ARD/dct_partition.cpp: never cleared)
     sim.set_force(1, 6, 10, 5, 1e6)
@@ -95,3 +102,23 @@ def test_set_force_and_get_pressure():
     got = sim.get_pressure(1, 6, 10, 5)
     assert abs(got - ref[5, 10, 6]) <= TOL * np.abs(ref).max()
     assert sim.time_step == 15
+
+
+def test_hall_batched_matches_plain():
+    """Full-size hall (74M DCT cells): the two forms agree step by step."""
+    sc = S.hall()
+    n = 30
+    print("case hall plain vs batched, %d steps" % n, flush=True)
+    tr = {}
+    fields = {}
+    for form in FORMS:
+        sim = A.Simulation(sc, form=form)
+        sim.update(n)
+        tr[form] = sim.recorders()
+        fields[form] = sim.pressure_field(3)
+        sim.close()
+    e = rel_l2(tr[A.FORM_BATCHED], tr[A.FORM_PLAIN])
+    ef = rel_l2(fields[A.FORM_BATCHED], fields[A.FORM_PLAIN])
+    print("hall batched vs plain: trace rel-L2 %.3e, field rel-L2 %.3e" % (e, ef))
+    assert np.abs(tr[A.FORM_PLAIN]).max() > 0
+    assert e <= TOL and ef <= TOL
