@@ -25,8 +25,11 @@ EXPORTED = [
     "ard_add_pml_partition", "ard_add_boundary", "ard_build_topology", "ard_add_source", "ard_add_recorder",
     "ard_finalize", "ard_num_partitions", "ard_num_boundaries", "ard_num_recorders", "ard_partition_info",
     "ard_boundary_info", "ard_recorder_partition", "ard_step", "ard_time_step", "ard_fetch_recorders",
-    "ard_copy_pressure_field", "ard_get_pressure", "ard_set_force", "ard_synchronize",
+    "ard_copy_pressure_field", "ard_get_pressure", "ard_set_force", "ard_synchronize", "ard_set_kernel_timing",
+    "ard_kernel_times",
 ]
+PHASES = ["sources", "pml", "fold", "face_fwd1", "face_fwd2", "modal", "unfold", "face_inv1", "face_inv2",
+          "interface", "record", "dct_fwd", "dct_inv"]
 
 
 class ArdError(RuntimeError):
@@ -75,6 +78,8 @@ def lib():
         L.ard_copy_pressure_field.argtypes = [vp, i, fp]
         L.ard_get_pressure.argtypes = [vp, i, i, i, i, fp]
         L.ard_set_force.argtypes = [vp, i, i, i, i, f]
+        L.ard_set_kernel_timing.argtypes = [vp, i]
+        L.ard_kernel_times.argtypes = [vp, fp, ip, i]
         _lib = L
     return _lib
 
@@ -184,3 +189,14 @@ class Simulation:
 
     def set_force(self, part, x, y, z, f):
         _check(self.L.ard_set_force(self.ctx, part, x, y, z, f))
+
+    def set_kernel_timing(self, on):
+        _check(self.L.ard_set_kernel_timing(self.ctx, int(on)))
+
+    def kernel_times(self):
+        """{phase: (average ms per launch, launches)} since timing was switched on."""
+        n = len(PHASES)
+        ms = np.zeros(n, dtype=np.float32)
+        cnt = np.zeros(n, dtype=np.int32)
+        _check(self.L.ard_kernel_times(self.ctx, _fp(ms), cnt.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), n))
+        return {PHASES[i]: (float(ms[i]), int(cnt[i])) for i in range(n) if cnt[i]}

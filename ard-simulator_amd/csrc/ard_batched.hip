@@ -519,21 +519,30 @@ int finalize_batched(Ctx& c) {
 
 namespace {
 
-int enqueue_step(Ctx& c, int par) {
+int enqueue_step(Ctx& c, int par, PhaseTimer* T) {
   BatchedState& B = *c.bs;
+  hipStream_t st = c.stream;
   int r = 0;
-  r |= launch_bsources(B.d_src, B.nsrc, c.d_step, c.courant, c.stream);
-  r |= launch_pml(B.d_pml, B.npml, B.pml_blocks, par, c.prm.h, c.prm.dt, c.prm.c0, c.stream);
-  r |= launch_fold(B.d_fold, B.d_fold_start, B.nfold, B.fold_total, c.stream);
-  r |= launch_gemm(B.d_gemm[0], B.ngemm[0], B.gemm_tiles[0], true, true, 0, c.stream);
-  r |= launch_gemm(B.d_gemm[1], B.ngemm[1], B.gemm_tiles[1], false, false, 0, c.stream);
-  for (auto& g : B.modal) r |= launch_modal(g.rpw, g.d_desc[par], g.d_start, g.n, g.blocks, par, c.stream);
-  r |= launch_unfold(B.d_unfold, B.d_unfold_start, B.nunfold, B.unfold_total, c.stream);
-  r |= launch_gemm(B.d_gemm[2], B.ngemm[2], B.gemm_tiles[2], true, true, 0, c.stream);
-  r |= launch_gemm(B.d_gemm[3], B.ngemm[3], B.gemm_tiles[3], false, false, 0, c.stream);
-  // interface reads the PML pressure just written: buffer par ^ 1
-  r |= launch_interface(B.d_bnd, B.nbnd, B.d_parts, par ^ 1, B.bnd_total, c.stream);
-  r |= launch_brecord(B.d_rec, int(c.topo.recorders.size()), c.d_trace, c.trace_cap, c.d_step, c.stream);
+  auto phase = [&](int ph, auto&& launch) {
+    const int ev = T ? T->begin(st, ph) : -1;
+    r |= launch();
+    if (T) T->end(st, ev);
+  };
+  phase(ARD_PH_SOURCES, [&] { return launch_bsources(B.d_src, B.nsrc, c.d_step, c.courant, st); });
+  phase(ARD_PH_PML, [&] { return launch_pml(B.d_pml, B.npml, B.pml_blocks, par, c.prm.h, c.prm.dt, c.prm.c0, st); });
+  phase(ARD_PH_FOLD, [&] { return launch_fold(B.d_fold, B.d_fold_start, B.nfold, B.fold_total, st); });
+  phase(ARD_PH_FACE_FWD1, [&] { return launch_gemm(B.d_gemm[0], B.ngemm[0], B.gemm_tiles[0], true, true, 0, st); });
+  phase(ARD_PH_FACE_FWD2, [&] { return launch_gemm(B.d_gemm[1], B.ngemm[1], B.gemm_tiles[1], false, false, 0, st); });
+  for (auto& g : B.modal)
+    phase(ARD_PH_MODAL, [&] { return launch_modal(g.rpw, g.d_desc[par], g.d_start, g.n, g.blocks, par, st); });
+  phase(ARD_PH_UNFOLD, [&] { return launch_unfold(B.d_unfold, B.d_unfold_start, B.nunfold, B.unfold_total, st); });
+  phase(ARD_PH_FACE_INV1, [&] { return launch_gemm(B.d_gemm[2], B.ngemm[2], B.gemm_tiles[2], true, true, 0, st); });
+  phase(ARD_PH_FACE_INV2, [&] { return launch_gemm(B.d_gemm[3], B.ngemm[3], B.gemm_tiles[3], false, false, 0, st); });
+  // the interface reads the PML pressure just written: buffer par ^ 1
+  phase(ARD_PH_INTERFACE, [&] { return launch_interface(B.d_bnd, B.nbnd, B.d_parts, par ^ 1, B.bnd_total, st); });
+  phase(ARD_PH_RECORD, [&] {
+    return launch_brecord(B.d_rec, int(c.topo.recorders.size()), c.d_trace, c.trace_cap, c.d_step, st);
+  });
   return r ? set_error(ARD_ERR_HIP, "batched step launch") : ARD_OK;
 }
 
@@ -546,7 +555,7 @@ int build_graphs(Ctx& c) {
     hipGraph_t g;
     hipError_t e = hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal);
     if (e != hipSuccess) return hip_fail(e, "hipStreamBeginCapture");
-    int r = enqueue_step(c, par);
+    int r = enqueue_step(c, par, nullptr);
     e = hipStreamEndCapture(c.stream, &g);
     if (r) return r;
     if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
@@ -563,6 +572,15 @@ int build_graphs(Ctx& c) {
 
 int step_batched(Ctx& c, int n) {
   BatchedState& B = *c.bs;
+  if (c.timer.on) {  // direct launches between events
+    for (int s = 0; s < n; ++s) {
+      int r = enqueue_step(c, c.time_step & 1, &c.timer);
+      if (r) return r;
+      c.time_step++;
+      if (c.timer.used > 2048 && (r = c.timer.collect())) return r;
+    }
+    return ARD_OK;
+  }
   if (!B.graphs_valid || B.graph_trace != c.d_trace) {
     int r = build_graphs(c);
     if (r) return r;

@@ -39,8 +39,50 @@ int hip_fail(hipError_t e, const char* what) {
 
 int set_error(int code, const std::string& msg) { return fail(code, msg); }
 
+int PhaseTimer::begin(hipStream_t s, int phase) {
+  if (!on) return -1;
+  if (used + 2 > pool.size()) {
+    for (int k = 0; k < 64; ++k) {
+      hipEvent_t e;
+      if (hipEventCreate(&e) != hipSuccess) return -1;
+      pool.push_back(e);
+    }
+  }
+  const int pair = int(used / 2);
+  hipEventRecord(pool[used], s);
+  used += 2;
+  phase_of.resize(pair + 1);
+  phase_of[pair] = phase;
+  return pair;
+}
+
+void PhaseTimer::end(hipStream_t s, int pair) {
+  if (pair >= 0) hipEventRecord(pool[2 * size_t(pair) + 1], s);
+}
+
+int PhaseTimer::collect() {
+  for (size_t i = 0; i + 1 < used; i += 2) {
+    hipError_t e = hipEventSynchronize(pool[i + 1]);
+    if (e != hipSuccess) return hip_fail(e, "hipEventSynchronize");
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, pool[i], pool[i + 1]);
+    const int ph = phase_of[i / 2];
+    total_ms[ph] += ms;
+    count[ph] += 1;
+  }
+  used = 0;
+  return ARD_OK;
+}
+
+void PhaseTimer::release() {
+  for (hipEvent_t e : pool) hipEventDestroy(e);
+  pool.clear();
+  used = 0;
+}
+
 Ctx::~Ctx() {
   if (device >= 0) hipSetDevice(device);
+  timer.release();
   if (graph_exec[0]) hipGraphExecDestroy(graph_exec[0]);
   if (graph_exec[1]) hipGraphExecDestroy(graph_exec[1]);
   destroy_batched(*this);
@@ -315,6 +357,35 @@ int ard_step(ard_ctx* ctx, int n) {
   if (r != ARD_OK) return r;
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return hip_fail(e, "kernel launch");
+  return ARD_OK;
+}
+
+int ard_set_kernel_timing(ard_ctx* ctx, int on) {
+  Ctx* c = reinterpret_cast<Ctx*>(ctx);
+  if (!c) return fail(ARD_ERR_INVALID, "null context");
+  if (!c->finalized) return fail(ARD_ERR_STATE, "not finalized");
+  hipSetDevice(c->device);
+  int r = c->timer.collect();
+  if (r) return r;
+  c->timer.on = on != 0;
+  for (int i = 0; i < ARD_NUM_PHASES; ++i) {
+    c->timer.total_ms[i] = 0;
+    c->timer.count[i] = 0;
+  }
+  return ARD_OK;
+}
+
+int ard_kernel_times(ard_ctx* ctx, float* avg_ms, int* launches, int nphase) {
+  Ctx* c = reinterpret_cast<Ctx*>(ctx);
+  if (!c || !avg_ms || !launches) return fail(ARD_ERR_INVALID, "null argument");
+  if (nphase < 0 || nphase > ARD_NUM_PHASES) return fail(ARD_ERR_INVALID, "bad phase count");
+  if (c->device >= 0) hipSetDevice(c->device);
+  int r = c->timer.collect();
+  if (r) return r;
+  for (int i = 0; i < nphase; ++i) {
+    launches[i] = int(c->timer.count[i]);
+    avg_ms[i] = c->timer.count[i] ? float(c->timer.total_ms[i] / c->timer.count[i]) : 0.f;
+  }
   return ARD_OK;
 }
 

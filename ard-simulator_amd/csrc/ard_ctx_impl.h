@@ -39,6 +39,20 @@ struct PmlBuffers {
   int desc = -1;  // index into the PML descriptor table
 };
 
+// HIP-event timing of the step phases (ard_set_kernel_timing).
+struct PhaseTimer {
+  bool on = false;
+  std::vector<hipEvent_t> pool;
+  size_t used = 0;
+  std::vector<int> phase_of;  // phase of the event pair (2i, 2i+1)
+  double total_ms[ARD_NUM_PHASES] = {};
+  long long count[ARD_NUM_PHASES] = {};
+  int begin(hipStream_t s, int phase);  // records the start event; returns the pair index or -1
+  void end(hipStream_t s, int pair);
+  int collect();                        // synchronises and accumulates
+  void release();
+};
+
 struct Ctx {
   int device = -1;
   hipStream_t stream = nullptr;
@@ -75,6 +89,8 @@ struct Ctx {
   SourceDesc* d_src = nullptr;
   int nsrc = 0;
   RecDesc* d_rec = nullptr;
+
+  PhaseTimer timer;
 
   // batched form
   BatchedState* bs = nullptr;

@@ -298,31 +298,50 @@ int finalize_plain(Ctx& c) {
 // writer wins on shared cells), then the recorder sample.
 int step_plain(Ctx& c, int n) {
   const int nr = int(c.topo.recorders.size());
+  PhaseTimer& T = c.timer;
   for (int s = 0; s < n; ++s) {
     const int par = c.time_step & 1;
+    int ev = T.begin(c.stream, ARD_PH_SOURCES);
     if (launch_sources(c.d_src, c.nsrc, c.d_step, c.courant, c.stream)) return set_error(ARD_ERR_HIP, "sources");
+    T.end(c.stream, ev);
     for (size_t i = 0; i < c.dct.size(); ++i) {
       const DctPlain& D = c.dct[i];
       if (!D.F) continue;
       const GemmDesc* g = c.d_gemm + D.gemm0 + par * 6;
       int r = 0;
+      ev = T.begin(c.stream, ARD_PH_DCT_FWD);
       r |= launch_gemm(g + 0, 1, D.tiles[0], true, true, 0, c.stream);
       r |= launch_gemm(g + 1, 1, D.tiles[1], false, false, 0, c.stream);
       r |= launch_gemm(g + 2, 1, D.tiles[2], false, false, 1, c.stream);
+      T.end(c.stream, ev);
+      ev = T.begin(c.stream, ARD_PH_DCT_INV);
       r |= launch_gemm(g + 3, 1, D.tiles[3], true, true, 0, c.stream);
       r |= launch_gemm(g + 4, 1, D.tiles[4], false, false, 0, c.stream);
       r |= launch_gemm(g + 5, 1, D.tiles[5], false, false, 0, c.stream);
+      T.end(c.stream, ev);
       if (r) return set_error(ARD_ERR_HIP, "DCT pass launch");
     }
-    for (size_t i = 0; i < c.h_pml.size(); ++i)
+    for (size_t i = 0; i < c.h_pml.size(); ++i) {
+      ev = T.begin(c.stream, ARD_PH_PML);
       if (launch_pml(c.d_pml + i, 1, c.pml_blocks[i], par, c.prm.h, c.prm.dt, c.prm.c0, c.stream))
         return set_error(ARD_ERR_HIP, "PML launch");
-    for (size_t b = 0; b < c.h_bnd.size(); ++b)
+      T.end(c.stream, ev);
+    }
+    for (size_t b = 0; b < c.h_bnd.size(); ++b) {
+      ev = T.begin(c.stream, ARD_PH_INTERFACE);
       if (launch_interface(c.d_bnd + b, 1, c.d_parts, par ^ 1, c.bnd_elems[b], c.stream))
         return set_error(ARD_ERR_HIP, "interface launch");
+      T.end(c.stream, ev);
+    }
+    ev = T.begin(c.stream, ARD_PH_RECORD);
     if (launch_record(c.d_rec, nr, c.d_trace, c.trace_cap, c.d_step, c.stream))
       return set_error(ARD_ERR_HIP, "recorder launch");
+    T.end(c.stream, ev);
     c.time_step++;
+    if (T.on && T.used > 2048) {
+      int r = T.collect();
+      if (r) return r;
+    }
   }
   return ARD_OK;
 }

@@ -129,6 +129,29 @@ int ard_set_force(ard_ctx* ctx, int part, int x, int y, int z, float f);
 /* Blocks until all work queued on the context's stream is done. */
 int ard_synchronize(ard_ctx* ctx);
 
+/* Kernel timing (no counterpart in the reference, which only prints wall-clock
+ * totals, ARD/main.cpp:49,124-126). While on, ard_step launches each phase
+ * directly on the context's stream between two HIP events (no graph replay)
+ * and accumulates the per-launch durations; ard_kernel_times returns, per
+ * phase, the average duration of one launch in ms and the number of launches
+ * since timing was switched on. Phases: */
+#define ARD_PH_SOURCES 0
+#define ARD_PH_PML 1
+#define ARD_PH_FOLD 2
+#define ARD_PH_FACE_FWD1 3
+#define ARD_PH_FACE_FWD2 4
+#define ARD_PH_MODAL 5
+#define ARD_PH_UNFOLD 6
+#define ARD_PH_FACE_INV1 7
+#define ARD_PH_FACE_INV2 8
+#define ARD_PH_INTERFACE 9
+#define ARD_PH_RECORD 10
+#define ARD_PH_DCT_FWD 11 /* plain form: the three DCT-II passes with the modal update */
+#define ARD_PH_DCT_INV 12 /* plain form: the three DCT-III passes */
+#define ARD_NUM_PHASES 13
+int ard_set_kernel_timing(ard_ctx* ctx, int on);
+int ard_kernel_times(ard_ctx* ctx, float* avg_ms, int* launches, int nphase);
+
 #ifdef __cplusplus
 }
 #endif
