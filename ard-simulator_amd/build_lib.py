@@ -13,6 +13,7 @@ OBJ = os.path.join(HERE, "_build")
 
 HIP_SOURCES = ["ard_kernels.hip", "ard_batched_kernels.hip", "ard_batched.hip", "ard_ctx.cpp", "ard_plain.cpp"]
 CXX_SOURCES = ["ard_topology.cpp"]
+HOST_SOURCES = ["host/ard_host.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
@@ -42,9 +43,16 @@ def build(verbose=False):
         obj = os.path.join(OBJ, src + ".o")
         run(["g++", "-c", os.path.join(CSRC, src), "-o", obj] + COMMON)
         objs.append(obj)
+    for src in HOST_SOURCES:  # the C++ class mirror, over the C-ABI
+        obj = os.path.join(OBJ, os.path.basename(src) + ".o")
+        run(["g++", "-c", os.path.join(HERE, src), "-o", obj] + COMMON)
+        objs.append(obj)
     tmp = OUT + ".tmp"
     run([HIPCC, "-shared", "-o", tmp] + objs + ["--offload-arch=" + ARCH, "-fPIC"])
     os.replace(tmp, OUT)
+    # command-line driver (the reference's main loop without the window)
+    run(["g++", "-O2", "-std=c++17", "-o", os.path.join(HERE, "ard_run"), os.path.join(HERE, "host", "ard_run.cpp"),
+         "-L" + HERE, "-lard", "-Wl,-rpath,$ORIGIN"])
     if verbose:
         print("built", OUT)
     return OUT

@@ -1,0 +1,192 @@
+// C++ host mirror of the reference's scene API, running on libard.so.
+//
+// Same class names, constructors and methods as the reference
+// (ARD/simulation.h, ARD/partition.h, ARD/dct_partition.h, ARD/pml_partition.h,
+// ARD/boundary.h, ARD/sound_source.h, ARD/gaussian_source.h, ARD/recorder.h),
+// so a caller of Simulation::Update keeps its code: the partitions, sources
+// and recorders are imported and constructed as before, the Simulation
+// constructor builds the same topology (inside the C-ABI), and Update() runs
+// one time step of every partition and boundary on the GPU.
+//
+// Differences, all at the edges of the API:
+//  * get_pressure_field() returns a host copy (std::vector) instead of the
+//    partition's own array, which lives on the device;
+//  * Partition handles of the PML slabs are created by the Simulation
+//    constructor and appended to its partition list, as in the reference;
+//  * errors throw ard::host::Error instead of asserting or crashing.
+#pragma once
+
+#include <fstream>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/ard.h"
+
+namespace ard {
+namespace host {
+
+typedef float real_t;  // ARD/types.h:2
+
+struct Error : std::runtime_error {
+  explicit Error(const std::string& m) : std::runtime_error(m) {}
+};
+
+class Simulation;
+class Recorder;
+
+class Partition {
+ public:
+  static real_t m_absorption;  // ARD/partition.h:45
+
+  Partition(int xs, int ys, int zs, int w, int h, int d);
+  virtual ~Partition() = default;
+
+  // ARD/partition.h:49-57
+  virtual bool is_pml() const = 0;
+  std::vector<real_t> get_pressure_field();
+  std::vector<real_t> get_xy_plane(int z);
+  std::vector<real_t> get_yz_plane(int x);
+  std::vector<real_t> get_xz_plane(int y);
+  real_t get_pressure(int x, int y, int z);
+  void set_force(int x, int y, int z, real_t f);
+  void Info();
+
+  // ARD/partition.cpp:151-176
+  static std::vector<std::shared_ptr<Partition>> ImportPartitions(const std::string& path);
+
+  int x_start() const { return x_start_; }
+  int y_start() const { return y_start_; }
+  int z_start() const { return z_start_; }
+  int width() const { return width_; }
+  int height() const { return height_; }
+  int depth() const { return depth_; }
+  int id() const { return id_; }
+
+ protected:
+  int x_start_, y_start_, z_start_;
+  int width_, height_, depth_;
+  int id_ = -1;               // index in the context, set by Simulation
+  ard_ctx* ctx_ = nullptr;    // owning context, set by Simulation
+  friend class Simulation;
+  friend class Recorder;
+};
+
+class DctPartition : public Partition {
+ public:
+  DctPartition(int xs, int ys, int zs, int w, int h, int d) : Partition(xs, ys, zs, w, h, d) {}
+  bool is_pml() const override { return false; }
+};
+
+class PmlPartition : public Partition {
+ public:
+  enum PmlType { P_LEFT, P_RIGHT, P_TOP, P_BOTTOM, P_FRONT, P_BACK };  // ARD/pml_partition.h:39-46
+  PmlPartition(std::shared_ptr<Partition> neighbor, PmlType type, int xs, int ys, int zs, int w, int h, int d)
+      : Partition(xs, ys, zs, w, h, d), neighbor_(neighbor), type_(type) {}
+  bool is_pml() const override { return true; }
+  PmlType type() const { return type_; }
+
+ private:
+  std::shared_ptr<Partition> neighbor_;
+  PmlType type_;
+};
+
+class SoundSource {
+ public:
+  SoundSource(int x, int y, int z) : x_(x), y_(y), z_(z) {}
+  virtual ~SoundSource() = default;
+  virtual real_t SampleValue(real_t t) = 0;
+  // ARD/sound_source.cpp:24-48 (without writing source_<id>.txt)
+  static std::vector<std::shared_ptr<SoundSource>> ImportSources(const std::string& path);
+  int x() const { return x_; }
+  int y() const { return y_; }
+  int z() const { return z_; }
+  // samples for steps 0..n-1 if the source is a fixed waveform; empty for the
+  // Gaussian pulse, which the device evaluates itself
+  virtual std::vector<float> Waveform() const { return {}; }
+
+ protected:
+  int x_, y_, z_;
+};
+
+class GaussianSource : public SoundSource {
+ public:
+  GaussianSource(int x, int y, int z) : SoundSource(x, y, z) {}
+  real_t SampleValue(real_t t) override;  // ARD/gaussian_source.cpp:16-20
+};
+
+// A sampled waveform (band-limited noise drives the gradient checks).
+class WaveformSource : public SoundSource {
+ public:
+  WaveformSource(int x, int y, int z, std::vector<float> w) : SoundSource(x, y, z), w_(std::move(w)) {}
+  real_t SampleValue(real_t t) override {
+    const int i = int(t);
+    return i >= 0 && i < int(w_.size()) ? w_[i] : 0.f;
+  }
+  std::vector<float> Waveform() const override { return w_; }
+
+ private:
+  std::vector<float> w_;
+};
+
+class Recorder {
+ public:
+  Recorder(int x, int y, int z, int total_steps = 1000);
+  // ARD/recorder.cpp:26-38
+  void FindPartition(const std::vector<std::shared_ptr<Partition>>& partitions);
+  // ARD/recorder.cpp:40-57: appends the point sample of `time_step` to the
+  // response stream (the 10^3 cube of out_<id>.txt is not written)
+  void RecordField(int time_step = 0);
+  void OpenResponse(const std::string& path);
+  static std::vector<std::shared_ptr<Recorder>> ImportRecorders(const std::string& path);
+  int x() const { return x_; }
+  int y() const { return y_; }
+  int z() const { return z_; }
+
+ private:
+  int x_, y_, z_, total_steps_;
+  std::shared_ptr<Partition> part_;
+  Simulation* sim_ = nullptr;
+  int device_id_ = -1;  // recorder index in the context when registered
+  std::ofstream response_;
+  friend class Simulation;
+};
+
+class Simulation {
+ public:
+  // statics of ARD/simulation.h:41-46, set before constructing partitions
+  static real_t m_duration;
+  static real_t m_dh;
+  static real_t m_dt;
+  static real_t m_c0;
+  static int m_pml_layers;
+  static int m_form;           // ARD_FORM_PLAIN or ARD_FORM_BATCHED
+  static int m_recorder_mode;  // ARD_REC_REFERENCE or ARD_REC_LOCAL
+  static int m_device;
+
+  // ARD/simulation.cpp:14-294. `partitions` receives the PML slabs, like the
+  // reference's member list. Recorders given here are sampled on the device
+  // every step; their RecordField then reads the device trace.
+  Simulation(std::vector<std::shared_ptr<Partition>>& partitions,
+             std::vector<std::shared_ptr<SoundSource>>& sources,
+             std::vector<std::shared_ptr<Recorder>> recorders = {});
+  ~Simulation();
+  Simulation(const Simulation&) = delete;
+  Simulation& operator=(const Simulation&) = delete;
+
+  int Update();  // ARD/simulation.cpp:300-317: returns the step index before the update
+  void Info();
+  ard_ctx* context() { return ctx_; }
+  int time_step() const { return time_step_; }
+
+ private:
+  ard_ctx* ctx_ = nullptr;
+  std::vector<std::shared_ptr<Partition>> partitions_;
+  std::vector<std::shared_ptr<Recorder>> recorders_;
+  int time_step_ = 0;
+  friend class Recorder;
+};
+
+}  // namespace host
+}  // namespace ard

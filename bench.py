@@ -1,12 +1,14 @@
 """Throughput of the ARD time step on one MI355X.
 
 Metric: Gcell-steps/s = (DCT cells + PML cells) * steps / seconds / 1e9 over
-the timed steps, with the scene resident in HBM. Default workload: the
-L-shaped 40 x 25 x 10 m hall at h = 0.05 m (20 boxes, 74M DCT cells), FP32,
-batched form; the plain per-partition form is timed beside it.
+the timed steps, with the whole scene resident in HBM. Workload: the
+L-shaped 40 x 25 x 10 m hall at h = 0.05 m (20 boxes from seeded guillotine
+cuts, 74M DCT + 6.3M PML cells, 8 sources, 16 recorders), FP32. The batched
+form (hipGraph per step) gives `value`; the plain per-partition form is timed
+beside it on the same scene, and their recorder traces are compared.
 
-    python bench.py [--steps K] [--warmup W] [--form batched|plain]
-                    [--workload hall|scene3|two_rooms] [--no-cpu-baseline]
+    python bench.py [--steps K] [--warmup W] [--workload hall|scene3|two_rooms]
+                    [--no-plain] [--no-cpu-baseline]
 """
 import argparse
 import json
@@ -24,6 +26,7 @@ import ard_sim as A  # noqa: E402
 
 METRIC = "Gcell-steps/s on 1 MI355X, batched vs plain GPU at recorder rel-L2 ≤1e-4"
 HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3
 # algorithmic bytes per unit of work (dense formulation, FP32)
 B_DCT, B_PML, B_FACE = 20, 44, 48
 
@@ -32,7 +35,7 @@ def scene_by_name(name):
     return {"hall": S.hall, "scene3": S.scene3, "two_rooms": S.two_rooms}[name]()
 
 
-def cell_counts(sim):
+def counts(sim):
     parts = sim.partitions()
     n_dct = sum(p[6] * p[7] * p[8] for p in parts if p[0] == 0)
     n_pml = sum(p[6] * p[7] * p[8] for p in parts if p[0] == 1)
@@ -43,40 +46,114 @@ def cell_counts(sim):
     return n_dct, n_pml, faces
 
 
-def time_form(sc, form, steps, warmup):
+def face_gemm_flops(sim):
+    """2*M*N*K of the four face-transform GEMM launches of one batched step."""
+    fl = 0
+    for p in sim.partitions():
+        if p[0]:
+            continue
+        W, H, D = p[6:9]
+        # forward and inverse: x faces (H then D), y faces (W then D), z faces (W then H)
+        fl += 2 * 2 * 6 * (D * H * H + H * D * D + D * W * W + W * D * D + H * W * W + W * H * H)
+    return fl
+
+
+def run_form(sc, form, steps, warmup, timing_steps):
     sim = A.Simulation(sc, form=form)
     sim.update(warmup)
     sim.synchronize()
     t0 = time.perf_counter()
     sim.update(steps)
     sim.synchronize()
-    dt = time.perf_counter() - t0
-    return sim, dt
+    secs = time.perf_counter() - t0
+    phases = {}
+    if timing_steps:
+        sim.set_kernel_timing(True)
+        sim.update(timing_steps)
+        sim.synchronize()
+        phases = sim.kernel_times()
+        sim.set_kernel_timing(False)
+    trace = sim.recorders(0, warmup + steps)
+    return sim, secs, phases, trace
+
+
+def traffic_from_profile(kernel):
+    """HBM bytes per launch measured by rocprofv3 --pmc (FETCH_SIZE, WRITE_SIZE),
+    recorded in profiles/traffic.json by the profiling run; None if absent."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        d = json.load(f)
+    return d.get(kernel)
+
+
+def cpu_baseline(sc):
+    """The CPU oracle (FP32 restatement of the reference's step, dense DCTs,
+    OpenMP) timed on this host for one step of the same scene."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import ard_oracle  # the checker, used here only as the timed CPU baseline
+
+    cores = min(os.cpu_count() or 1, 16)
+    os.environ["OMP_NUM_THREADS"] = str(cores)
+    o = ard_oracle.from_scene(sc)
+    n_dct = sc.dct_cells()
+    parts = o.partitions()
+    n_pml = sum(p[6] * p[7] * p[8] for p in parts if p[0] == 1)
+    t0 = time.perf_counter()
+    o.step(1)
+    secs = time.perf_counter() - t0
+    del o
+    return {"value": round((n_dct + n_pml) / secs / 1e9, 6), "unit": "Gcell-steps/s", "cores": cores,
+            "kind": "port", "sample": "1 step of the %s scene (%d cells), setup excluded, %.1f s" % (
+                sc.name, n_dct + n_pml, secs)}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
-    ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--form", default="plain", choices=["batched", "plain"])
+    ap.add_argument("--steps", type=int, default=400)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--timing-steps", type=int, default=20)
     ap.add_argument("--workload", default="hall")
-    ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: the ARD step here runs on a single MI355X "
-              "(multi-GPU sharding is a design note only)", file=sys.stderr)
+        print("bench.py measures one GPU: this ARD path runs on a single MI355X; "
+              "multi-GPU sharding is a design note only", file=sys.stderr)
         sys.exit(2)
 
     sc = scene_by_name(args.workload)
-    form = A.FORM_BATCHED if args.form == "batched" else A.FORM_PLAIN
-    sim, secs = time_form(sc, form, args.steps, args.warmup)
-    n_dct, n_pml, faces = cell_counts(sim)
+    sim, secs, phases, trace_b = run_form(sc, A.FORM_BATCHED, args.steps, args.warmup, args.timing_steps)
+    n_dct, n_pml, faces = counts(sim)
+    gflop_face = face_gemm_flops(sim)
+    sim.close()
     cells = n_dct + n_pml
     value = cells * args.steps / secs / 1e9
     ms = secs * 1e3 / args.steps
-    alg_bytes = B_DCT * n_dct + B_PML * n_pml + B_FACE * faces
+
+    # roofline of the dominant kernel (the phase with the largest time per step)
+    per_step = {k: v[0] * v[1] / max(args.timing_steps, 1) for k, v in phases.items()}
+    dom = max(per_step, key=per_step.get) if per_step else None
+    alg = {"modal": (B_DCT * n_dct, "hbm"), "pml": (B_PML * n_pml, "hbm"), "interface": (B_FACE * faces, "hbm")}
+    roof = None
+    if dom:
+        avg_ms, launches = phases[dom]
+        if dom in alg:
+            # modal launches run once per rows-per-wave group; the DCT cells are split among them
+            per_launch = alg[dom][0] * args.timing_steps / launches
+            achieved = per_launch / (avg_ms * 1e-3) / 1e9
+            roof = {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                    "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                    "traffic": traffic_from_profile(dom), "avg_ms": round(avg_ms, 4)}
+        else:
+            per_launch = gflop_face / 4 * args.timing_steps / launches
+            achieved = per_launch / (avg_ms * 1e-3) / 1e12
+            roof = {"kernel": dom, "bound": "mfma", "achieved": round(achieved, 2), "peak": FP32_PEAK_TFLOPS,
+                    "unit": "TFLOP/s", "frac": round(achieved / FP32_PEAK_TFLOPS, 4),
+                    "traffic": traffic_from_profile(dom), "avg_ms": round(avg_ms, 4)}
+
     out = {
         "metric": METRIC,
         "value": round(value, 4),
@@ -90,12 +167,25 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": "%s (%s form)" % (sc.name, args.form), "dct_cells": n_dct, "pml_cells": n_pml,
-                   "face_elements": faces, "h": sc.h, "dt": sc.dt, "boxes": len(sc.boxes)},
-        "step_roofline": {"alg_bytes_per_step": alg_bytes,
-                          "achieved_GBs": round(alg_bytes / (ms * 1e-3) / 1e9, 1)},
+        "config": {"workload": "%s: batched face-shell form, hipGraph per step" % sc.name, "dct_cells": n_dct,
+                   "pml_cells": n_pml, "face_elements": faces, "boxes": len(sc.boxes), "h": sc.h, "dt": sc.dt,
+                   "sources": len(sc.sources), "recorders": len(sc.recorders)},
+        "roofline": roof,
+        "step_algorithmic_GBs": round((B_DCT * n_dct + B_PML * n_pml + B_FACE * faces) / (ms * 1e-3) / 1e9, 1),
+        "phase_ms_per_step": {k: round(v, 4) for k, v in sorted(per_step.items(), key=lambda kv: -kv[1])},
     }
-    sim.close()
+    if not args.no_plain:
+        simp, secs_p, phases_p, trace_p = run_form(sc, A.FORM_PLAIN, args.steps, args.warmup, 0)
+        simp.close()
+        n_p = args.steps
+        plain_value = cells * n_p / secs_p / 1e9
+        nt = min(len(trace_p), len(trace_b))
+        rel = float(np.linalg.norm(trace_b[:nt] - trace_p[:nt]) / max(np.linalg.norm(trace_p[:nt]), 1e-30))
+        out["plain"] = {"value": round(plain_value, 4), "ms_per_step": round(secs_p * 1e3 / n_p, 3), "steps": n_p}
+        out["batched_over_plain"] = round(value / plain_value, 2)
+        out["recorder_rel_l2_batched_vs_plain"] = {"value": rel, "steps": nt}
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(sc)
     print(json.dumps(out))
 
 

@@ -58,9 +58,10 @@ def test_scene3_trace_matches_golden(form):
 def small_scene():
     """Three boxes with partial faces (PML runs, edge cells written twice)."""
     sc = S.two_rooms()
-    sc.boxes = [(0, 0, 0, 16, 12, 10), (16, 0, 0, 12, 20, 10), (0, 12, 0, 16, 8, 10)]
-    sc.sources = [S.Source(5, 5, 4), S.Source(20, 15, 6)]
-    sc.recorders = [(22, 10, 5)]
+    sc.boxes = [(0, 0, 0, 16, 12, 16), (16, 0, 0, 12, 20, 16), (0, 12, 0, 16, 8, 16)]
+    # one source inside a face slab of box 0, one in the interior of box 1
+    sc.sources = [S.Source(5, 1, 4), S.Source(21, 10, 8)]
+    sc.recorders = [(22, 10, 8)]
     sc.recorder_mode = S.REC_LOCAL
     return sc
 
@@ -105,9 +106,10 @@ def test_set_force_and_get_pressure(form):
 
 
 def test_hall_batched_matches_plain():
-    """Full-size hall (74M DCT cells): the two forms agree step by step."""
+    """Full-size hall (74M DCT cells): the two forms agree once the pulses have
+    reached the recorders (before that the traces are rounding noise)."""
     sc = S.hall()
-    n = 30
+    n = 400
     print("case hall plain vs batched, %d steps" % n, flush=True)
     tr = {}
     fields = {}
