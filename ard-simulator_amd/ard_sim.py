@@ -26,7 +26,7 @@ EXPORTED = [
     "ard_finalize", "ard_num_partitions", "ard_num_boundaries", "ard_num_recorders", "ard_partition_info",
     "ard_boundary_info", "ard_recorder_partition", "ard_step", "ard_time_step", "ard_fetch_recorders",
     "ard_copy_pressure_field", "ard_get_pressure", "ard_set_force", "ard_synchronize", "ard_set_kernel_timing",
-    "ard_kernel_times",
+    "ard_kernel_times", "ard_selftest_transform", "ard_set_transform_variant",
 ]
 PHASES = ["sources", "pml", "fold", "face_fwd1", "face_fwd2", "modal", "unfold", "face_inv1", "face_inv2",
           "interface", "record", "dct_fwd", "dct_inv"]
@@ -80,6 +80,8 @@ def lib():
         L.ard_set_force.argtypes = [vp, i, i, i, i, f]
         L.ard_set_kernel_timing.argtypes = [vp, i]
         L.ard_kernel_times.argtypes = [vp, fp, ip, i]
+        L.ard_selftest_transform.argtypes = [i, i, i, i, i, i, fp, fp, fp]
+        L.ard_set_transform_variant.argtypes = [i]
         _lib = L
     return _lib
 
@@ -92,6 +94,22 @@ def _check(r):
 
 def _fp(a):
     return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+def selftest_transform(A, B, variant=1, b_layout=0, device=0):
+    """C = A @ B through the library's dense transform kernel (B given as K x N,
+    or as N x K when b_layout is 1)."""
+    A = np.ascontiguousarray(A, dtype=np.float32)
+    B = np.ascontiguousarray(B, dtype=np.float32)
+    M, K = A.shape
+    N = B.shape[1] if b_layout == 0 else B.shape[0]
+    C = np.zeros((M, N), dtype=np.float32)
+    _check(lib().ard_selftest_transform(device, variant, b_layout, M, N, K, _fp(A), _fp(B), _fp(C)))
+    return C
+
+
+def set_transform_variant(v):
+    _check(lib().ard_set_transform_variant(v))
 
 
 class Simulation:

@@ -462,4 +462,51 @@ int ard_set_force(ard_ctx* ctx, int part, int x, int y, int z, float f) {
   return set_force_cell(*c, part, x, y, z, f);
 }
 
+int ard_set_transform_variant(int variant) {
+  if (variant < 0 || variant > 1) return fail(ARD_ERR_INVALID, "unknown transform variant");
+  g_gemm_variant = variant;
+  return ARD_OK;
+}
+
+int ard_selftest_transform(int device, int variant, int b_layout, int M, int N, int K, const float* A, const float* B,
+                           float* C) {
+  if (!A || !B || !C || M <= 0 || N <= 0 || K <= 0) return fail(ARD_ERR_INVALID, "bad arguments");
+  if (variant < 0 || variant > 1 || b_layout < 0 || b_layout > 1) return fail(ARD_ERR_INVALID, "bad variant");
+  hipError_t e = hipSetDevice(device);
+  if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+  float *dA = nullptr, *dB = nullptr, *dC = nullptr;
+  GemmDesc* dD = nullptr;
+  const size_t sa = size_t(M) * K * 4, sb = size_t(K) * N * 4, sc = size_t(M) * N * 4;
+  int r = ARD_OK;
+  if ((e = hipMalloc(&dA, sa)) != hipSuccess || (e = hipMalloc(&dB, sb)) != hipSuccess ||
+      (e = hipMalloc(&dC, sc)) != hipSuccess || (e = hipMalloc(&dD, sizeof(GemmDesc))) != hipSuccess) {
+    r = hip_fail(e, "hipMalloc");
+  } else {
+    GemmDesc g{};
+    g.A = dA; g.B = dB; g.C = dC;
+    g.M = M; g.N = N; g.K = K; g.N1 = N;
+    if (b_layout == 0) { g.bK = N; g.b1 = 1; } else { g.bK = 1; g.b1 = K; }
+    g.cM = N; g.c1 = 1;
+    g.alpha = 1.f;
+    g.tiles_n = (N + 63) / 64;
+    g.tile0 = 0;
+    const int tiles = ((M + 63) / 64) * g.tiles_n;
+    const int saved = g_gemm_variant;
+    g_gemm_variant = variant;
+    if (hipMemcpy(dA, A, sa, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(dB, B, sb, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(dD, &g, sizeof g, hipMemcpyHostToDevice) != hipSuccess)
+      r = fail(ARD_ERR_HIP, "hipMemcpy");
+    else if (launch_gemm(dD, 1, tiles, b_layout == 1, false, 0, nullptr))
+      r = fail(ARD_ERR_HIP, "launch");
+    else if ((e = hipDeviceSynchronize()) != hipSuccess)
+      r = hip_fail(e, "hipDeviceSynchronize");
+    else if ((e = hipMemcpy(C, dC, sc, hipMemcpyDeviceToHost)) != hipSuccess)
+      r = hip_fail(e, "hipMemcpy D2H");
+    g_gemm_variant = saved;
+  }
+  hipFree(dA); hipFree(dB); hipFree(dC); hipFree(dD);
+  return r;
+}
+
 }  // extern "C"

@@ -129,10 +129,116 @@ __global__ __launch_bounds__(256) void k_gemm(const GemmDesc* __restrict__ descs
   }
 }
 
+// ---------------------------------------------------------------------------
+// The same grouped GEMM on the matrix cores: each of the 4 waves owns a 32x32
+// quarter of the 64x64 tile and accumulates it with v_mfma_f32_32x32x2_f32
+// (FP32 in, FP32 accumulate). Operand layout of the instruction: lane l holds
+// A[l % 32][l / 32] and B[l / 32][l % 32]; accumulator register r of lane l is
+// D[8 (r / 4) + 4 (l / 32) + r % 4][l % 32].
+// ---------------------------------------------------------------------------
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+template <bool B_K_CONTIG, int EP>
+__global__ __launch_bounds__(256) void k_gemm_mfma(const GemmDesc* __restrict__ descs, int ndesc) {
+  __shared__ float As[16][68];
+  __shared__ float Bs[16][68];
+  const int tile = blockIdx.x;
+  int lo = 0, hi = ndesc - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (descs[mid].tile0 <= tile) lo = mid;
+    else hi = mid - 1;
+  }
+  const GemmDesc& g = descs[lo];
+  const int M = g.M, N = g.N, K = g.K, N1 = g.N1;
+  const int t = tile - g.tile0;
+  const int m0 = (t / g.tiles_n) * 64;
+  const int n0 = (t % g.tiles_n) * 64;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+
+  long long boff[4];
+  int bn[4];
+  if (B_K_CONTIG) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int n = n0 + tid / 16 + 16 * r;
+      bn[r] = n;
+      boff[r] = n < N ? (long long)(n % N1) * g.b1 + (long long)(n / N1) * g.b2 : 0;
+    }
+  } else {
+    const int n = n0 + (tid & 63);
+    bn[0] = n;
+    boff[0] = n < N ? (long long)(n % N1) * g.b1 + (long long)(n / N1) * g.b2 : 0;
+  }
+  f32x16 acc;
+#pragma unroll
+  for (int r = 0; r < 16; ++r) acc[r] = 0.f;
+
+  const float* __restrict__ A = g.A;
+  const float* __restrict__ B = g.B;
+  for (int k0 = 0; k0 < K; k0 += 16) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int k = tid & 15, m = (tid >> 4) + 16 * r;
+      const int gm = m0 + m, gk = k0 + k;
+      As[k][m] = (gm < M && gk < K) ? A[(long long)gm * K + gk] : 0.f;
+    }
+    if (B_K_CONTIG) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int k = tid & 15, n = (tid >> 4) + 16 * r;
+        const int gk = k0 + k;
+        Bs[k][n] = (bn[r] < N && gk < K) ? B[(long long)gk * g.bK + boff[r]] : 0.f;
+      }
+    } else {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int k = (tid >> 6) + 4 * r, n = tid & 63;
+        const int gk = k0 + k;
+        Bs[k][n] = (bn[0] < N && gk < K) ? B[(long long)gk * g.bK + boff[0]] : 0.f;
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < 16; kk += 2) {
+      const float a = As[kk + (lane >> 5)][wm + (lane & 31)];
+      const float b = Bs[kk + (lane >> 5)][wn + (lane & 31)];
+      acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, acc, 0, 0, 0);
+    }
+    __syncthreads();
+  }
+  const int j = n0 + wn + (lane & 31);
+  if (j >= N) return;
+  const long long cn = (long long)(j % N1) * g.c1 + (long long)(j / N1) * g.c2;
+  float* __restrict__ C = g.C;
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    const int i = m0 + wm + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+    if (i >= M) continue;
+    const long long o = (long long)i * g.cM + cn;
+    const float v = g.alpha * acc[r];
+    if (EP == 0) C[o] = v;
+    else C[o] = g.ep_a[o] * g.ep_cur[o] - g.ep_b * C[o] + g.ep_g[o] * v;
+  }
+}
+
+// variant: 0 vector FMA, 1 matrix cores (FP32)
+int g_gemm_variant = 1;
+
 int launch_gemm(const GemmDesc* d_descs, int ndesc, int total_tiles, bool b_k_contig, bool c_m_contig, int ep,
                 hipStream_t s) {
   if (total_tiles <= 0) return 0;
   dim3 grid(total_tiles), block(256);
+  if (g_gemm_variant == 1) {
+    if (b_k_contig && ep == 0) hipLaunchKernelGGL((k_gemm_mfma<true, 0>), grid, block, 0, s, d_descs, ndesc);
+    else if (!b_k_contig && ep == 0) hipLaunchKernelGGL((k_gemm_mfma<false, 0>), grid, block, 0, s, d_descs, ndesc);
+    else if (!b_k_contig && ep == 1) hipLaunchKernelGGL((k_gemm_mfma<false, 1>), grid, block, 0, s, d_descs, ndesc);
+    else return -1;
+    return hipGetLastError() == hipSuccess ? 0 : -2;
+  }
   if (b_k_contig && c_m_contig && ep == 0)
     hipLaunchKernelGGL((k_gemm<true, true, 0>), grid, block, 0, s, d_descs, ndesc);
   else if (!b_k_contig && !c_m_contig && ep == 0)

@@ -118,12 +118,25 @@ def main():
     ap.add_argument("--workload", default="hall")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--transform-variant", type=int, default=1, help="0 vector FMA, 1 matrix cores (FP32)")
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="also run one device copy of this size (HBM counter calibration under rocprofv3)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this ARD path runs on a single MI355X; "
               "multi-GPU sharding is a design note only", file=sys.stderr)
         sys.exit(2)
 
+    A.set_transform_variant(args.transform_variant)
+    if args.copy_calibration_mib:
+        import torch
+
+        n = args.copy_calibration_mib * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     sc = scene_by_name(args.workload)
     sim, secs, phases, trace_b = run_form(sc, A.FORM_BATCHED, args.steps, args.warmup, args.timing_steps)
     n_dct, n_pml, faces = counts(sim)
@@ -167,7 +180,8 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": "%s: batched face-shell form, hipGraph per step" % sc.name, "dct_cells": n_dct,
+        "config": {"workload": "%s: batched face-shell form, hipGraph per step" % sc.name,
+                   "transform": ["vector-fma", "mfma-fp32"][args.transform_variant], "dct_cells": n_dct,
                    "pml_cells": n_pml, "face_elements": faces, "boxes": len(sc.boxes), "h": sc.h, "dt": sc.dt,
                    "sources": len(sc.sources), "recorders": len(sc.recorders)},
         "roofline": roof,

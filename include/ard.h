@@ -152,6 +152,16 @@ int ard_synchronize(ard_ctx* ctx);
 int ard_set_kernel_timing(ard_ctx* ctx, int on);
 int ard_kernel_times(ard_ctx* ctx, float* avg_ms, int* launches, int nphase);
 
+/* Self-test of the dense transform kernels (no counterpart in the reference):
+ * C[M][N] = A[M][K] * B on device `device`, where B is K x N row-major
+ * (b_layout 0) or N x K row-major (b_layout 1). variant: 0 vector FMA,
+ * 1 matrix cores FP32. Synchronous; allocates and frees its own buffers. */
+int ard_selftest_transform(int device, int variant, int b_layout, int M, int N, int K, const float* A, const float* B,
+                           float* C);
+/* Selects the dense transform kernels used by contexts finalized afterwards
+ * (0 vector FMA, 1 matrix cores FP32). Default 1. */
+int ard_set_transform_variant(int variant);
+
 #ifdef __cplusplus
 }
 #endif
