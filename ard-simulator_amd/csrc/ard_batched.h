@@ -3,7 +3,8 @@
 
 #include "ard_device.h"
 
-#define ARD_MAX_REC_PER_PART 8
+#define ARD_MAX_REC_PER_PART 4
+#define ARD_MAX_SRC_PER_PART 4  // interior point sources per partition
 
 namespace ard {
 
@@ -77,7 +78,7 @@ struct BRecDesc {
 int launch_fold(const FoldDesc* d, const long long* starts, int n, long long total, hipStream_t s);
 int launch_unfold(const UnfoldDesc* d, const long long* starts, int n, long long total, hipStream_t s);
 int launch_bsources(const BSourceDesc* d, int n, const int* d_step, float courant, hipStream_t s);
-int launch_modal(int rpw, const ModalDesc* d, const long long* starts, int n, long long blocks, int par,
+int launch_modal(int rpw, bool pts, const ModalDesc* d, const long long* starts, int n, long long blocks, int par,
                  hipStream_t s);
 int launch_brecord(const BRecDesc* d, int n, float* trace, int cap, int* d_step, hipStream_t s);
 

@@ -90,6 +90,7 @@ struct BatchedState {
   int ngemm[4] = {}, gemm_tiles[4] = {};
   struct ModalGroup {
     int rpw;
+    bool pts;
     ModalDesc* d_desc[2] = {};  // same table for both parities (the kernel takes the parity)
     long long* d_start = nullptr;
     int n = 0;
@@ -219,7 +220,7 @@ int build_sources(Ctx& c) {
 int build_modal(Ctx& c) {
   BatchedState& B = *c.bs;
   int r;
-  std::map<int, std::vector<ModalDesc>> groups;
+  std::map<std::pair<int, int>, std::vector<ModalDesc>> groups;  // (rows per wave, has points)
   for (int i = 0; i < int(c.topo.parts.size()); ++i) {
     const PartGeom& p = c.topo.parts[i];
     if (p.is_pml) continue;
@@ -235,6 +236,8 @@ int build_modal(Ctx& c) {
     for (auto& s : B.srcs)
       if (s.part == i && !s.in_shell) st.push_back(SrcTab{s.tab, s.tab + p.w, s.tab + p.w + p.h, s.slot});
     d.nsrc = int(st.size());
+    if (d.nsrc > ARD_MAX_SRC_PER_PART)
+      return set_error(ARD_ERR_UNSUPPORTED, "more than 4 interior point forces in one partition");
     if (d.nsrc) {
       SrcTab* ds;
       if ((r = upload_vec(c, (void**)&ds, st.data(), st.size() * sizeof(SrcTab)))) return r;
@@ -258,12 +261,13 @@ int build_modal(Ctx& c) {
       d.rec = dr;
     }
     d.rec_part = P.rec_part;
-    groups[P.rpw].push_back(d);
+    groups[std::make_pair(P.rpw, (d.nsrc + d.nrec) > 0 ? 1 : 0)].push_back(d);
   }
   B.modal.clear();
   for (auto& kv : groups) {
     BatchedState::ModalGroup g;
-    g.rpw = kv.first;
+    g.rpw = kv.first.first;
+    g.pts = kv.first.second != 0;
     std::vector<long long> starts;
     long long blocks = 0;
     for (auto& d : kv.second) {
@@ -373,7 +377,7 @@ int finalize_batched(Ctx& c) {
     if (p.is_pml) continue;
     BPart& P = B.parts[i];
     if (int(P.recs.size()) > ARD_MAX_REC_PER_PART)
-      return set_error(ARD_ERR_UNSUPPORTED, "more than 8 recorders in one partition");
+      return set_error(ARD_ERR_UNSUPPORTED, "more than 4 recorders in one partition");
     const long long W = p.w, H = p.h, D = p.d, N = W * H * D;
     P.plane[0] = D * H; P.plane[1] = D * W; P.plane[2] = H * W;
     // tiling of the modal pass: rows per wave with the least padding, kz chunks of ~48
@@ -534,7 +538,7 @@ int enqueue_step(Ctx& c, int par, PhaseTimer* T) {
   phase(ARD_PH_FACE_FWD1, [&] { return launch_gemm(B.d_gemm[0], B.ngemm[0], B.gemm_tiles[0], true, true, 0, st); });
   phase(ARD_PH_FACE_FWD2, [&] { return launch_gemm(B.d_gemm[1], B.ngemm[1], B.gemm_tiles[1], false, false, 0, st); });
   for (auto& g : B.modal)
-    phase(ARD_PH_MODAL, [&] { return launch_modal(g.rpw, g.d_desc[par], g.d_start, g.n, g.blocks, par, st); });
+    phase(ARD_PH_MODAL, [&] { return launch_modal(g.rpw, g.pts, g.d_desc[par], g.d_start, g.n, g.blocks, par, st); });
   phase(ARD_PH_UNFOLD, [&] { return launch_unfold(B.d_unfold, B.d_unfold_start, B.nunfold, B.unfold_total, st); });
   phase(ARD_PH_FACE_INV1, [&] { return launch_gemm(B.d_gemm[2], B.ngemm[2], B.gemm_tiles[2], true, true, 0, st); });
   phase(ARD_PH_FACE_INV2, [&] { return launch_gemm(B.d_gemm[3], B.ngemm[3], B.gemm_tiles[3], false, false, 0, st); });

@@ -111,14 +111,23 @@ __global__ void k_bsources(const BSourceDesc* __restrict__ s, int n, const int* 
 //   Tz[par kz][l][ky][kx] += Rz[kz][l] M+   (along the kz walk)
 // Partial sums go to per-chunk slots, summed by k_unfold.
 // ---------------------------------------------------------------------------
-template <int RPW>
+// sum over the 8 lanes of one parity in each 16-lane row (DPP, no LDS):
+// xor 2 inside the quad, then rotations by 4 and 8 inside the row
+__device__ __forceinline__ float row_parity_sum(float v) {
+  v += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x4E, 0xF, 0xF, false));  // quad_perm [2,3,0,1]
+  v += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x124, 0xF, 0xF, false)); // row_ror:4
+  v += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x128, 0xF, 0xF, false)); // row_ror:8
+  return v;
+}
+
+template <int RPW, bool PTS>
 __global__ __launch_bounds__(256) void k_modal(const ModalDesc* __restrict__ descs,
                                                const long long* __restrict__ starts, int ndesc, int par) {
-  __shared__ float s_ty[4][3][64];
+  __shared__ float s_ty[2][4][3][64];
   __shared__ float s_rec[4][ARD_MAX_REC_PER_PART];
   const int bid = blockIdx.x;
   const int di = find_desc_ll(starts, ndesc, bid);
-  const ModalDesc& P = descs[di];
+  const ModalDesc& P = descs[di];  // fields used in the loops are copied to locals below
   const int lb = bid - int(starts[di]);
   const int bx = lb % P.nbx;
   const int by = (lb / P.nbx) % P.nby;
@@ -136,6 +145,18 @@ __global__ __launch_bounds__(256) void k_modal(const ModalDesc* __restrict__ des
   const int kz1 = min(D, kz0 + P.kz_chunk);
   const long long HW = (long long)H * W;
   const long long DH = (long long)D * H, DW = (long long)D * W;
+  const float* __restrict__ Gx = P.Gx;
+  const float* __restrict__ Gy = P.Gy;
+  const float* __restrict__ Ez = P.Ez;
+  const float* __restrict__ Rz = P.Rz;
+  const float* __restrict__ Ey = P.Ey;
+  const float* __restrict__ Ry = P.Ry;
+  const float* __restrict__ Aa = P.a;
+  const float* __restrict__ Ag = P.g;
+  const float* __restrict__ Mc = P.M[par];
+  float* __restrict__ Mp = P.M[par ^ 1];
+  float* __restrict__ Txp = P.Tx + (long long)bx * 6 * DH;
+  float* __restrict__ Typ = P.Ty + (long long)by * 6 * DW;
 
   float ex[3], rx[3];
 #pragma unroll
@@ -156,99 +177,116 @@ __global__ __launch_bounds__(256) void k_modal(const ModalDesc* __restrict__ des
       tz[r][q] = 0.f;
     }
   }
-  float rec_acc[ARD_MAX_REC_PER_PART];
+  // point sources and recorders of this partition: their x factor is fixed per lane
+  const int nsrc = PTS ? min(P.nsrc, ARD_MAX_SRC_PER_PART) : 0;
+  const int nrec = PTS ? P.nrec : 0;
+  float sfx[ARD_MAX_SRC_PER_PART], rfx[ARD_MAX_REC_PER_PART], rec_acc[ARD_MAX_REC_PER_PART];
 #pragma unroll
-  for (int q = 0; q < ARD_MAX_REC_PER_PART; ++q) rec_acc[q] = 0.f;
-  const int nrec = P.nrec;
-  const int nsrc = P.nsrc;
-
-  const float* __restrict__ Mc = P.M[par];
-  float* __restrict__ Mp = P.M[par ^ 1];
+  for (int q = 0; q < ARD_MAX_SRC_PER_PART; ++q) sfx[q] = PTS && q < nsrc ? P.src[q].fx[kxc] * *P.src[q].val : 0.f;
+#pragma unroll
+  for (int q = 0; q < ARD_MAX_REC_PER_PART; ++q) {
+    rfx[q] = PTS && q < nrec && xok ? P.rec[q].fx[kxc] : 0.f;
+    rec_acc[q] = 0.f;
+  }
   const float bdamp = 0.999f;
 
-  auto body = [&](const int kz, auto pzc) {
+  auto body = [&](const int kz, auto pzc, float (&ty)[3]) {
     constexpr int PZ = decltype(pzc)::value;
-    const float ez0 = P.Ez[kz * 3 + 0], ez1 = P.Ez[kz * 3 + 1], ez2 = P.Ez[kz * 3 + 2];
-    const float rz0 = P.Rz[kz * 3 + 0], rz1 = P.Rz[kz * 3 + 1], rz2 = P.Rz[kz * 3 + 2];
-    // y-face force spectrum at (kx, kz) for this wave's ky parity
+    const float ez0 = Ez[kz * 3 + 0], ez1 = Ez[kz * 3 + 1], ez2 = Ez[kz * 3 + 2];
+    const float rz0 = Rz[kz * 3 + 0], rz1 = Rz[kz * 3 + 1], rz2 = Rz[kz * 3 + 2];
     float gy[3];
 #pragma unroll
-    for (int j = 0; j < 3; ++j) gy[j] = xok ? P.Gy[(py * 3 + j) * DW + (long long)kz * W + kx] : 0.f;
-    float ty[3] = {0.f, 0.f, 0.f};
+    for (int j = 0; j < 3; ++j) gy[j] = xok ? Gy[(py * 3 + j) * DW + (long long)kz * W + kx] : 0.f;
+    float sz[ARD_MAX_SRC_PER_PART], rz[ARD_MAX_REC_PER_PART];
+#pragma unroll
+    for (int q = 0; q < ARD_MAX_SRC_PER_PART; ++q) sz[q] = PTS && q < nsrc ? P.src[q].fz[kz] : 0.f;
+#pragma unroll
+    for (int q = 0; q < ARD_MAX_REC_PER_PART; ++q) rz[q] = PTS && q < nrec ? P.rec[q].fz[kz] : 0.f;
 #pragma unroll
     for (int r = 0; r < RPW; ++r) {
       const int ky = ky0 + 4 * r;
       if (ky >= H) break;  // uniform across the wave
-      const float ey0 = P.Ey[ky * 3 + 0], ey1 = P.Ey[ky * 3 + 1], ey2 = P.Ey[ky * 3 + 2];
-      const float ry0 = P.Ry[ky * 3 + 0], ry1 = P.Ry[ky * 3 + 1], ry2 = P.Ry[ky * 3 + 2];
-      // x-face force spectrum at (ky, kz): uniform across lanes, parity per lane
       const long long gxo = (long long)kz * H + ky;
+      const long long o = (long long)kz * HW + (long long)ky * W + kx;
+      float a = 0.f, mc = 0.f, mp = 0.f, g = 0.f;
+      if (xok) {
+        a = Aa[o];
+        mc = Mc[o];
+        mp = Mp[o];
+        g = Ag[o];
+      }
+      // x-face force spectrum at (ky, kz): uniform across lanes, parity per lane
       float fh = 0.f;
 #pragma unroll
       for (int i = 0; i < 3; ++i) {
-        const float gs = P.Gx[i * DH + gxo];
-        const float gd = P.Gx[(3 + i) * DH + gxo];
+        const float gs = Gx[i * DH + gxo];
+        const float gd = Gx[(3 + i) * DH + gxo];
         fh = fmaf(ex[i], px ? gd : gs, fh);
       }
-      fh = fmaf(ey0, gy[0], fh);
-      fh = fmaf(ey1, gy[1], fh);
-      fh = fmaf(ey2, gy[2], fh);
+      fh = fmaf(Ey[ky * 3 + 0], gy[0], fh);
+      fh = fmaf(Ey[ky * 3 + 1], gy[1], fh);
+      fh = fmaf(Ey[ky * 3 + 2], gy[2], fh);
       fh = fmaf(ez0, gz[r][PZ * 3 + 0], fh);
       fh = fmaf(ez1, gz[r][PZ * 3 + 1], fh);
       fh = fmaf(ez2, gz[r][PZ * 3 + 2], fh);
-      for (int s = 0; s < nsrc; ++s) {
-        const SrcTab& T = P.src[s];
-        fh = fmaf(*T.val * T.fy[ky] * T.fz[kz], T.fx[kxc], fh);
+      if (PTS) {
+#pragma unroll
+        for (int q = 0; q < ARD_MAX_SRC_PER_PART; ++q)
+          if (q < nsrc) fh = fmaf(sfx[q], P.src[q].fy[ky] * sz[q], fh);
       }
-      const long long o = (long long)kz * HW + (long long)ky * W + kx;
-      float mn = 0.f;
-      if (xok) {
-        mn = P.a[o] * Mc[o] - bdamp * Mp[o] + P.g[o] * fh;
-        Mp[o] = mn;
-      }
+      const float mn = xok ? (a * mc - bdamp * mp + g * fh) : 0.f;
+      if (xok) Mp[o] = mn;
       tz[r][PZ * 3 + 0] = fmaf(rz0, mn, tz[r][PZ * 3 + 0]);
       tz[r][PZ * 3 + 1] = fmaf(rz1, mn, tz[r][PZ * 3 + 1]);
       tz[r][PZ * 3 + 2] = fmaf(rz2, mn, tz[r][PZ * 3 + 2]);
-      ty[0] = fmaf(ry0, mn, ty[0]);
-      ty[1] = fmaf(ry1, mn, ty[1]);
-      ty[2] = fmaf(ry2, mn, ty[2]);
+      ty[0] = fmaf(Ry[ky * 3 + 0], mn, ty[0]);
+      ty[1] = fmaf(Ry[ky * 3 + 1], mn, ty[1]);
+      ty[2] = fmaf(Ry[ky * 3 + 2], mn, ty[2]);
+      if (PTS) {
 #pragma unroll
-      for (int q = 0; q < ARD_MAX_REC_PER_PART; ++q) {
-        if (q < nrec) {
-          const RecTab& T = P.rec[q];
-          rec_acc[q] = fmaf(T.fx[kxc] * T.fy[ky] * T.fz[kz], mn, rec_acc[q]);
-        }
+        for (int q = 0; q < ARD_MAX_REC_PER_PART; ++q)
+          if (q < nrec) rec_acc[q] = fmaf(rfx[q] * P.rec[q].fy[ky] * rz[q], mn, rec_acc[q]);
       }
       // x reduction across lanes, keeping even and odd kx apart
-      float v0 = rx[0] * mn, v1 = rx[1] * mn, v2 = rx[2] * mn;
-#pragma unroll
-      for (int m = 2; m < 64; m <<= 1) {
-        v0 += __shfl_xor(v0, m);
-        v1 += __shfl_xor(v1, m);
-        v2 += __shfl_xor(v2, m);
-      }
+      float v0 = row_parity_sum(rx[0] * mn);
+      float v1 = row_parity_sum(rx[1] * mn);
+      float v2 = row_parity_sum(rx[2] * mn);
+      v0 += __shfl_xor(v0, 16);
+      v1 += __shfl_xor(v1, 16);
+      v2 += __shfl_xor(v2, 16);
+      v0 += __shfl_xor(v0, 32);
+      v1 += __shfl_xor(v1, 32);
+      v2 += __shfl_xor(v2, 32);
       if (lane < 2) {
-        float* tx = P.Tx + (long long)bx * 6 * DH;
-        tx[(lane * 3 + 0) * DH + gxo] = v0;
-        tx[(lane * 3 + 1) * DH + gxo] = v1;
-        tx[(lane * 3 + 2) * DH + gxo] = v2;
+        Txp[(lane * 3 + 0) * DH + gxo] = v0;
+        Txp[(lane * 3 + 1) * DH + gxo] = v1;
+        Txp[(lane * 3 + 2) * DH + gxo] = v2;
       }
     }
-    // y reduction: waves 0,2 hold even ky, waves 1,3 odd ky
+  };
+  // y reduction over the rows of the four waves: waves 0,2 hold even ky, waves 1,3 odd ky;
+  // one LDS exchange per pair of kz planes
+  for (int kz = kz0; kz < kz1; kz += 2) {
+    float ty0[3] = {0.f, 0.f, 0.f}, ty1[3] = {0.f, 0.f, 0.f};
+    const bool two = kz + 1 < kz1;
+    body(kz, std::integral_constant<int, 0>(), ty0);
+    if (two) body(kz + 1, std::integral_constant<int, 1>(), ty1);
 #pragma unroll
-    for (int j = 0; j < 3; ++j) s_ty[wave][j][lane] = ty[j];
-    __syncthreads();
-    if (wave < 2 && xok) {
-      float* typ = P.Ty + (long long)by * 6 * DW;
-#pragma unroll
-      for (int j = 0; j < 3; ++j)
-        typ[(wave * 3 + j) * DW + (long long)kz * W + kx] = s_ty[wave][j][lane] + s_ty[wave + 2][j][lane];
+    for (int j = 0; j < 3; ++j) {
+      s_ty[0][wave][j][lane] = ty0[j];
+      s_ty[1][wave][j][lane] = ty1[j];
     }
     __syncthreads();
-  };
-  for (int kz = kz0; kz < kz1; kz += 2) {
-    body(kz, std::integral_constant<int, 0>());
-    if (kz + 1 < kz1) body(kz + 1, std::integral_constant<int, 1>());
+    if (xok) {
+      // wave w: plane (w >> 1), parity (w & 1)
+      const int pl = wave >> 1, pw = wave & 1;
+      if (pl == 0 || two) {
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+          Typ[(pw * 3 + j) * DW + (long long)(kz + pl) * W + kx] = s_ty[pl][pw][j][lane] + s_ty[pl][pw + 2][j][lane];
+      }
+    }
+    __syncthreads();
   }
   // z reduction partials of this kz chunk
   float* tzp = P.Tz + (long long)bz * 6 * HW;
@@ -260,7 +298,7 @@ __global__ __launch_bounds__(256) void k_modal(const ModalDesc* __restrict__ des
     for (int q = 0; q < 6; ++q) tzp[q * HW + (long long)ky * W + kx] = tz[r][q];
   }
   // recorder partials of this block
-  if (nrec > 0) {
+  if (PTS && nrec > 0) {
 #pragma unroll
     for (int q = 0; q < ARD_MAX_REC_PER_PART; ++q) {
       if (q < nrec) {
@@ -316,15 +354,23 @@ int launch_bsources(const BSourceDesc* d, int n, const int* d_step, float couran
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 
-int launch_modal(int rpw, const ModalDesc* d, const long long* starts, int n, long long blocks, int par,
+int launch_modal(int rpw, bool pts, const ModalDesc* d, const long long* starts, int n, long long blocks, int par,
                  hipStream_t s) {
   if (blocks <= 0) return 0;
+  const dim3 g{unsigned(blocks)}, b{256};
+#define ARD_MODAL_CASE(R)                                                                      \
+  case R:                                                                                      \
+    if (pts) hipLaunchKernelGGL((k_modal<R, true>), g, b, 0, s, d, starts, n, par);            \
+    else hipLaunchKernelGGL((k_modal<R, false>), g, b, 0, s, d, starts, n, par);               \
+    break;
   switch (rpw) {
-    case 4: hipLaunchKernelGGL(k_modal<4>, dim3(unsigned(blocks)), dim3(256), 0, s, d, starts, n, par); break;
-    case 5: hipLaunchKernelGGL(k_modal<5>, dim3(unsigned(blocks)), dim3(256), 0, s, d, starts, n, par); break;
-    case 8: hipLaunchKernelGGL(k_modal<8>, dim3(unsigned(blocks)), dim3(256), 0, s, d, starts, n, par); break;
-    default: return -1;
+    ARD_MODAL_CASE(4)
+    ARD_MODAL_CASE(5)
+    ARD_MODAL_CASE(8)
+    default:
+      return -1;
   }
+#undef ARD_MODAL_CASE
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 

@@ -58,8 +58,10 @@ def face_gemm_flops(sim):
     return fl
 
 
-def run_form(sc, form, steps, warmup, timing_steps):
+def run_form(sc, form, steps, warmup, timing_steps, direct=False):
     sim = A.Simulation(sc, form=form)
+    if direct:
+        sim.set_kernel_timing(True)
     sim.update(warmup)
     sim.synchronize()
     t0 = time.perf_counter()
@@ -119,6 +121,8 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--transform-variant", type=int, default=1, help="0 vector FMA, 1 matrix cores (FP32)")
+    ap.add_argument("--direct", action="store_true",
+                    help="launch each phase directly instead of replaying the step graph (counter collection)")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="also run one device copy of this size (HBM counter calibration under rocprofv3)")
     args = ap.parse_args()
@@ -138,7 +142,8 @@ def main():
         torch.cuda.synchronize()
         del src, dst
     sc = scene_by_name(args.workload)
-    sim, secs, phases, trace_b = run_form(sc, A.FORM_BATCHED, args.steps, args.warmup, args.timing_steps)
+    sim, secs, phases, trace_b = run_form(sc, A.FORM_BATCHED, args.steps, args.warmup, args.timing_steps,
+                                          args.direct)
     n_dct, n_pml, faces = counts(sim)
     gflop_face = face_gemm_flops(sim)
     sim.close()
