@@ -48,8 +48,11 @@ struct BSourceDesc {
 struct ModalDesc {
   int W, H, D;
   int nbx, nby, nbz, kz_chunk;
-  const float* a;
-  const float* g;
+  float dt;
+  float gscale;       // .999 * 4 * s_f * s_i (force spectrum to modes)
+  const float* wx2;   // [W] (c0 pi (kx+1) / Lx)^2, and likewise for y, z
+  const float* wy2;
+  const float* wz2;
   float* M[2];
   const float* Ex;  // [W][3]  2 c_W(kx, x)
   const float* Rx;  // [W][3]  e(kx) c_W(kx, x)

@@ -58,8 +58,8 @@ struct BPart {
   float* P[3] = {};   // face-slab pressure
   float* T[3] = {};   // partial reductions [nchunk][6][plane]
   float* M[2] = {};
-  float* a = nullptr;
-  float* g = nullptr;
+  float* w2[3] = {};     // per-axis squared wavenumbers (c0 pi (k+1) / L)^2
+  float gscale = 0.f;
   float* E[3] = {};   // Ex, Ey, Ez
   float* R[3] = {};   // Rx, Ry, Rz
   float* rec_part = nullptr;
@@ -228,7 +228,10 @@ int build_modal(Ctx& c) {
     ModalDesc d{};
     d.W = p.w; d.H = p.h; d.D = p.d;
     d.nbx = P.nbx; d.nby = P.nby; d.nbz = P.nbz; d.kz_chunk = P.kz_chunk;
-    d.a = P.a; d.g = P.g; d.M[0] = P.M[0]; d.M[1] = P.M[1];
+    d.dt = c.prm.dt;
+    d.gscale = P.gscale;
+    d.wx2 = P.w2[0]; d.wy2 = P.w2[1]; d.wz2 = P.w2[2];
+    d.M[0] = P.M[0]; d.M[1] = P.M[1];
     d.Ex = P.E[0]; d.Rx = P.R[0]; d.Ey = P.E[1]; d.Ry = P.R[1]; d.Ez = P.E[2]; d.Rz = P.R[2];
     d.Gx = P.G[0]; d.Gy = P.G[1]; d.Gz = P.G[2];
     d.Tx = P.T[0]; d.Ty = P.T[1]; d.Tz = P.T[2];
@@ -395,8 +398,8 @@ int finalize_batched(Ctx& c) {
     const long long shell = 6 * (P.plane[0] + P.plane[1] + P.plane[2]);
     const long long tsz = 6 * (P.nbx * P.plane[0] + P.nby * P.plane[1] + P.nbz * P.plane[2]);
     float* blk;
-    if ((r = c.dalloc((void**)&blk, size_t(5 * shell + tsz + 4 * N) * 4))) return r;
-    if ((r = c.dzero(blk, size_t(5 * shell + tsz + 4 * N) * 4))) return r;
+    if ((r = c.dalloc((void**)&blk, size_t(5 * shell + tsz + 2 * N) * 4))) return r;
+    if ((r = c.dzero(blk, size_t(5 * shell + tsz + 2 * N) * 4))) return r;
     float* q = blk;
     for (float** arr : {P.F, P.S, P.U, P.G, P.P})
       for (int w = 0; w < 3; ++w) { arr[w] = q; q += 6 * P.plane[w]; }
@@ -405,11 +408,23 @@ int finalize_batched(Ctx& c) {
     P.T[2] = q; q += 6 * P.nbz * P.plane[2];
     P.M[0] = q; q += N;
     P.M[1] = q; q += N;
-    P.a = q; q += N;
-    P.g = q; q += N;
-    std::vector<float> a, g;
-    modal_tables(p.w, p.h, p.d, c.prm.h, c.prm.dt, c.prm.c0, a, g);
-    if ((r = c.upload(P.a, a.data(), N * 4)) || (r = c.upload(P.g, g.data(), N * 4))) return r;
+    // modal coefficients are formed in the kernel from per-axis squared
+    // wavenumbers (ARD/dct_partition.cpp:24-33, 1-based), tabulated in FP64
+    {
+      const double pi = 3.14159265358979323846;
+      const int n3[3] = {p.w, p.h, p.d};
+      for (int ax = 0; ax < 3; ++ax) {
+        std::vector<float> t(static_cast<size_t>(n3[ax]));
+        const double L = n3[ax] * double(c.prm.h);
+        for (int k = 0; k < n3[ax]; ++k) {
+          const double w = double(c.prm.c0) * pi * (k + 1) / L;
+          t[size_t(k)] = float(w * w);
+        }
+        if ((r = upload_vec(c, (void**)&P.w2[ax], t.data(), t.size() * 4))) return r;
+      }
+      const double Nd = double(N);
+      P.gscale = float(double(0.999f) * 4.0 / (2.0 * std::sqrt(2.0 * Nd)) / std::sqrt(2.0 * Nd));
+    }
     const int dims[3] = {p.w, p.h, p.d};
     for (int ax = 0; ax < 3; ++ax) {
       std::vector<float> E, R;

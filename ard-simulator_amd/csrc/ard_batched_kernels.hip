@@ -120,10 +120,26 @@ __device__ __forceinline__ float row_parity_sum(float v) {
   return v;
 }
 
+// sin(x) for 0 <= x <= 1.3 (half the largest mode phase c dt pi sqrt(3) / h at
+// Courant 0.43): odd Taylor series to x^11, truncation error < x^13/13! < 6e-9.
+__device__ __forceinline__ float sin_small(float x) {
+  const float x2 = x * x;
+  float p = -2.5052108e-8f;           // -1/11!
+  p = fmaf(p, x2, 2.7557319e-6f);     //  1/9!
+  p = fmaf(p, x2, -1.9841270e-4f);    // -1/7!
+  p = fmaf(p, x2, 8.3333333e-3f);     //  1/5!
+  p = fmaf(p, x2, -1.6666667e-1f);    // -1/3!
+  p = fmaf(p, x2, 1.0f);
+  return x * p;
+}
+
 template <int RPW, bool PTS>
 __global__ __launch_bounds__(256) void k_modal(const ModalDesc* __restrict__ descs,
                                                const long long* __restrict__ starts, int ndesc, int par) {
-  __shared__ float s_ty[2][4][3][64];
+  constexpr int TY = 4 * RPW;  // rows of the block tile
+  constexpr int KB = 8;        // kz planes accumulated in LDS between flushes
+  __shared__ float s_ty[KB][2][3][64];    // [plane][ky parity][j][lane]
+  __shared__ float s_tx[KB][TY][2][3];    // [plane][row][kx parity][i]
   __shared__ float s_rec[4][ARD_MAX_REC_PER_PART];
   const int bid = blockIdx.x;
   const int di = find_desc_ll(starts, ndesc, bid);
@@ -133,15 +149,17 @@ __global__ __launch_bounds__(256) void k_modal(const ModalDesc* __restrict__ des
   const int by = (lb / P.nbx) % P.nby;
   const int bz = lb / (P.nbx * P.nby);
   const int W = P.W, H = P.H, D = P.D;
-  const int lane = threadIdx.x & 63;
-  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform
   const int kx = bx * 64 + lane;
   const bool xok = kx < W;
   const int kxc = xok ? kx : W - 1;
   const int px = kx & 1;
-  const int ky0 = by * (4 * RPW) + wave;
-  const int py = ky0 & 1;  // every row of this wave has the parity of ky0
-  const int kz0 = bz * P.kz_chunk;  // kz_chunk is even, so kz0 is even
+  const int kyb = by * TY;           // first row of the block
+  const int ky0 = kyb + wave;        // rows of this wave: ky0 + 4 r
+  const int py = ky0 & 1;            // shared by every row of this wave
+  const int kz0 = bz * P.kz_chunk;   // kz_chunk is even, so kz0 is even
   const int kz1 = min(D, kz0 + P.kz_chunk);
   const long long HW = (long long)H * W;
   const long long DH = (long long)D * H, DW = (long long)D * W;
@@ -151,12 +169,12 @@ __global__ __launch_bounds__(256) void k_modal(const ModalDesc* __restrict__ des
   const float* __restrict__ Rz = P.Rz;
   const float* __restrict__ Ey = P.Ey;
   const float* __restrict__ Ry = P.Ry;
-  const float* __restrict__ Aa = P.a;
-  const float* __restrict__ Ag = P.g;
+  const float* __restrict__ Wz2 = P.wz2;
+  const float* __restrict__ Wy2 = P.wy2;
   const float* __restrict__ Mc = P.M[par];
   float* __restrict__ Mp = P.M[par ^ 1];
-  float* __restrict__ Txp = P.Tx + (long long)bx * 6 * DH;
-  float* __restrict__ Typ = P.Ty + (long long)by * 6 * DW;
+  const float hdt = 0.5f * P.dt;
+  const float gsc = P.gscale;  // .999 * 4 * s_f * s_i
 
   float ex[3], rx[3];
 #pragma unroll
@@ -164,8 +182,7 @@ __global__ __launch_bounds__(256) void k_modal(const ModalDesc* __restrict__ des
     ex[i] = P.Ex[kxc * 3 + i];
     rx[i] = xok ? P.Rx[kxc * 3 + i] : 0.f;
   }
-  // per-row constants: force expansion along z (both kz parities) and the
-  // z-reduction accumulators (both parities)
+  const float wx2 = P.wx2[kxc];
   float gz[RPW][6], tz[RPW][6];
 #pragma unroll
   for (int r = 0; r < RPW; ++r) {
@@ -177,7 +194,6 @@ __global__ __launch_bounds__(256) void k_modal(const ModalDesc* __restrict__ des
       tz[r][q] = 0.f;
     }
   }
-  // point sources and recorders of this partition: their x factor is fixed per lane
   const int nsrc = PTS ? min(P.nsrc, ARD_MAX_SRC_PER_PART) : 0;
   const int nrec = PTS ? P.nrec : 0;
   float sfx[ARD_MAX_SRC_PER_PART], rfx[ARD_MAX_REC_PER_PART], rec_acc[ARD_MAX_REC_PER_PART];
@@ -188,34 +204,43 @@ __global__ __launch_bounds__(256) void k_modal(const ModalDesc* __restrict__ des
     rfx[q] = PTS && q < nrec && xok ? P.rec[q].fx[kxc] : 0.f;
     rec_acc[q] = 0.f;
   }
-  const float bdamp = 0.999f;
+  for (int e = tid; e < KB * 2 * 3 * 64; e += 256) (&s_ty[0][0][0][0])[e] = 0.f;
+  for (int e = tid; e < KB * TY * 6; e += 256) (&s_tx[0][0][0][0])[e] = 0.f;
+  __syncthreads();
 
-  auto body = [&](const int kz, auto pzc, float (&ty)[3]) {
+  auto body = [&](const int kz, const int kl, auto pzc) {
     constexpr int PZ = decltype(pzc)::value;
     const float ez0 = Ez[kz * 3 + 0], ez1 = Ez[kz * 3 + 1], ez2 = Ez[kz * 3 + 2];
     const float rz0 = Rz[kz * 3 + 0], rz1 = Rz[kz * 3 + 1], rz2 = Rz[kz * 3 + 2];
+    const float wxz2 = wx2 + Wz2[kz];
     float gy[3];
 #pragma unroll
     for (int j = 0; j < 3; ++j) gy[j] = xok ? Gy[(py * 3 + j) * DW + (long long)kz * W + kx] : 0.f;
-    float sz[ARD_MAX_SRC_PER_PART], rz[ARD_MAX_REC_PER_PART];
+    float sz[ARD_MAX_SRC_PER_PART], rzp[ARD_MAX_REC_PER_PART];
 #pragma unroll
     for (int q = 0; q < ARD_MAX_SRC_PER_PART; ++q) sz[q] = PTS && q < nsrc ? P.src[q].fz[kz] : 0.f;
 #pragma unroll
-    for (int q = 0; q < ARD_MAX_REC_PER_PART; ++q) rz[q] = PTS && q < nrec ? P.rec[q].fz[kz] : 0.f;
+    for (int q = 0; q < ARD_MAX_REC_PER_PART; ++q) rzp[q] = PTS && q < nrec ? P.rec[q].fz[kz] : 0.f;
+    float ty[3] = {0.f, 0.f, 0.f};
 #pragma unroll
     for (int r = 0; r < RPW; ++r) {
       const int ky = ky0 + 4 * r;
       if (ky >= H) break;  // uniform across the wave
       const long long gxo = (long long)kz * H + ky;
       const long long o = (long long)kz * HW + (long long)ky * W + kx;
-      float a = 0.f, mc = 0.f, mp = 0.f, g = 0.f;
+      float mc = 0.f, mp = 0.f;
       if (xok) {
-        a = Aa[o];
         mc = Mc[o];
         mp = Mp[o];
-        g = Ag[o];
       }
-      // x-face force spectrum at (ky, kz): uniform across lanes, parity per lane
+      // modal coefficients (ARD/dct_partition.cpp:24-33,68) from the separable
+      // w^2 = wx2 + wy2 + wz2 (1-based wavenumbers):
+      //   a = .999 * 2 cos(w dt) = .999 * 2 (1 - 2 s^2),  g = .999 * 4 s^2 / w^2,  s = sin(w dt / 2)
+      const float w2 = wxz2 + Wy2[ky];
+      const float sn = sin_small(hdt * __builtin_amdgcn_sqrtf(w2));
+      const float s2 = sn * sn;
+      const float a = fmaf(-3.996f, s2, 1.998f);
+      const float g = gsc * s2 * __builtin_amdgcn_rcpf(w2);
       float fh = 0.f;
 #pragma unroll
       for (int i = 0; i < 3; ++i) {
@@ -234,7 +259,7 @@ __global__ __launch_bounds__(256) void k_modal(const ModalDesc* __restrict__ des
         for (int q = 0; q < ARD_MAX_SRC_PER_PART; ++q)
           if (q < nsrc) fh = fmaf(sfx[q], P.src[q].fy[ky] * sz[q], fh);
       }
-      const float mn = xok ? (a * mc - bdamp * mp + g * fh) : 0.f;
+      const float mn = xok ? (a * mc - 0.999f * mp + g * fh) : 0.f;
       if (xok) Mp[o] = mn;
       tz[r][PZ * 3 + 0] = fmaf(rz0, mn, tz[r][PZ * 3 + 0]);
       tz[r][PZ * 3 + 1] = fmaf(rz1, mn, tz[r][PZ * 3 + 1]);
@@ -245,48 +270,55 @@ __global__ __launch_bounds__(256) void k_modal(const ModalDesc* __restrict__ des
       if (PTS) {
 #pragma unroll
         for (int q = 0; q < ARD_MAX_REC_PER_PART; ++q)
-          if (q < nrec) rec_acc[q] = fmaf(rfx[q] * P.rec[q].fy[ky] * rz[q], mn, rec_acc[q]);
+          if (q < nrec) rec_acc[q] = fmaf(rfx[q] * P.rec[q].fy[ky] * rzp[q], mn, rec_acc[q]);
       }
-      // x reduction across lanes, keeping even and odd kx apart
-      float v0 = row_parity_sum(rx[0] * mn);
-      float v1 = row_parity_sum(rx[1] * mn);
-      float v2 = row_parity_sum(rx[2] * mn);
-      v0 += __shfl_xor(v0, 16);
-      v1 += __shfl_xor(v1, 16);
-      v2 += __shfl_xor(v2, 16);
-      v0 += __shfl_xor(v0, 32);
-      v1 += __shfl_xor(v1, 32);
-      v2 += __shfl_xor(v2, 32);
-      if (lane < 2) {
-        Txp[(lane * 3 + 0) * DH + gxo] = v0;
-        Txp[(lane * 3 + 1) * DH + gxo] = v1;
-        Txp[(lane * 3 + 2) * DH + gxo] = v2;
+      // x reduction: parity sums inside each 16-lane row, then the four rows
+      // meet in LDS
+      const float v0 = row_parity_sum(rx[0] * mn);
+      const float v1 = row_parity_sum(rx[1] * mn);
+      const float v2 = row_parity_sum(rx[2] * mn);
+      if ((lane & 14) == 0) {
+        float* t = s_tx[kl][wave + 4 * r][lane & 1];
+        atomicAdd(t + 0, v0);
+        atomicAdd(t + 1, v1);
+        atomicAdd(t + 2, v2);
       }
     }
+    // y reduction: waves 0,2 (even ky) and 1,3 (odd ky) meet in LDS
+#pragma unroll
+    for (int j = 0; j < 3; ++j) atomicAdd(&s_ty[kl][py][j][lane], ty[j]);
   };
-  // y reduction over the rows of the four waves: waves 0,2 hold even ky, waves 1,3 odd ky;
-  // one LDS exchange per pair of kz planes
+  // write the LDS accumulators of planes kb .. kb+n-1 to the partial
+  // reductions and clear them
+  auto flush = [&](const int kb, const int n) {
+    __syncthreads();
+    float* __restrict__ Txp = P.Tx + (long long)bx * 6 * DH;
+    float* __restrict__ Typ = P.Ty + (long long)by * 6 * DW;
+    for (int e = tid; e < n * TY * 6; e += 256) {
+      const int q = e % 6, row = (e / 6) % TY, kl = e / (6 * TY);  // q = parity * 3 + i
+      const int ky = kyb + row;
+      float* src = &s_tx[kl][row][q / 3][q % 3];
+      if (ky < H) Txp[q * DH + (long long)(kb + kl) * H + ky] = *src;
+      *src = 0.f;
+    }
+    for (int e = tid; e < n * 6 * 64; e += 256) {
+      const int l = e & 63, q = (e >> 6) % 6, kl = e / 384;
+      const int kxx = bx * 64 + l;
+      float* src = &s_ty[kl][q / 3][q % 3][l];
+      if (kxx < W) Typ[q * DW + (long long)(kb + kl) * W + kxx] = *src;
+      *src = 0.f;
+    }
+    __syncthreads();
+  };
+  int kb = kz0;
   for (int kz = kz0; kz < kz1; kz += 2) {
-    float ty0[3] = {0.f, 0.f, 0.f}, ty1[3] = {0.f, 0.f, 0.f};
-    const bool two = kz + 1 < kz1;
-    body(kz, std::integral_constant<int, 0>(), ty0);
-    if (two) body(kz + 1, std::integral_constant<int, 1>(), ty1);
-#pragma unroll
-    for (int j = 0; j < 3; ++j) {
-      s_ty[0][wave][j][lane] = ty0[j];
-      s_ty[1][wave][j][lane] = ty1[j];
+    body(kz, kz - kb, std::integral_constant<int, 0>());
+    if (kz + 1 < kz1) body(kz + 1, kz + 1 - kb, std::integral_constant<int, 1>());
+    const int next = kz + 2;
+    if (next - kb >= KB || next >= kz1) {
+      flush(kb, min(next, kz1) - kb);
+      kb = next;
     }
-    __syncthreads();
-    if (xok) {
-      // wave w: plane (w >> 1), parity (w & 1)
-      const int pl = wave >> 1, pw = wave & 1;
-      if (pl == 0 || two) {
-#pragma unroll
-        for (int j = 0; j < 3; ++j)
-          Typ[(pw * 3 + j) * DW + (long long)(kz + pl) * W + kx] = s_ty[pl][pw][j][lane] + s_ty[pl][pw + 2][j][lane];
-      }
-    }
-    __syncthreads();
   }
   // z reduction partials of this kz chunk
   float* tzp = P.Tz + (long long)bz * 6 * HW;
@@ -309,9 +341,9 @@ __global__ __launch_bounds__(256) void k_modal(const ModalDesc* __restrict__ des
       }
     }
     __syncthreads();
-    if (threadIdx.x < nrec)
-      P.rec_part[(long long)lb * ARD_MAX_REC_PER_PART + threadIdx.x] =
-          s_rec[0][threadIdx.x] + s_rec[1][threadIdx.x] + s_rec[2][threadIdx.x] + s_rec[3][threadIdx.x];
+    if (tid < nrec)
+      P.rec_part[(long long)lb * ARD_MAX_REC_PER_PART + tid] =
+          s_rec[0][tid] + s_rec[1][tid] + s_rec[2][tid] + s_rec[3][tid];
   }
 }
 
