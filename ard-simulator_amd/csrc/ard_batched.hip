@@ -384,12 +384,9 @@ int finalize_batched(Ctx& c) {
     const long long W = p.w, H = p.h, D = p.d, N = W * H * D;
     P.plane[0] = D * H; P.plane[1] = D * W; P.plane[2] = H * W;
     // tiling of the modal pass: rows per wave with the least padding, kz chunks of ~48
-    int best = 4, bestw = 1 << 30;
-    for (int rpw : {8, 5, 4}) {
-      const int waste = int(ceil_div(H, 4 * rpw) * 4 * rpw - H);
-      if (waste < bestw) { bestw = waste; best = rpw; }
-    }
-    P.rpw = best;
+    // 4 rows per wave keeps the modal kernel's registers lowest; 5 only when
+    // it saves whole row tiles of padding
+    P.rpw = (ceil_div(H, 20) * 20 - H) + 8 <= (ceil_div(H, 16) * 16 - H) ? 5 : 4;
     P.nbx = int(ceil_div(W, 64));
     P.nby = int(ceil_div(H, 4 * P.rpw));
     P.nbz = int(ceil_div(D, 48));

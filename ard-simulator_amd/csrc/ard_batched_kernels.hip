@@ -141,6 +141,7 @@ __global__ __launch_bounds__(256) void k_modal(const ModalDesc* __restrict__ des
   __shared__ float s_ty[KB][2][3][64];    // [plane][ky parity][j][lane]
   __shared__ float s_tx[KB][TY][2][3];    // [plane][row][kx parity][i]
   __shared__ float s_rec[4][ARD_MAX_REC_PER_PART];
+  __shared__ float s_gz[RPW][6][256];     // thread-private: z-face force spectrum of this thread's rows
   const int bid = blockIdx.x;
   const int di = find_desc_ll(starts, ndesc, bid);
   const ModalDesc& P = descs[di];  // fields used in the loops are copied to locals below
@@ -183,14 +184,13 @@ __global__ __launch_bounds__(256) void k_modal(const ModalDesc* __restrict__ des
     rx[i] = xok ? P.Rx[kxc * 3 + i] : 0.f;
   }
   const float wx2 = P.wx2[kxc];
-  float gz[RPW][6], tz[RPW][6];
+  float tz[RPW][6];
 #pragma unroll
   for (int r = 0; r < RPW; ++r) {
-    const int ky = ky0 + 4 * r;
-    const bool ok = ky < H && xok;
+    const int kyc = min(ky0 + 4 * r, H - 1);
 #pragma unroll
     for (int q = 0; q < 6; ++q) {
-      gz[r][q] = ok ? P.Gz[q * HW + (long long)ky * W + kx] : 0.f;
+      s_gz[r][q][tid] = P.Gz[q * HW + (long long)kyc * W + kxc];
       tz[r][q] = 0.f;
     }
   }
@@ -215,69 +215,82 @@ __global__ __launch_bounds__(256) void k_modal(const ModalDesc* __restrict__ des
     const float wxz2 = wx2 + Wz2[kz];
     float gy[3];
 #pragma unroll
-    for (int j = 0; j < 3; ++j) gy[j] = xok ? Gy[(py * 3 + j) * DW + (long long)kz * W + kx] : 0.f;
+    for (int j = 0; j < 3; ++j) gy[j] = Gy[(py * 3 + j) * DW + (long long)kz * W + kxc];
     float sz[ARD_MAX_SRC_PER_PART], rzp[ARD_MAX_REC_PER_PART];
 #pragma unroll
     for (int q = 0; q < ARD_MAX_SRC_PER_PART; ++q) sz[q] = PTS && q < nsrc ? P.src[q].fz[kz] : 0.f;
 #pragma unroll
     for (int q = 0; q < ARD_MAX_REC_PER_PART; ++q) rzp[q] = PTS && q < nrec ? P.rec[q].fz[kz] : 0.f;
     float ty[3] = {0.f, 0.f, 0.f};
+    // loads of all rows first, at clamped (always valid) addresses, so that
+    // they are in flight together; rows and lanes outside the partition are
+    // masked afterwards
+    float mcv[RPW], mpv[RPW], gxs[RPW][3], gxd[RPW][3], eyv[RPW][3], ryv[RPW][3], wyv[RPW];
+    long long ov[RPW];
+#pragma unroll
+    for (int r = 0; r < RPW; ++r) {
+      const int kyc = min(ky0 + 4 * r, H - 1);
+      const long long gxo = (long long)kz * H + kyc;
+      ov[r] = (long long)kz * HW + (long long)kyc * W + kxc;
+      mcv[r] = Mc[ov[r]];
+      mpv[r] = Mp[ov[r]];
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        gxs[r][i] = Gx[i * DH + gxo];
+        gxd[r][i] = Gx[(3 + i) * DH + gxo];
+        eyv[r][i] = Ey[kyc * 3 + i];
+        ryv[r][i] = Ry[kyc * 3 + i];
+      }
+      wyv[r] = Wy2[kyc];
+    }
 #pragma unroll
     for (int r = 0; r < RPW; ++r) {
       const int ky = ky0 + 4 * r;
-      if (ky >= H) break;  // uniform across the wave
-      const long long gxo = (long long)kz * H + ky;
-      const long long o = (long long)kz * HW + (long long)ky * W + kx;
-      float mc = 0.f, mp = 0.f;
-      if (xok) {
-        mc = Mc[o];
-        mp = Mp[o];
-      }
+      const bool rok = ky < H;  // uniform across the wave
+      const bool ok = rok && xok;
       // modal coefficients (ARD/dct_partition.cpp:24-33,68) from the separable
       // w^2 = wx2 + wy2 + wz2 (1-based wavenumbers):
       //   a = .999 * 2 cos(w dt) = .999 * 2 (1 - 2 s^2),  g = .999 * 4 s^2 / w^2,  s = sin(w dt / 2)
-      const float w2 = wxz2 + Wy2[ky];
+      const float w2 = wxz2 + wyv[r];
       const float sn = sin_small(hdt * __builtin_amdgcn_sqrtf(w2));
       const float s2 = sn * sn;
       const float a = fmaf(-3.996f, s2, 1.998f);
       const float g = gsc * s2 * __builtin_amdgcn_rcpf(w2);
       float fh = 0.f;
 #pragma unroll
-      for (int i = 0; i < 3; ++i) {
-        const float gs = Gx[i * DH + gxo];
-        const float gd = Gx[(3 + i) * DH + gxo];
-        fh = fmaf(ex[i], px ? gd : gs, fh);
-      }
-      fh = fmaf(Ey[ky * 3 + 0], gy[0], fh);
-      fh = fmaf(Ey[ky * 3 + 1], gy[1], fh);
-      fh = fmaf(Ey[ky * 3 + 2], gy[2], fh);
-      fh = fmaf(ez0, gz[r][PZ * 3 + 0], fh);
-      fh = fmaf(ez1, gz[r][PZ * 3 + 1], fh);
-      fh = fmaf(ez2, gz[r][PZ * 3 + 2], fh);
+      for (int i = 0; i < 3; ++i) fh = fmaf(ex[i], px ? gxd[r][i] : gxs[r][i], fh);
+      fh = fmaf(eyv[r][0], gy[0], fh);
+      fh = fmaf(eyv[r][1], gy[1], fh);
+      fh = fmaf(eyv[r][2], gy[2], fh);
+      fh = fmaf(ez0, s_gz[r][PZ * 3 + 0][tid], fh);
+      fh = fmaf(ez1, s_gz[r][PZ * 3 + 1][tid], fh);
+      fh = fmaf(ez2, s_gz[r][PZ * 3 + 2][tid], fh);
       if (PTS) {
+        const int kyc = min(ky, H - 1);
 #pragma unroll
         for (int q = 0; q < ARD_MAX_SRC_PER_PART; ++q)
-          if (q < nsrc) fh = fmaf(sfx[q], P.src[q].fy[ky] * sz[q], fh);
+          if (q < nsrc) fh = fmaf(sfx[q], P.src[q].fy[kyc] * sz[q], fh);
       }
-      const float mn = xok ? (a * mc - 0.999f * mp + g * fh) : 0.f;
-      if (xok) Mp[o] = mn;
+      const float mn = ok ? (a * mcv[r] - 0.999f * mpv[r] + g * fh) : 0.f;
+      if (ok) Mp[ov[r]] = mn;
       tz[r][PZ * 3 + 0] = fmaf(rz0, mn, tz[r][PZ * 3 + 0]);
       tz[r][PZ * 3 + 1] = fmaf(rz1, mn, tz[r][PZ * 3 + 1]);
       tz[r][PZ * 3 + 2] = fmaf(rz2, mn, tz[r][PZ * 3 + 2]);
-      ty[0] = fmaf(Ry[ky * 3 + 0], mn, ty[0]);
-      ty[1] = fmaf(Ry[ky * 3 + 1], mn, ty[1]);
-      ty[2] = fmaf(Ry[ky * 3 + 2], mn, ty[2]);
+      ty[0] = fmaf(ryv[r][0], mn, ty[0]);
+      ty[1] = fmaf(ryv[r][1], mn, ty[1]);
+      ty[2] = fmaf(ryv[r][2], mn, ty[2]);
       if (PTS) {
+        const int kyc = min(ky, H - 1);
 #pragma unroll
         for (int q = 0; q < ARD_MAX_REC_PER_PART; ++q)
-          if (q < nrec) rec_acc[q] = fmaf(rfx[q] * P.rec[q].fy[ky] * rzp[q], mn, rec_acc[q]);
+          if (q < nrec) rec_acc[q] = fmaf(rfx[q] * P.rec[q].fy[kyc] * rzp[q], mn, rec_acc[q]);
       }
       // x reduction: parity sums inside each 16-lane row, then the four rows
       // meet in LDS
       const float v0 = row_parity_sum(rx[0] * mn);
       const float v1 = row_parity_sum(rx[1] * mn);
       const float v2 = row_parity_sum(rx[2] * mn);
-      if ((lane & 14) == 0) {
+      if (rok && (lane & 14) == 0) {
         float* t = s_tx[kl][wave + 4 * r][lane & 1];
         atomicAdd(t + 0, v0);
         atomicAdd(t + 1, v1);

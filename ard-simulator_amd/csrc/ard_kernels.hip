@@ -69,21 +69,24 @@ __global__ __launch_bounds__(256) void k_gemm(const GemmDesc* __restrict__ descs
     for (int r = 0; r < 4; ++r) {
       const int k = tid & 15, m = (tid >> 4) + 16 * r;
       const int gm = m0 + m, gk = k0 + k;
-      As[k][m] = (gm < M && gk < K) ? A[(long long)gm * K + gk] : 0.f;
+      const float v = A[(long long)min(gm, M - 1) * K + min(gk, K - 1)];  // clamped: loads stay unconditional
+      As[k][m] = (gm < M && gk < K) ? v : 0.f;
     }
     if (B_K_CONTIG) {
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int k = tid & 15, n = (tid >> 4) + 16 * r;
         const int gk = k0 + k;
-        Bs[k][n] = (bn[r] < N && gk < K) ? B[(long long)gk * g.bK + boff[r]] : 0.f;
+        const float v = B[(long long)min(gk, K - 1) * g.bK + boff[r]];
+        Bs[k][n] = (bn[r] < N && gk < K) ? v : 0.f;
       }
     } else {
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int k = (tid >> 6) + 4 * r, n = tid & 63;
         const int gk = k0 + k;
-        Bs[k][n] = (bn[0] < N && gk < K) ? B[(long long)gk * g.bK + boff[0]] : 0.f;
+        const float v = B[(long long)min(gk, K - 1) * g.bK + boff[0]];
+        Bs[k][n] = (bn[0] < N && gk < K) ? v : 0.f;
       }
     }
     __syncthreads();
@@ -184,21 +187,24 @@ __global__ __launch_bounds__(256) void k_gemm_mfma(const GemmDesc* __restrict__ 
     for (int r = 0; r < 4; ++r) {
       const int k = tid & 15, m = (tid >> 4) + 16 * r;
       const int gm = m0 + m, gk = k0 + k;
-      As[k][m] = (gm < M && gk < K) ? A[(long long)gm * K + gk] : 0.f;
+      const float v = A[(long long)min(gm, M - 1) * K + min(gk, K - 1)];  // clamped: loads stay unconditional
+      As[k][m] = (gm < M && gk < K) ? v : 0.f;
     }
     if (B_K_CONTIG) {
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int k = tid & 15, n = (tid >> 4) + 16 * r;
         const int gk = k0 + k;
-        Bs[k][n] = (bn[r] < N && gk < K) ? B[(long long)gk * g.bK + boff[r]] : 0.f;
+        const float v = B[(long long)min(gk, K - 1) * g.bK + boff[r]];
+        Bs[k][n] = (bn[r] < N && gk < K) ? v : 0.f;
       }
     } else {
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int k = (tid >> 6) + 4 * r, n = tid & 63;
         const int gk = k0 + k;
-        Bs[k][n] = (bn[0] < N && gk < K) ? B[(long long)gk * g.bK + boff[0]] : 0.f;
+        const float v = B[(long long)min(gk, K - 1) * g.bK + boff[0]];
+        Bs[k][n] = (bn[0] < N && gk < K) ? v : 0.f;
       }
     }
     __syncthreads();
@@ -280,8 +286,13 @@ __global__ __launch_bounds__(256) void k_pml(const PmlDesc* __restrict__ descs, 
   const float* __restrict__ fy = P.phi[par][1];
   const float* __restrict__ fz = P.phi[par][2];
   const long long sx = 1, sy = W, sz = (long long)W * H;
+  // neighbour read with the zero sentinel outside the slab; the load itself
+  // goes to a clamped (valid) address so that it is never conditional
   auto at = [&](const float* a, int x, int y, int z) -> float {
-    return (x < 0 || x >= W || y < 0 || y >= H || z < 0 || z >= D) ? 0.f : a[z * sz + y * sy + x * sx];
+    const bool in = !(x < 0 || x >= W || y < 0 || y >= H || z < 0 || z >= D);
+    const int xc = min(max(x, 0), W - 1), yc = min(max(y, 0), H - 1), zc = min(max(z, 0), D - 1);
+    const float v = a[zc * sz + yc * sy + xc * sx];
+    return in ? v : 0.f;
   };
   const float c6[7] = {2.f, -27.f, 270.f, -490.f, 270.f, -27.f, 2.f};
   const float c4[5] = {1.f, -8.f, 0.f, 8.f, -1.f};
@@ -366,8 +377,10 @@ __device__ __forceinline__ float read_p(const PartAccess& A, int par, int x, int
     const long long o = shell_index(A, x, y, z, &w);
     return A.ps[w][o];
   }
-  if (x < 0 || x >= A.W || y < 0 || y >= A.H || z < 0 || z >= A.D) return 0.f;
-  return A.p[par][((long long)z * A.H + y) * A.W + x];
+  const bool in = !(x < 0 || x >= A.W || y < 0 || y >= A.H || z < 0 || z >= A.D);
+  const int xc = min(max(x, 0), A.W - 1), yc = min(max(y, 0), A.H - 1), zc = min(max(z, 0), A.D - 1);
+  const float v = A.p[par][((long long)zc * A.H + yc) * A.W + xc];
+  return in ? v : 0.f;
 }
 
 __device__ __forceinline__ void write_f(const PartAccess& A, int x, int y, int z, float v) {
