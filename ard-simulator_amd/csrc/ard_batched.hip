@@ -324,7 +324,7 @@ int build_masks(Ctx& c, std::vector<BndDesc>& descs, const std::vector<long long
   for (int b = 0; b < int(descs.size()); ++b) {
     const BndDesc& d = descs[b];
     for (long long e = 0; e < elems[b]; ++e) {
-      const int u = d.t0s + int(e / d.t1n), v = d.t1s + int(e % d.t1n);
+      const int u = d.t0s + int(e % d.t0n), v = d.t1s + int(e / d.t0n);  // k_interface's element order
       for (int r = 0; r < 6; ++r) {
         int* s;
         target(d, r < 3 ? 0 : 1, u, v, r - 3, &s);
@@ -339,7 +339,7 @@ int build_masks(Ctx& c, std::vector<BndDesc>& descs, const std::vector<long long
     const BndDesc& d = descs[b];
     mstart[b] = (long long)mask.size();
     for (long long e = 0; e < elems[b]; ++e) {
-      const int u = d.t0s + int(e / d.t1n), v = d.t1s + int(e % d.t1n);
+      const int u = d.t0s + int(e % d.t0n), v = d.t1s + int(e / d.t0n);
       uint8_t m = 0;
       for (int r = 0; r < 6; ++r) {
         int* s;

@@ -46,12 +46,15 @@ __global__ __launch_bounds__(256) void k_fold(const FoldDesc* __restrict__ d, co
   const FoldDesc& F = d[i];
   const long long le = e - starts[i];
   if (le >= 3 * F.plane) return;
-  const int q = int(le / F.plane);
-  const long long o = le % F.plane;
-  const float a = F.src[q * F.plane + o];
-  const float b = F.src[(5 - q) * F.plane + o];
-  F.dst[q * F.plane + o] = a + b;
-  F.dst[(3 + q) * F.plane + o] = a - b;
+  const unsigned pl = unsigned(F.plane);  // planes hold < 2^31 cells
+  const int q = int(unsigned(le) / pl);
+  const long long o = unsigned(le) - unsigned(q) * pl;
+  const gfloat* src = ARD_CG(F.src);
+  gfloat* dst = ARD_G(F.dst);
+  const float a = src[q * F.plane + o];
+  const float b = src[(5 - q) * F.plane + o];
+  dst[q * F.plane + o] = a + b;
+  dst[(3 + q) * F.plane + o] = a - b;
 }
 
 // ---------------------------------------------------------------------------
@@ -65,16 +68,17 @@ __global__ __launch_bounds__(256) void k_unfold(const UnfoldDesc* __restrict__ d
   const UnfoldDesc& U = d[i];
   const long long le = e - starts[i];
   if (le >= 3 * U.plane) return;
-  const int q = int(le / U.plane);
-  const long long o = le % U.plane;
+  const unsigned pl = unsigned(U.plane);  // planes hold < 2^31 cells
+  const int q = int(unsigned(le) / pl);
+  const long long o = unsigned(le) - unsigned(q) * pl;
   float ev = 0.f, od = 0.f;
   for (int c = 0; c < U.nchunk; ++c) {
-    const float* p = U.part + (long long)c * 6 * U.plane;
+    const gfloat* p = ARD_CG(U.part) + (long long)c * 6 * U.plane;
     ev += p[q * U.plane + o];
     od += p[(3 + q) * U.plane + o];
   }
-  U.dst[q * U.plane + o] = ev + od;
-  U.dst[(5 - q) * U.plane + o] = ev - od;
+  ARD_G(U.dst)[q * U.plane + o] = ev + od;
+  ARD_G(U.dst)[(5 - q) * U.plane + o] = ev - od;
 }
 
 // ---------------------------------------------------------------------------
@@ -164,33 +168,33 @@ __global__ __launch_bounds__(256) void k_modal(const ModalDesc* __restrict__ des
   const int kz1 = min(D, kz0 + P.kz_chunk);
   const long long HW = (long long)H * W;
   const long long DH = (long long)D * H, DW = (long long)D * W;
-  const float* __restrict__ Gx = P.Gx;
-  const float* __restrict__ Gy = P.Gy;
-  const float* __restrict__ Ez = P.Ez;
-  const float* __restrict__ Rz = P.Rz;
-  const float* __restrict__ Ey = P.Ey;
-  const float* __restrict__ Ry = P.Ry;
-  const float* __restrict__ Wz2 = P.wz2;
-  const float* __restrict__ Wy2 = P.wy2;
-  const float* __restrict__ Mc = P.M[par];
-  float* __restrict__ Mp = P.M[par ^ 1];
+  const gfloat* __restrict__ Gx = ARD_CG(P.Gx);
+  const gfloat* __restrict__ Gy = ARD_CG(P.Gy);
+  const gfloat* __restrict__ Ez = ARD_CG(P.Ez);
+  const gfloat* __restrict__ Rz = ARD_CG(P.Rz);
+  const gfloat* __restrict__ Ey = ARD_CG(P.Ey);
+  const gfloat* __restrict__ Ry = ARD_CG(P.Ry);
+  const gfloat* __restrict__ Wz2 = ARD_CG(P.wz2);
+  const gfloat* __restrict__ Wy2 = ARD_CG(P.wy2);
+  const gfloat* __restrict__ Mc = ARD_CG(P.M[par]);
+  gfloat* __restrict__ Mp = ARD_G(P.M[par ^ 1]);
   const float hdt = 0.5f * P.dt;
   const float gsc = P.gscale;  // .999 * 4 * s_f * s_i
 
   float ex[3], rx[3];
 #pragma unroll
   for (int i = 0; i < 3; ++i) {
-    ex[i] = P.Ex[kxc * 3 + i];
-    rx[i] = xok ? P.Rx[kxc * 3 + i] : 0.f;
+    ex[i] = ARD_CG(P.Ex)[kxc * 3 + i];
+    rx[i] = xok ? ARD_CG(P.Rx)[kxc * 3 + i] : 0.f;
   }
-  const float wx2 = P.wx2[kxc];
+  const float wx2 = ARD_CG(P.wx2)[kxc];
   float tz[RPW][6];
 #pragma unroll
   for (int r = 0; r < RPW; ++r) {
     const int kyc = min(ky0 + 4 * r, H - 1);
 #pragma unroll
     for (int q = 0; q < 6; ++q) {
-      s_gz[r][q][tid] = P.Gz[q * HW + (long long)kyc * W + kxc];
+      s_gz[r][q][tid] = ARD_CG(P.Gz)[q * HW + (long long)kyc * W + kxc];
       tz[r][q] = 0.f;
     }
   }
@@ -305,8 +309,8 @@ __global__ __launch_bounds__(256) void k_modal(const ModalDesc* __restrict__ des
   // reductions and clear them
   auto flush = [&](const int kb, const int n) {
     __syncthreads();
-    float* __restrict__ Txp = P.Tx + (long long)bx * 6 * DH;
-    float* __restrict__ Typ = P.Ty + (long long)by * 6 * DW;
+    gfloat* __restrict__ Txp = ARD_G(P.Tx) + (long long)bx * 6 * DH;
+    gfloat* __restrict__ Typ = ARD_G(P.Ty) + (long long)by * 6 * DW;
     for (int e = tid; e < n * TY * 6; e += 256) {
       const int q = e % 6, row = (e / 6) % TY, kl = e / (6 * TY);  // q = parity * 3 + i
       const int ky = kyb + row;
@@ -334,7 +338,7 @@ __global__ __launch_bounds__(256) void k_modal(const ModalDesc* __restrict__ des
     }
   }
   // z reduction partials of this kz chunk
-  float* tzp = P.Tz + (long long)bz * 6 * HW;
+  gfloat* tzp = ARD_G(P.Tz) + (long long)bz * 6 * HW;
 #pragma unroll
   for (int r = 0; r < RPW; ++r) {
     const int ky = ky0 + 4 * r;

@@ -7,6 +7,13 @@
 
 namespace ard {
 
+// Pointers read from descriptor tables are generic pointers to the compiler,
+// which then emits flat memory instructions; these casts state that they
+// point to global memory, so it can use global and scalar loads.
+typedef __attribute__((address_space(1))) float gfloat;
+#define ARD_G(p) ((gfloat*)(p))
+#define ARD_CG(p) ((const gfloat*)(p))
+
 // One problem of a grouped, strided GEMM  C = alpha * A * B  (A: M x K row-major).
 // B(k, n) = B[k*bK + (n % N1)*b1 + (n / N1)*b2], C(m, n) likewise with cM, c1, c2.
 // Every per-axis DCT pass is one such problem: A is the cosine matrix of the

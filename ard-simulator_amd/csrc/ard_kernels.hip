@@ -61,8 +61,8 @@ __global__ __launch_bounds__(256) void k_gemm(const GemmDesc* __restrict__ descs
 #pragma unroll
     for (int j = 0; j < 4; ++j) acc[i][j] = 0.f;
 
-  const float* __restrict__ A = g.A;
-  const float* __restrict__ B = g.B;
+  const gfloat* __restrict__ A = ARD_CG(g.A);
+  const gfloat* __restrict__ B = ARD_CG(g.B);
   for (int k0 = 0; k0 < K; k0 += 16) {
     // A tile: As[k][m] = A[m0+m][k0+k]
 #pragma unroll
@@ -112,7 +112,7 @@ __global__ __launch_bounds__(256) void k_gemm(const GemmDesc* __restrict__ descs
     nok[j] = n < N;
     coff[j] = nok[j] ? (long long)(n % N1) * g.c1 + (long long)(n / N1) * g.c2 : 0;
   }
-  float* __restrict__ C = g.C;
+  gfloat* __restrict__ C = ARD_G(g.C);
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
     const int m = m0 + 4 * tm + i;
@@ -126,7 +126,7 @@ __global__ __launch_bounds__(256) void k_gemm(const GemmDesc* __restrict__ descs
       if (EP == 0) {
         C[o] = v;
       } else {
-        C[o] = g.ep_a[o] * g.ep_cur[o] - g.ep_b * C[o] + g.ep_g[o] * v;
+        C[o] = ARD_CG(g.ep_a)[o] * ARD_CG(g.ep_cur)[o] - g.ep_b * C[o] + ARD_CG(g.ep_g)[o] * v;
       }
     }
   }
@@ -180,8 +180,8 @@ __global__ __launch_bounds__(256) void k_gemm_mfma(const GemmDesc* __restrict__ 
 #pragma unroll
   for (int r = 0; r < 16; ++r) acc[r] = 0.f;
 
-  const float* __restrict__ A = g.A;
-  const float* __restrict__ B = g.B;
+  const gfloat* __restrict__ A = ARD_CG(g.A);
+  const gfloat* __restrict__ B = ARD_CG(g.B);
   for (int k0 = 0; k0 < K; k0 += 16) {
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
@@ -219,7 +219,7 @@ __global__ __launch_bounds__(256) void k_gemm_mfma(const GemmDesc* __restrict__ 
   const int j = n0 + wn + (lane & 31);
   if (j >= N) return;
   const long long cn = (long long)(j % N1) * g.c1 + (long long)(j / N1) * g.c2;
-  float* __restrict__ C = g.C;
+  gfloat* __restrict__ C = ARD_G(g.C);
 #pragma unroll
   for (int r = 0; r < 16; ++r) {
     const int i = m0 + wm + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
@@ -227,7 +227,7 @@ __global__ __launch_bounds__(256) void k_gemm_mfma(const GemmDesc* __restrict__ 
     const long long o = (long long)i * g.cM + cn;
     const float v = g.alpha * acc[r];
     if (EP == 0) C[o] = v;
-    else C[o] = g.ep_a[o] * g.ep_cur[o] - g.ep_b * C[o] + g.ep_g[o] * v;
+    else C[o] = ARD_CG(g.ep_a)[o] * ARD_CG(g.ep_cur)[o] - g.ep_b * C[o] + ARD_CG(g.ep_g)[o] * v;
   }
 }
 
@@ -275,20 +275,21 @@ __global__ __launch_bounds__(256) void k_pml(const PmlDesc* __restrict__ descs, 
   }
   const PmlDesc& P = descs[lo];
   const int W = P.W, H = P.H, D = P.D;
-  const long long cell = (b - P.block0) * 256 + threadIdx.x;
-  if (cell >= (long long)W * H * D) return;
-  const int i = int(cell % W);
-  const int j = int((cell / W) % H);
-  const int k = int(cell / ((long long)W * H));
-  const float* __restrict__ p = P.p[par];
-  float* __restrict__ pn = P.p[par ^ 1];  // holds p_old on entry
-  const float* __restrict__ fx = P.phi[par][0];
-  const float* __restrict__ fy = P.phi[par][1];
-  const float* __restrict__ fz = P.phi[par][2];
+  const unsigned cell = unsigned(b - P.block0) * 256u + threadIdx.x;  // slabs hold < 2^31 cells
+  if (cell >= unsigned(W * H * D)) return;
+  const unsigned row = cell / unsigned(W);
+  const int i = int(cell - row * unsigned(W));
+  const int j = int(row % unsigned(H));
+  const int k = int(row / unsigned(H));
+  const gfloat* __restrict__ p = ARD_CG(P.p[par]);
+  gfloat* __restrict__ pn = ARD_G(P.p[par ^ 1]);  // holds p_old on entry
+  const gfloat* __restrict__ fx = ARD_CG(P.phi[par][0]);
+  const gfloat* __restrict__ fy = ARD_CG(P.phi[par][1]);
+  const gfloat* __restrict__ fz = ARD_CG(P.phi[par][2]);
   const long long sx = 1, sy = W, sz = (long long)W * H;
   // neighbour read with the zero sentinel outside the slab; the load itself
   // goes to a clamped (valid) address so that it is never conditional
-  auto at = [&](const float* a, int x, int y, int z) -> float {
+  auto at = [&](const gfloat* a, int x, int y, int z) -> float {
     const bool in = !(x < 0 || x >= W || y < 0 || y >= H || z < 0 || z >= D);
     const int xc = min(max(x, 0), W - 1), yc = min(max(y, 0), H - 1), zc = min(max(z, 0), D - 1);
     const float v = a[zc * sz + yc * sy + xc * sx];
@@ -307,7 +308,7 @@ __global__ __launch_bounds__(256) void k_pml(const PmlDesc* __restrict__ descs, 
   dxx /= inv180;
   dyy /= inv180;
   dzz /= inv180;
-  const float z = P.zprof[P.axis == 0 ? i : (P.axis == 1 ? j : k)];
+  const float z = ARD_CG(P.zprof)[P.axis == 0 ? i : (P.axis == 1 ? j : k)];
   const float zx = P.axis == 0 ? z : 0.f, zy = P.axis == 1 ? z : 0.f, zz = P.axis == 2 ? z : 0.f;
   const float pc = p[cell];
   const float po = pn[cell];
@@ -331,13 +332,13 @@ __global__ __launch_bounds__(256) void k_pml(const PmlDesc* __restrict__ descs, 
   dpx /= d12; dpy /= d12; dpz /= d12;
   dux /= d12; duy /= d12; duz /= d12;
   const float term6 = dpx + dpy + dpz;
-  float* __restrict__ f = P.f;
+  gfloat* __restrict__ f = ARD_G(P.f);
   const float fc = f[cell];
   pn[cell] = term1 + term2 + dt * dt * (term3 + term4 + term5 + term6 + fc);
   const float phx = fx[cell], phy = fy[cell], phz = fz[cell];
-  P.phi[par ^ 1][0][cell] = phx - dt * zx * phx + dt * (zy + zz - zx) * dux;
-  P.phi[par ^ 1][1][cell] = phy - dt * zy * phy + dt * (zx + zz - zy) * duy;
-  P.phi[par ^ 1][2][cell] = phz - dt * zz * phz + dt * (zx + zy - zz) * duz;
+  ARD_G(P.phi[par ^ 1][0])[cell] = phx - dt * zx * phx + dt * (zy + zz - zx) * dux;
+  ARD_G(P.phi[par ^ 1][1])[cell] = phy - dt * zy * phy + dt * (zx + zz - zy) * duy;
+  ARD_G(P.phi[par ^ 1][2])[cell] = phz - dt * zz * phz + dt * (zx + zy - zz) * duz;
   if (fc != 0.f) f[cell] = 0.f;  // force cleared after the update (:199)
 }
 
@@ -375,11 +376,11 @@ __device__ __forceinline__ float read_p(const PartAccess& A, int par, int x, int
   if (A.kind == 2) {
     int w;
     const long long o = shell_index(A, x, y, z, &w);
-    return A.ps[w][o];
+    return ARD_CG(A.ps[w])[o];
   }
   const bool in = !(x < 0 || x >= A.W || y < 0 || y >= A.H || z < 0 || z >= A.D);
   const int xc = min(max(x, 0), A.W - 1), yc = min(max(y, 0), A.H - 1), zc = min(max(z, 0), A.D - 1);
-  const float v = A.p[par][((long long)zc * A.H + yc) * A.W + xc];
+  const float v = ARD_CG(A.p[par])[((long long)zc * A.H + yc) * A.W + xc];
   return in ? v : 0.f;
 }
 
@@ -387,10 +388,10 @@ __device__ __forceinline__ void write_f(const PartAccess& A, int x, int y, int z
   if (A.kind == 2) {
     int w;
     const long long o = shell_index(A, x, y, z, &w);
-    A.fs[w][o] = v;
+    ARD_G(A.fs[w])[o] = v;
     return;
   }
-  A.f[((long long)z * A.H + y) * A.W + x] = v;
+  ARD_G(A.f)[((long long)z * A.H + y) * A.W + x] = v;
 }
 
 __global__ __launch_bounds__(256) void k_interface(const BndDesc* __restrict__ descs, int ndesc,
@@ -405,9 +406,12 @@ __global__ __launch_bounds__(256) void k_interface(const BndDesc* __restrict__ d
     else hi = mid - 1;
   }
   const BndDesc& B = descs[lo];
-  const long long le = e - B.elem0;
-  const int u = B.t0s + int(le / B.t1n);  // outer tangential (reference loop order)
-  const int v = B.t1s + int(le % B.t1n);
+  // elements of one boundary write distinct cells, so their order is free:
+  // the first tangential coordinate runs fastest (contiguous in the face slabs)
+  const unsigned le = unsigned(e - B.elem0);
+  const unsigned lv = le / unsigned(B.t0n);
+  const int u = B.t0s + int(le - lv * unsigned(B.t0n));
+  const int v = B.t1s + int(lv);
   const PartAccess& S0 = parts[B.side[0]];
   const PartAccess& S1 = parts[B.side[1]];
   const int u0 = u - B.org[0][0], v0 = v - B.org[0][1];
@@ -429,7 +433,7 @@ __global__ __launch_bounds__(256) void k_interface(const BndDesc* __restrict__ d
   }
   const float C[6][6] = {{0, 0, -2, 2, 0, 0},       {0, -2, 27, -27, 2, 0},  {-2, 27, -270, 270, -27, 2},
                          {2, -27, 270, -270, 27, -2}, {0, 2, -27, 27, -2, 0}, {0, 0, 2, -2, 0, 0}};
-  const unsigned mask = B.mask ? B.mask[le] : 0x3fu;
+  const unsigned mask = B.mask ? ((const __attribute__((address_space(1))) uint8_t*)B.mask)[le] : 0x3fu;
 #pragma unroll
   for (int r = 0; r < 6; ++r) {
     if (!(mask & (1u << r))) continue;
