@@ -26,7 +26,7 @@ EXPORTED = [
     "ard_finalize", "ard_num_partitions", "ard_num_boundaries", "ard_num_recorders", "ard_partition_info",
     "ard_boundary_info", "ard_recorder_partition", "ard_step", "ard_time_step", "ard_fetch_recorders",
     "ard_copy_pressure_field", "ard_get_pressure", "ard_set_force", "ard_synchronize", "ard_set_kernel_timing",
-    "ard_kernel_times", "ard_selftest_transform", "ard_set_transform_variant",
+    "ard_kernel_times", "ard_selftest_transform", "ard_set_transform_variant", "ard_batched_config",
 ]
 PHASES = ["sources", "pml", "fold", "face_fwd1", "face_fwd2", "modal", "unfold", "face_inv1", "face_inv2",
           "interface", "record", "dct_fwd", "dct_inv"]
@@ -40,7 +40,7 @@ class Params(ctypes.Structure):
     _fields_ = [
         ("h", ctypes.c_float), ("dt", ctypes.c_float), ("c0", ctypes.c_float), ("pml_layers", ctypes.c_int),
         ("absorption", ctypes.c_float), ("form", ctypes.c_int), ("recorder_mode", ctypes.c_int),
-        ("reserved", ctypes.c_int * 9),
+        ("autotune", ctypes.c_int), ("reserved", ctypes.c_int * 8),
     ]
 
 
@@ -82,6 +82,7 @@ def lib():
         L.ard_kernel_times.argtypes = [vp, fp, ip, i]
         L.ard_selftest_transform.argtypes = [i, i, i, i, i, i, fp, fp, fp]
         L.ard_set_transform_variant.argtypes = [i]
+        L.ard_batched_config.argtypes = [vp, ip]
         _lib = L
     return _lib
 
@@ -119,11 +120,12 @@ class Simulation:
     whose topology can be inspected without a GPU.
     """
 
-    def __init__(self, scene, form=FORM_PLAIN, device=0, pml_scale=None, finalize=True):
+    def __init__(self, scene, form=FORM_PLAIN, device=0, pml_scale=None, finalize=True, autotune=False):
         self.L = lib()
         self.scene = scene
         self.form = form
-        p = Params(scene.h, scene.dt, scene.c0, scene.pml_layers, scene.absorption, form, scene.recorder_mode)
+        p = Params(scene.h, scene.dt, scene.c0, scene.pml_layers, scene.absorption, form, scene.recorder_mode,
+                   int(autotune))
         h = ctypes.c_void_p()
         _check(self.L.ard_ctx_create(device, ctypes.byref(p), ctypes.byref(h)))
         self.ctx = h
@@ -218,3 +220,9 @@ class Simulation:
         cnt = np.zeros(n, dtype=np.int32)
         _check(self.L.ard_kernel_times(self.ctx, _fp(ms), cnt.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), n))
         return {PHASES[i]: (float(ms[i]), int(cnt[i])) for i in range(n) if cnt[i]}
+
+    def batched_config(self):
+        """(rows per wave, kz planes per block, transform variant, candidates timed)."""
+        out = (ctypes.c_int * 4)()
+        _check(self.L.ard_batched_config(self.ctx, out))
+        return tuple(out)

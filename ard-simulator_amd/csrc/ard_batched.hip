@@ -48,6 +48,9 @@ int shell_of(int W, int H, int D, int x, int y, int z, long long* off) {
 }
 }  // namespace
 
+int apply_tiling(Ctx& c, int rpw, int kzc);
+int autotune(Ctx& c);
+
 struct BPart {
   // sizes of the three face-slab families: [6][D][H], [6][D][W], [6][H][W]
   long long plane[3] = {0, 0, 0};
@@ -110,6 +113,8 @@ struct BatchedState {
   GemmDesc* d_dense = nullptr;
   const float* graph_trace = nullptr;
   bool graphs_valid = false;
+  // tiling of the modal pass and transform kernel, chosen at finalize
+  int rpw = 4, kz_chunk = 48, gemm_variant = 1, tuned = 0;
 };
 
 namespace {
@@ -364,6 +369,48 @@ int build_masks(Ctx& c, std::vector<BndDesc>& descs, const std::vector<long long
 
 }  // namespace
 
+
+// Sets the modal-pass tiling of every DCT partition (rows per wave, kz planes
+// per block) and rebuilds the descriptors that depend on it.
+int apply_tiling(Ctx& c, int rpw, int kzc) {
+  BatchedState& B = *c.bs;
+  int r;
+  std::vector<UnfoldDesc> unfolds;
+  for (int i = 0; i < int(c.topo.parts.size()); ++i) {
+    const PartGeom& p = c.topo.parts[i];
+    if (p.is_pml) continue;
+    BPart& P = B.parts[i];
+    P.rpw = rpw;
+    P.nbx = int(ceil_div(p.w, 64));
+    P.nby = int(ceil_div(p.h, 4 * rpw));
+    P.nbz = int(ceil_div(p.d, kzc));
+    P.kz_chunk = int(ceil_div(ceil_div(p.d, P.nbz), 2) * 2);  // even: each block starts on an even plane
+    P.nbz = int(ceil_div(p.d, P.kz_chunk));
+    for (int w = 0; w < 3; ++w)
+      unfolds.push_back(UnfoldDesc{P.T[w], P.S[w], P.plane[w], w == 0 ? P.nbx : (w == 1 ? P.nby : P.nbz)});
+  }
+  std::vector<long long> su;
+  long long t = 0;
+  for (auto& u : unfolds) { su.push_back(t); t += 3 * u.plane; }
+  B.nunfold = int(unfolds.size());
+  B.unfold_total = t;
+  if ((r = upload_vec(c, (void**)&B.d_unfold, unfolds.data(), unfolds.size() * sizeof(UnfoldDesc)))) return r;
+  if ((r = upload_vec(c, (void**)&B.d_unfold_start, su.data(), su.size() * 8))) return r;
+  if ((r = build_modal(c))) return r;
+  std::vector<BRecDesc> rd;
+  for (size_t q = 0; q < c.topo.recorders.size(); ++q) {
+    const BPart& P = B.parts[c.topo.recorders[q].part];
+    const int slot = int(std::find(P.recs.begin(), P.recs.end(), int(q)) - P.recs.begin());
+    rd.push_back(BRecDesc{P.rec_part + slot, P.nbx * P.nby * P.nbz});
+  }
+  if (!rd.empty() && (r = upload_vec(c, (void**)&B.d_rec, rd.data(), rd.size() * sizeof(BRecDesc)))) return r;
+  B.rpw = rpw;
+  B.kz_chunk = kzc;
+  B.graphs_valid = false;
+  hipError_t e = hipStreamSynchronize(c.stream);
+  return e == hipSuccess ? ARD_OK : hip_fail(e, "hipStreamSynchronize");
+}
+
 int finalize_batched(Ctx& c) {
   int r;
   if ((r = setup_common(c))) return r;
@@ -383,15 +430,13 @@ int finalize_batched(Ctx& c) {
       return set_error(ARD_ERR_UNSUPPORTED, "more than 4 recorders in one partition");
     const long long W = p.w, H = p.h, D = p.d, N = W * H * D;
     P.plane[0] = D * H; P.plane[1] = D * W; P.plane[2] = H * W;
-    // tiling of the modal pass: rows per wave with the least padding, kz chunks of ~48
-    // 4 rows per wave keeps the modal kernel's registers lowest; 5 only when
-    // it saves whole row tiles of padding
-    P.rpw = (ceil_div(H, 20) * 20 - H) + 8 <= (ceil_div(H, 16) * 16 - H) ? 5 : 4;
+    // partial-reduction buffers are sized for the finest tiling the
+    // autotuner may pick: 4 rows per wave, 16 kz planes per block
+    P.rpw = 4;
     P.nbx = int(ceil_div(W, 64));
-    P.nby = int(ceil_div(H, 4 * P.rpw));
-    P.nbz = int(ceil_div(D, 48));
-    P.kz_chunk = int(ceil_div(ceil_div(D, P.nbz), 2) * 2);
-    P.nbz = int(ceil_div(D, P.kz_chunk));
+    P.nby = int(ceil_div(H, 16));
+    P.kz_chunk = 16;
+    P.nbz = int(ceil_div(D, 16));
     const long long shell = 6 * (P.plane[0] + P.plane[1] + P.plane[2]);
     const long long tsz = 6 * (P.nbx * P.plane[0] + P.nby * P.plane[1] + P.nbz * P.plane[2]);
     float* blk;
@@ -430,10 +475,7 @@ int finalize_batched(Ctx& c) {
       if ((r = upload_vec(c, (void**)&P.R[ax], R.data(), R.size() * 4))) return r;
     }
     if ((r = c.dalloc((void**)&P.rec_part, size_t(P.nbx) * P.nby * P.nbz * ARD_MAX_REC_PER_PART * 4))) return r;
-    for (int w = 0; w < 3; ++w) {
-      folds.push_back(FoldDesc{P.F[w], P.S[w], P.plane[w]});
-      unfolds.push_back(UnfoldDesc{P.T[w], P.S[w], P.plane[w], w == 0 ? P.nbx : (w == 1 ? P.nby : P.nbz)});
-    }
+    for (int w = 0; w < 3; ++w) folds.push_back(FoldDesc{P.F[w], P.S[w], P.plane[w]});
     const float *fw, *iw, *fh, *ih, *fd, *id;
     if ((r = c.cos_tables(p.w, &fw, &iw)) || (r = c.cos_tables(p.h, &fh, &ih)) || (r = c.cos_tables(p.d, &fd, &id)))
       return r;
@@ -463,13 +505,6 @@ int finalize_batched(Ctx& c) {
     B.fold_total = t;
     if ((r = upload_vec(c, (void**)&B.d_fold, folds.data(), folds.size() * sizeof(FoldDesc)))) return r;
     if ((r = upload_vec(c, (void**)&B.d_fold_start, st.data(), st.size() * 8))) return r;
-    std::vector<long long> su;
-    t = 0;
-    for (auto& u : unfolds) { su.push_back(t); t += 3 * u.plane; }
-    B.nunfold = int(unfolds.size());
-    B.unfold_total = t;
-    if ((r = upload_vec(c, (void**)&B.d_unfold, unfolds.data(), unfolds.size() * sizeof(UnfoldDesc)))) return r;
-    if ((r = upload_vec(c, (void**)&B.d_unfold_start, su.data(), su.size() * 8))) return r;
   }
   // PML slabs: one grouped launch
   {
@@ -520,15 +555,9 @@ int finalize_batched(Ctx& c) {
     }
     B.srcs.push_back(ps);
   }
-  if ((r = build_sources(c)) || (r = build_modal(c))) return r;
-  // recorders
-  std::vector<BRecDesc> rd;
-  for (const RecorderDef& rc : c.topo.recorders) {
-    const BPart& P = B.parts[rc.part];
-    int slot = int(std::find(P.recs.begin(), P.recs.end(), int(&rc - &c.topo.recorders[0])) - P.recs.begin());
-    rd.push_back(BRecDesc{P.rec_part + slot, P.nbx * P.nby * P.nbz});
-  }
-  if (!rd.empty() && (r = upload_vec(c, (void**)&B.d_rec, rd.data(), rd.size() * sizeof(BRecDesc)))) return r;
+  if ((r = build_sources(c))) return r;
+  if ((r = apply_tiling(c, 4, 48))) return r;
+  if (c.prm.autotune && (r = autotune(c))) return r;
   hipError_t e = hipStreamSynchronize(c.stream);
   return e == hipSuccess ? ARD_OK : hip_fail(e, "hipStreamSynchronize");
 }
@@ -547,13 +576,13 @@ int enqueue_step(Ctx& c, int par, PhaseTimer* T) {
   phase(ARD_PH_SOURCES, [&] { return launch_bsources(B.d_src, B.nsrc, c.d_step, c.courant, st); });
   phase(ARD_PH_PML, [&] { return launch_pml(B.d_pml, B.npml, B.pml_blocks, par, c.prm.h, c.prm.dt, c.prm.c0, st); });
   phase(ARD_PH_FOLD, [&] { return launch_fold(B.d_fold, B.d_fold_start, B.nfold, B.fold_total, st); });
-  phase(ARD_PH_FACE_FWD1, [&] { return launch_gemm(B.d_gemm[0], B.ngemm[0], B.gemm_tiles[0], true, true, 0, st); });
-  phase(ARD_PH_FACE_FWD2, [&] { return launch_gemm(B.d_gemm[1], B.ngemm[1], B.gemm_tiles[1], false, false, 0, st); });
+  phase(ARD_PH_FACE_FWD1, [&] { return launch_gemm(B.d_gemm[0], B.ngemm[0], B.gemm_tiles[0], true, true, 0, st, B.gemm_variant); });
+  phase(ARD_PH_FACE_FWD2, [&] { return launch_gemm(B.d_gemm[1], B.ngemm[1], B.gemm_tiles[1], false, false, 0, st, B.gemm_variant); });
   for (auto& g : B.modal)
     phase(ARD_PH_MODAL, [&] { return launch_modal(g.rpw, g.pts, g.d_desc[par], g.d_start, g.n, g.blocks, par, st); });
   phase(ARD_PH_UNFOLD, [&] { return launch_unfold(B.d_unfold, B.d_unfold_start, B.nunfold, B.unfold_total, st); });
-  phase(ARD_PH_FACE_INV1, [&] { return launch_gemm(B.d_gemm[2], B.ngemm[2], B.gemm_tiles[2], true, true, 0, st); });
-  phase(ARD_PH_FACE_INV2, [&] { return launch_gemm(B.d_gemm[3], B.ngemm[3], B.gemm_tiles[3], false, false, 0, st); });
+  phase(ARD_PH_FACE_INV1, [&] { return launch_gemm(B.d_gemm[2], B.ngemm[2], B.gemm_tiles[2], true, true, 0, st, B.gemm_variant); });
+  phase(ARD_PH_FACE_INV2, [&] { return launch_gemm(B.d_gemm[3], B.ngemm[3], B.gemm_tiles[3], false, false, 0, st, B.gemm_variant); });
   // the interface reads the PML pressure just written: buffer par ^ 1
   phase(ARD_PH_INTERFACE, [&] { return launch_interface(B.d_bnd, B.nbnd, B.d_parts, par ^ 1, B.bnd_total, st); });
   phase(ARD_PH_RECORD, [&] {
@@ -585,6 +614,65 @@ int build_graphs(Ctx& c) {
 }
 
 }  // namespace
+
+
+// The autotuner: times the modal pass for each tiling candidate and the face
+// transforms for each kernel variant on this scene, keeps the fastest. Runs at
+// finalize, when the state is all zeros, so the timed launches change nothing.
+int autotune(Ctx& c) {
+  BatchedState& B = *c.bs;
+  hipEvent_t e0, e1;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
+    return set_error(ARD_ERR_HIP, "hipEventCreate");
+  auto time_ms = [&](auto&& launch, int reps) {
+    launch();
+    hipEventRecord(e0, c.stream);
+    for (int i = 0; i < reps; ++i) launch();
+    hipEventRecord(e1, c.stream);
+    hipEventSynchronize(e1);
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, e0, e1);
+    return ms / reps;
+  };
+  int best_rpw = 4, best_kzc = 48, tried = 0;
+  float best = 1e30f;
+  int r = ARD_OK;
+  for (int rpw : {4, 5}) {
+    for (int kzc : {16, 32, 48, 96}) {
+      if ((r = apply_tiling(c, rpw, kzc))) break;
+      const float ms = time_ms([&] {
+        for (auto& g : B.modal) launch_modal(g.rpw, g.pts, g.d_desc[0], g.d_start, g.n, g.blocks, 0, c.stream);
+      }, 3);
+      tried++;
+      if (ms < best) { best = ms; best_rpw = rpw; best_kzc = kzc; }
+    }
+    if (r) break;
+  }
+  float bestg = 1e30f;
+  for (int v : {0, 1}) {
+    const float ms = time_ms([&] {
+      for (int k = 0; k < 4; ++k)
+        launch_gemm(B.d_gemm[k], B.ngemm[k], B.gemm_tiles[k], k % 2 == 0, k % 2 == 0, 0, c.stream, v);
+    }, 3);
+    tried++;
+    if (ms < bestg) { bestg = ms; B.gemm_variant = v; }
+  }
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  if (r) return r;
+  if ((r = apply_tiling(c, best_rpw, best_kzc))) return r;
+  B.tuned = tried;
+  return ARD_OK;
+}
+
+int batched_config(Ctx& c, int* out4) {
+  if (!c.bs) return set_error(ARD_ERR_STATE, "not a finalized batched context");
+  out4[0] = c.bs->rpw;
+  out4[1] = c.bs->kz_chunk;
+  out4[2] = c.bs->gemm_variant;
+  out4[3] = c.bs->tuned;
+  return ARD_OK;
+}
 
 int step_batched(Ctx& c, int n) {
   BatchedState& B = *c.bs;

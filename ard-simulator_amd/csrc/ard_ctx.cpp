@@ -462,6 +462,12 @@ int ard_set_force(ard_ctx* ctx, int part, int x, int y, int z, float f) {
   return set_force_cell(*c, part, x, y, z, f);
 }
 
+int ard_batched_config(ard_ctx* ctx, int* out4) {
+  Ctx* c = reinterpret_cast<Ctx*>(ctx);
+  if (!c || !out4) return fail(ARD_ERR_INVALID, "null argument");
+  return batched_config(*c, out4);
+}
+
 int ard_set_transform_variant(int variant) {
   if (variant < 0 || variant > 1) return fail(ARD_ERR_INVALID, "unknown transform variant");
   g_gemm_variant = variant;

@@ -122,6 +122,7 @@ int step_batched(Ctx& c, int n);
 int batched_pressure_field(Ctx& c, int part, const float** out);
 int batched_set_force(Ctx& c, int part, int x, int y, int z, float f);
 void destroy_batched(Ctx& c);
+int batched_config(Ctx& c, int* out4);
 
 // field access (either form)
 int pressure_field_ptr(Ctx& c, int part, const float** out);

@@ -235,10 +235,11 @@ __global__ __launch_bounds__(256) void k_gemm_mfma(const GemmDesc* __restrict__ 
 int g_gemm_variant = 1;
 
 int launch_gemm(const GemmDesc* d_descs, int ndesc, int total_tiles, bool b_k_contig, bool c_m_contig, int ep,
-                hipStream_t s) {
+                hipStream_t s, int variant) {
   if (total_tiles <= 0) return 0;
   dim3 grid(total_tiles), block(256);
-  if (g_gemm_variant == 1) {
+  if (variant < 0) variant = g_gemm_variant;
+  if (variant == 1) {
     if (b_k_contig && ep == 0) hipLaunchKernelGGL((k_gemm_mfma<true, 0>), grid, block, 0, s, d_descs, ndesc);
     else if (!b_k_contig && ep == 0) hipLaunchKernelGGL((k_gemm_mfma<false, 0>), grid, block, 0, s, d_descs, ndesc);
     else if (!b_k_contig && ep == 1) hipLaunchKernelGGL((k_gemm_mfma<false, 1>), grid, block, 0, s, d_descs, ndesc);

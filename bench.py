@@ -58,8 +58,8 @@ def face_gemm_flops(sim):
     return fl
 
 
-def run_form(sc, form, steps, warmup, timing_steps, direct=False):
-    sim = A.Simulation(sc, form=form)
+def run_form(sc, form, steps, warmup, timing_steps, direct=False, autotune=False):
+    sim = A.Simulation(sc, form=form, autotune=autotune)
     if direct:
         sim.set_kernel_timing(True)
     sim.update(warmup)
@@ -121,6 +121,7 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--transform-variant", type=int, default=1, help="0 vector FMA, 1 matrix cores (FP32)")
+    ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--direct", action="store_true",
                     help="launch each phase directly instead of replaying the step graph (counter collection)")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
@@ -143,7 +144,8 @@ def main():
         del src, dst
     sc = scene_by_name(args.workload)
     sim, secs, phases, trace_b = run_form(sc, A.FORM_BATCHED, args.steps, args.warmup, args.timing_steps,
-                                          args.direct)
+                                          args.direct, autotune=not args.no_autotune)
+    tiling = sim.batched_config()
     n_dct, n_pml, faces = counts(sim)
     gflop_face = face_gemm_flops(sim)
     sim.close()
@@ -186,7 +188,8 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": "%s: batched face-shell form, hipGraph per step" % sc.name,
-                   "transform": ["vector-fma", "mfma-fp32"][args.transform_variant], "dct_cells": n_dct,
+                   "tiling": {"rows_per_wave": tiling[0], "kz_per_block": tiling[1],
+                              "transform": ["vector-fma", "mfma-fp32"][tiling[2]], "autotune_candidates": tiling[3]}, "dct_cells": n_dct,
                    "pml_cells": n_pml, "face_elements": faces, "boxes": len(sc.boxes), "h": sc.h, "dt": sc.dt,
                    "sources": len(sc.sources), "recorders": len(sc.recorders)},
         "roofline": roof,

@@ -63,7 +63,8 @@ typedef struct ard_params {
   float absorption; /* Partition::m_absorption (ARD/partition.h:45) */
   int form;         /* ARD_FORM_* */
   int recorder_mode;/* ARD_REC_* */
-  int reserved[9];  /* zero */
+  int autotune;     /* batched form: time candidate tilings and transform kernels at ard_finalize */
+  int reserved[8];  /* zero */
 } ard_params;
 
 /* Replaces the statics set in ARD/main.cpp:29-45 plus the GPU set-up of
@@ -161,6 +162,10 @@ int ard_selftest_transform(int device, int variant, int b_layout, int M, int N, 
 /* Selects the dense transform kernels used by contexts finalized afterwards
  * (0 vector FMA, 1 matrix cores FP32). Default 1. */
 int ard_set_transform_variant(int variant);
+/* Tiling of the batched form after ard_finalize (chosen by the autotuner when
+ * ard_params.autotune is set): out[4] = rows per wave, kz planes per block,
+ * transform variant, number of candidates timed. No counterpart in the reference. */
+int ard_batched_config(ard_ctx* ctx, int* out4);
 
 #ifdef __cplusplus
 }
