@@ -49,8 +49,8 @@ struct ModalDesc {
   int W, H, D;
   int nbx, nby, nbz, kz_chunk;
   float dt;
-  float gscale;       // .999 * 4 * s_f * s_i (force spectrum to modes)
-  const float* wx2;   // [W] (c0 pi (kx+1) / Lx)^2, and likewise for y, z
+  float gscale;       // .999 * 4 * s_f * s_i * dt^2 / 4 (force spectrum to modes)
+  const float* wx2;   // [W] u_x = (c0 pi (kx+1) / Lx)^2 dt^2 / 4, and likewise for y, z
   const float* wy2;
   const float* wz2;
   float* M[2];
@@ -60,7 +60,9 @@ struct ModalDesc {
   const float* Ry;
   const float* Ez;
   const float* Rz;
-  const float* Gx;  // [6][D][H] folded face spectra (q < 3 even, q >= 3 odd)
+  const float* yt;  // [H][8] {Ey[ky][0..2], Ry[ky][0..2], u_y, 0}
+  const float* zt;  // [D][8] likewise for z
+  const float* Gx;  // [D][H][6] folded face spectra (q < 3 even, q >= 3 odd)
   const float* Gy;  // [6][D][W]
   const float* Gz;  // [6][H][W]
   float* Tx;        // [nbx][6][D][H] partial reductions

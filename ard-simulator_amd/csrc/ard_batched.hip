@@ -65,6 +65,7 @@ struct BPart {
   float gscale = 0.f;
   float* E[3] = {};   // Ex, Ey, Ez
   float* R[3] = {};   // Rx, Ry, Rz
+  float* yz[2] = {};  // y, z: [n][8] = {E0, E1, E2, R0, R1, R2, u, 0}
   float* rec_part = nullptr;
   int rpw = 4, nbx = 0, nby = 0, nbz = 0, kz_chunk = 0;
   int nrec = 0;
@@ -239,6 +240,7 @@ int build_modal(Ctx& c) {
     d.M[0] = P.M[0]; d.M[1] = P.M[1];
     d.Ex = P.E[0]; d.Rx = P.R[0]; d.Ey = P.E[1]; d.Ry = P.R[1]; d.Ez = P.E[2]; d.Rz = P.R[2];
     d.Gx = P.G[0]; d.Gy = P.G[1]; d.Gz = P.G[2];
+    d.yt = P.yz[0]; d.zt = P.yz[1];
     d.Tx = P.T[0]; d.Ty = P.T[1]; d.Tz = P.T[2];
     std::vector<SrcTab> st;
     for (auto& s : B.srcs)
@@ -429,6 +431,8 @@ int finalize_batched(Ctx& c) {
     if (int(P.recs.size()) > ARD_MAX_REC_PER_PART)
       return set_error(ARD_ERR_UNSUPPORTED, "more than 4 recorders in one partition");
     const long long W = p.w, H = p.h, D = p.d, N = W * H * D;
+    if (N >= (1LL << 30))  // 32-bit cell and byte offsets in the modal pass
+      return set_error(ARD_ERR_UNSUPPORTED, "a DCT partition of 2^30 or more cells (split it)");
     P.plane[0] = D * H; P.plane[1] = D * W; P.plane[2] = H * W;
     // partial-reduction buffers are sized for the finest tiling the
     // autotuner may pick: 4 rows per wave, 16 kz planes per block
@@ -450,22 +454,32 @@ int finalize_batched(Ctx& c) {
     P.T[2] = q; q += 6 * P.nbz * P.plane[2];
     P.M[0] = q; q += N;
     P.M[1] = q; q += N;
-    // modal coefficients are formed in the kernel from per-axis squared
-    // wavenumbers (ARD/dct_partition.cpp:24-33, 1-based), tabulated in FP64
+    // modal coefficients are formed in the kernel from per-axis terms
+    // u = (w dt / 2)^2 of the 1-based wavenumbers (ARD/dct_partition.cpp:24-33),
+    // tabulated in FP64; the kernel's series for sin^2 holds for u <= 3, past
+    // the scheme's stability limit (u = pi^2 / 4 at dt = h / (c0 sqrt 3))
+    std::vector<float> uh[3];
     {
       const double pi = 3.14159265358979323846;
+      const double hdt = 0.5 * double(c.prm.dt);
       const int n3[3] = {p.w, p.h, p.d};
+      double umax = 0.0;
       for (int ax = 0; ax < 3; ++ax) {
         std::vector<float> t(static_cast<size_t>(n3[ax]));
         const double L = n3[ax] * double(c.prm.h);
         for (int k = 0; k < n3[ax]; ++k) {
           const double w = double(c.prm.c0) * pi * (k + 1) / L;
-          t[size_t(k)] = float(w * w);
+          t[size_t(k)] = float(w * w * hdt * hdt);
         }
+        umax += double(t.back());
         if ((r = upload_vec(c, (void**)&P.w2[ax], t.data(), t.size() * 4))) return r;
+        uh[ax] = t;
       }
+      if (umax > 3.0)
+        return set_error(ARD_ERR_UNSUPPORTED,
+                         "time step beyond the stability limit dt <= h / (c0 sqrt 3) (batched form; use the plain form)");
       const double Nd = double(N);
-      P.gscale = float(double(0.999f) * 4.0 / (2.0 * std::sqrt(2.0 * Nd)) / std::sqrt(2.0 * Nd));
+      P.gscale = float(double(0.999f) * 4.0 / (2.0 * std::sqrt(2.0 * Nd)) / std::sqrt(2.0 * Nd) * hdt * hdt);
     }
     const int dims[3] = {p.w, p.h, p.d};
     for (int ax = 0; ax < 3; ++ax) {
@@ -473,6 +487,18 @@ int finalize_batched(Ctx& c) {
       axis_tables(dims[ax], E, R);
       if ((r = upload_vec(c, (void**)&P.E[ax], E.data(), E.size() * 4))) return r;
       if ((r = upload_vec(c, (void**)&P.R[ax], R.data(), R.size() * 4))) return r;
+      if (ax == 0) continue;
+      // y and z: per wavenumber {E0, E1, E2, R0, R1, R2, u, 0}, one scalar load
+      const std::vector<float>& u = uh[ax];
+      std::vector<float> t(size_t(dims[ax]) * 8, 0.f);
+      for (int k = 0; k < dims[ax]; ++k) {
+        for (int j = 0; j < 3; ++j) {
+          t[size_t(k) * 8 + j] = E[size_t(k) * 3 + j];
+          t[size_t(k) * 8 + 3 + j] = R[size_t(k) * 3 + j];
+        }
+        t[size_t(k) * 8 + 6] = u[size_t(k)];
+      }
+      if ((r = upload_vec(c, (void**)&P.yz[ax - 1], t.data(), t.size() * 4))) return r;
     }
     if ((r = c.dalloc((void**)&P.rec_part, size_t(P.nbx) * P.nby * P.nbz * ARD_MAX_REC_PER_PART * 4))) return r;
     for (int w = 0; w < 3; ++w) folds.push_back(FoldDesc{P.F[w], P.S[w], P.plane[w]});
@@ -483,7 +509,14 @@ int finalize_batched(Ctx& c) {
     gm[0].push_back(gemm_k_contig(fh, p.h, P.S[0], P.U[0], int(6 * D), H));
     gm[0].push_back(gemm_k_contig(fw, p.w, P.S[1], P.U[1], int(6 * D), W));
     gm[0].push_back(gemm_k_contig(fw, p.w, P.S[2], P.U[2], int(6 * H), W));
-    gm[1].push_back(gemm_n_contig(fd, p.d, P.U[0], P.G[0], p.h));
+    {
+      // x-face spectra are stored [D][H][6] (the six folded slabs of one
+      // (kz, ky) adjacent), so the modal pass reads a row group with one
+      // scalar load
+      GemmDesc g = gemm_n_contig(fd, p.d, P.U[0], P.G[0], p.h);
+      g.cM = 6 * H; g.c1 = 6; g.c2 = 1;
+      gm[1].push_back(g);
+    }
     gm[1].push_back(gemm_n_contig(fd, p.d, P.U[1], P.G[1], p.w));
     gm[1].push_back(gemm_n_contig(fh, p.h, P.U[2], P.G[2], p.w));
     gm[2].push_back(gemm_k_contig(ih, p.h, P.S[0], P.U[0], int(6 * D), H));
@@ -637,7 +670,7 @@ int autotune(Ctx& c) {
   int best_rpw = 4, best_kzc = 48, tried = 0;
   float best = 1e30f;
   int r = ARD_OK;
-  for (int rpw : {4, 5}) {
+  for (int rpw : {4, 6}) {
     for (int kzc : {16, 32, 48, 96}) {
       if ((r = apply_tiling(c, rpw, kzc))) break;
       const float ms = time_ms([&] {

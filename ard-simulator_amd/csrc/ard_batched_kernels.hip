@@ -106,7 +106,7 @@ __global__ void k_bsources(const BSourceDesc* __restrict__ s, int n, const int* 
 
 // ---------------------------------------------------------------------------
 // The modal pass. Block = 4 waves; lane <-> kx (64 per block), wave w owns
-// rows ky = ky0 + w + 4 r (r < RPW), the block walks kz over its chunk.
+// rows ky = kyb + w RPW + r (r < RPW), the block walks kz over its chunk.
 // Per mode:   Fh = sum_i Ex[kx][i] Gx[par kx][i][kz][ky] + (y, z alike) + sources
 //             M+ = a M - b M- + g Fh           (stored over M-)
 // and reduces M+ onto the face planes and recorders:
@@ -115,40 +115,58 @@ __global__ void k_bsources(const BSourceDesc* __restrict__ s, int n, const int* 
 //   Tz[par kz][l][ky][kx] += Rz[kz][l] M+   (along the kz walk)
 // Partial sums go to per-chunk slots, summed by k_unfold.
 // ---------------------------------------------------------------------------
-// sum over the 8 lanes of one parity in each 16-lane row (DPP, no LDS):
-// xor 2 inside the quad, then rotations by 4 and 8 inside the row
-__device__ __forceinline__ float row_parity_sum(float v) {
-  v += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x4E, 0xF, 0xF, false));  // quad_perm [2,3,0,1]
-  v += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x124, 0xF, 0xF, false)); // row_ror:4
-  v += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x128, 0xF, 0xF, false)); // row_ror:8
-  return v;
+// sin^2(x) / x^2 as a polynomial in u = x^2, for 0 <= u <= 3 (the stability
+// limit of the scheme puts x = w dt / 2 at most at pi / 2, u <= 2.47): the
+// series of (1 - cos 2x) / (2 x^2), c_n = (-1)^(n+1) 2^(2n-1) / (2n)!, to
+// n = 10; truncation error < 3e-11, FP32 Horner error < 4e-7 relative.
+__device__ __forceinline__ float sinc2_poly(float u) {
+  float p = -2.1549902e-13f;
+  p = fmaf(p, u, 2.0472407e-11f);
+  p = fmaf(p, u, -1.5661391e-9f);
+  p = fmaf(p, u, 9.3968346e-8f);
+  p = fmaf(p, u, -4.2755596e-6f);
+  p = fmaf(p, u, 1.4109348e-4f);
+  p = fmaf(p, u, -3.1746032e-3f);
+  p = fmaf(p, u, 4.4444446e-2f);
+  p = fmaf(p, u, -3.3333334e-1f);
+  return fmaf(p, u, 1.0f);
 }
 
-// sin(x) for 0 <= x <= 1.3 (half the largest mode phase c dt pi sqrt(3) / h at
-// Courant 0.43): odd Taylor series to x^11, truncation error < x^13/13! < 6e-9.
-__device__ __forceinline__ float sin_small(float x) {
-  const float x2 = x * x;
-  float p = -2.5052108e-8f;           // -1/11!
-  p = fmaf(p, x2, 2.7557319e-6f);     //  1/9!
-  p = fmaf(p, x2, -1.9841270e-4f);    // -1/7!
-  p = fmaf(p, x2, 8.3333333e-3f);     //  1/5!
-  p = fmaf(p, x2, -1.6666667e-1f);    // -1/3!
-  p = fmaf(p, x2, 1.0f);
-  return x * p;
+typedef __amdgpu_buffer_rsrc_t brsrc;
+// raw buffer resource over n bytes (gfx9 word 3: 32-bit float data, no swizzle)
+__device__ __forceinline__ brsrc make_rsrc(const void* p, unsigned nbytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), 0, int(nbytes), 0x00020000);
+}
+__device__ __forceinline__ float bload(brsrc r, unsigned voff, unsigned soff) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, int(voff), int(soff), 0));
 }
 
+// Modal pass of the batched form, one block per (64 kx) x (4 RPW ky) x (kz
+// chunk) tile of one partition. For every mode:
+//   force spectrum   fh = sum of the face-slab expansions (x, y, z faces) + point forces
+//   modal update     M+ = a M - .999 M- + g fh         (ARD/dct_partition.cpp:66-70)
+//   reductions       Tx, Ty, Tz: M+ projected onto the face slabs (x, y, z), and recorders
+// Wave w owns rows ky0 = kyb + w RPW .. ky0 + RPW - 1 (parity r & 1 known at
+// compile time) and walks the chunk's kz planes in pairs (parity known), with
+// the next plane's loads in flight during the current one. z sums stay in
+// registers. Per batch of KB planes, each wave leaves its y sums and the new
+// modes in LDS; after a barrier the block forms the batch's x sums from the
+// mode tile and adds the four waves' y sums, then a second barrier frees LDS.
+// (No LDS atomics: float atomics on LDS serialise.)
 template <int RPW, bool PTS>
 __global__ __launch_bounds__(256) void k_modal(const ModalDesc* __restrict__ descs,
                                                const long long* __restrict__ starts, int ndesc, int par) {
+  static_assert(RPW % 2 == 0, "rows per wave must be even");
   constexpr int TY = 4 * RPW;  // rows of the block tile
-  constexpr int KB = 8;        // kz planes accumulated in LDS between flushes
-  __shared__ float s_ty[KB][2][3][64];    // [plane][ky parity][j][lane]
-  __shared__ float s_tx[KB][TY][2][3];    // [plane][row][kx parity][i]
+  constexpr int KB = 4;        // kz planes per batch of the x and y reductions
+  constexpr int LD = 68;       // padded row of the mode tile
+  __shared__ __attribute__((aligned(16))) float s_m[KB][TY][LD];     // new modes of a batch
+  __shared__ __attribute__((aligned(16))) float s_ty[KB][6][4][64];  // y sums: [plane][ky parity * 3 + j][wave][lane]
+  __shared__ __attribute__((aligned(16))) float s_rx[64 * 3];        // x reduction weights of the tile's kx
   __shared__ float s_rec[4][ARD_MAX_REC_PER_PART];
-  __shared__ float s_gz[RPW][6][256];     // thread-private: z-face force spectrum of this thread's rows
   const int bid = blockIdx.x;
   const int di = find_desc_ll(starts, ndesc, bid);
-  const ModalDesc& P = descs[di];  // fields used in the loops are copied to locals below
+  const ModalDesc& P = descs[di];
   const int lb = bid - int(starts[di]);
   const int bx = lb % P.nbx;
   const int by = (lb / P.nbx) % P.nby;
@@ -156,195 +174,256 @@ __global__ __launch_bounds__(256) void k_modal(const ModalDesc* __restrict__ des
   const int W = P.W, H = P.H, D = P.D;
   const int tid = threadIdx.x;
   const int lane = tid & 63;
-  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int kx = bx * 64 + lane;
   const bool xok = kx < W;
   const int kxc = xok ? kx : W - 1;
   const int px = kx & 1;
-  const int kyb = by * TY;           // first row of the block
-  const int ky0 = kyb + wave;        // rows of this wave: ky0 + 4 r
-  const int py = ky0 & 1;            // shared by every row of this wave
+  const int kyb = by * TY;
+  const int ky0 = kyb + wave * RPW;  // even
   const int kz0 = bz * P.kz_chunk;   // kz_chunk is even, so kz0 is even
   const int kz1 = min(D, kz0 + P.kz_chunk);
-  const long long HW = (long long)H * W;
-  const long long DH = (long long)D * H, DW = (long long)D * W;
-  const gfloat* __restrict__ Gx = ARD_CG(P.Gx);
-  const gfloat* __restrict__ Gy = ARD_CG(P.Gy);
-  const gfloat* __restrict__ Ez = ARD_CG(P.Ez);
-  const gfloat* __restrict__ Rz = ARD_CG(P.Rz);
-  const gfloat* __restrict__ Ey = ARD_CG(P.Ey);
-  const gfloat* __restrict__ Ry = ARD_CG(P.Ry);
-  const gfloat* __restrict__ Wz2 = ARD_CG(P.wz2);
-  const gfloat* __restrict__ Wy2 = ARD_CG(P.wy2);
-  const gfloat* __restrict__ Mc = ARD_CG(P.M[par]);
+  const int HW = H * W, DH = D * H, DW = D * W;  // partitions hold < 2^30 cells
+  const cfloat* __restrict__ Gxw = ARD_CC(P.Gx) + ky0 * 6;  // [D][H][6]: this wave's rows of a plane adjacent
+  const cfloat* __restrict__ Zt = ARD_CC(P.zt);
+  const brsrc rMc = make_rsrc(P.M[par], unsigned(D * HW) * 4u);
+  const brsrc rMp = make_rsrc(P.M[par ^ 1], unsigned(D * HW) * 4u);
+  const brsrc rGy = make_rsrc(P.Gy, unsigned(6 * DW) * 4u);
   gfloat* __restrict__ Mp = ARD_G(P.M[par ^ 1]);
-  const float hdt = 0.5f * P.dt;
-  const float gsc = P.gscale;  // .999 * 4 * s_f * s_i
+  gfloat* __restrict__ Txp = ARD_G(P.Tx) + (long long)bx * 6 * DH;
+  gfloat* __restrict__ Typ = ARD_G(P.Ty) + (long long)by * 6 * DW;
+  const float gsc = P.gscale;  // .999 * 4 * s_f * s_i * dt^2 / 4
+  const unsigned vx = unsigned(kxc) * 4u;
 
-  float ex[3], rx[3];
+  // x factors, split by parity so that the folded spectra need no select
+  float exs[3], exd[3];
 #pragma unroll
   for (int i = 0; i < 3; ++i) {
-    ex[i] = ARD_CG(P.Ex)[kxc * 3 + i];
-    rx[i] = xok ? ARD_CG(P.Rx)[kxc * 3 + i] : 0.f;
+    const float e = ARD_CG(P.Ex)[kxc * 3 + i];
+    exs[i] = px ? 0.f : e;
+    exd[i] = px ? e : 0.f;
   }
-  const float wx2 = ARD_CG(P.wx2)[kxc];
-  float tz[RPW][6];
+  if (wave == 0) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i) s_rx[lane * 3 + i] = xok ? ARD_CG(P.Rx)[kxc * 3 + i] : 0.f;
+  }
+  const float ux = ARD_CG(P.wx2)[kxc];
+  // per-row factors of this wave's rows (wave-uniform)
+  float ey[RPW][3], ry[RPW][3], uy[RPW];
+  bool rok[RPW];
+  int rof[RPW];  // row offset ky * W (clamped)
 #pragma unroll
   for (int r = 0; r < RPW; ++r) {
-    const int kyc = min(ky0 + 4 * r, H - 1);
+    const int kyc = min(ky0 + r, H - 1);
+    rok[r] = ky0 + r < H;
+    rof[r] = kyc * W;
+    const cfloat* yt = ARD_CC(P.yt) + kyc * 8;
+    uy[r] = yt[6];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      ey[r][j] = yt[j];
+      ry[r][j] = yt[3 + j];
+    }
+  }
+  // z-face force spectrum (both kz parities) and z sums of this thread's rows
+  float gz[RPW][6], tz[RPW][6];
+#pragma unroll
+  for (int r = 0; r < RPW; ++r) {
 #pragma unroll
     for (int q = 0; q < 6; ++q) {
-      s_gz[r][q][tid] = ARD_CG(P.Gz)[q * HW + (long long)kyc * W + kxc];
+      gz[r][q] = ARD_CG(P.Gz)[q * HW + rof[r] + kxc];
       tz[r][q] = 0.f;
     }
   }
   const int nsrc = PTS ? min(P.nsrc, ARD_MAX_SRC_PER_PART) : 0;
   const int nrec = PTS ? P.nrec : 0;
   float sfx[ARD_MAX_SRC_PER_PART], rfx[ARD_MAX_REC_PER_PART], rec_acc[ARD_MAX_REC_PER_PART];
+  float sfy[PTS ? RPW : 1][ARD_MAX_SRC_PER_PART], rfy[PTS ? RPW : 1][ARD_MAX_REC_PER_PART];
 #pragma unroll
-  for (int q = 0; q < ARD_MAX_SRC_PER_PART; ++q) sfx[q] = PTS && q < nsrc ? P.src[q].fx[kxc] * *P.src[q].val : 0.f;
+  for (int q = 0; q < ARD_MAX_SRC_PER_PART; ++q) {
+    sfx[q] = PTS && q < nsrc ? P.src[q].fx[kxc] * *P.src[q].val : 0.f;
+#pragma unroll
+    for (int r = 0; r < (PTS ? RPW : 1); ++r)
+      sfy[r][q] = PTS && q < nsrc ? P.src[q].fy[min(ky0 + r, H - 1)] : 0.f;
+  }
 #pragma unroll
   for (int q = 0; q < ARD_MAX_REC_PER_PART; ++q) {
     rfx[q] = PTS && q < nrec && xok ? P.rec[q].fx[kxc] : 0.f;
     rec_acc[q] = 0.f;
-  }
-  for (int e = tid; e < KB * 2 * 3 * 64; e += 256) (&s_ty[0][0][0][0])[e] = 0.f;
-  for (int e = tid; e < KB * TY * 6; e += 256) (&s_tx[0][0][0][0])[e] = 0.f;
-  __syncthreads();
-
-  auto body = [&](const int kz, const int kl, auto pzc) {
-    constexpr int PZ = decltype(pzc)::value;
-    const float ez0 = Ez[kz * 3 + 0], ez1 = Ez[kz * 3 + 1], ez2 = Ez[kz * 3 + 2];
-    const float rz0 = Rz[kz * 3 + 0], rz1 = Rz[kz * 3 + 1], rz2 = Rz[kz * 3 + 2];
-    const float wxz2 = wx2 + Wz2[kz];
-    float gy[3];
 #pragma unroll
-    for (int j = 0; j < 3; ++j) gy[j] = Gy[(py * 3 + j) * DW + (long long)kz * W + kxc];
+    for (int r = 0; r < (PTS ? RPW : 1); ++r)
+      rfy[r][q] = PTS && q < nrec ? P.rec[q].fy[min(ky0 + r, H - 1)] : 0.f;
+  }
+
+  // streamed operands of one plane: modes of the rows (clamped rows) and both
+  // y-parity spectra
+  struct PlaneIn {
+    float mc[RPW], mp[RPW], gy[6];
+  };
+  auto load = [&](PlaneIn& in, const int kz) {
+    const unsigned po = unsigned(kz * HW);
+#pragma unroll
+    for (int r = 0; r < RPW; ++r) {
+      in.mc[r] = bload(rMc, vx, (po + unsigned(rof[r])) * 4u);
+      in.mp[r] = bload(rMp, vx, (po + unsigned(rof[r])) * 4u);
+    }
+#pragma unroll
+    for (int q = 0; q < 6; ++q) in.gy[q] = bload(rGy, vx, unsigned(q * DW + kz * W) * 4u);
+  };
+
+  // one kz plane of this wave's rows
+  auto plane = [&](const PlaneIn& in, const int kz, const int kl, auto pzc) {
+    constexpr int PZ = decltype(pzc)::value;
+    const cfloat* zt = Zt + kz * 8;
+    const float ez0 = zt[0], ez1 = zt[1], ez2 = zt[2];
+    const float rz0 = zt[3], rz1 = zt[4], rz2 = zt[5];
+    const float uxz = ux + zt[6];
+    const unsigned po = unsigned(kz * HW);
+    // folded x spectra of the rows: Gx[kz][ky0 + r][q], 6 RPW adjacent floats
+    // (rows past H read the next plane's values, inside the allocation; unused)
+    const cfloat* gxp = Gxw + kz * H * 6;
+    float gxr[6][RPW];
+#pragma unroll
+    for (int q = 0; q < 6; ++q)
+#pragma unroll
+      for (int r = 0; r < RPW; ++r) gxr[q][r] = gxp[r * 6 + q];
     float sz[ARD_MAX_SRC_PER_PART], rzp[ARD_MAX_REC_PER_PART];
 #pragma unroll
     for (int q = 0; q < ARD_MAX_SRC_PER_PART; ++q) sz[q] = PTS && q < nsrc ? P.src[q].fz[kz] : 0.f;
 #pragma unroll
     for (int q = 0; q < ARD_MAX_REC_PER_PART; ++q) rzp[q] = PTS && q < nrec ? P.rec[q].fz[kz] : 0.f;
-    float ty[3] = {0.f, 0.f, 0.f};
-    // loads of all rows first, at clamped (always valid) addresses, so that
-    // they are in flight together; rows and lanes outside the partition are
-    // masked afterwards
-    float mcv[RPW], mpv[RPW], gxs[RPW][3], gxd[RPW][3], eyv[RPW][3], ryv[RPW][3], wyv[RPW];
-    long long ov[RPW];
+    float ty[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
 #pragma unroll
     for (int r = 0; r < RPW; ++r) {
-      const int kyc = min(ky0 + 4 * r, H - 1);
-      const long long gxo = (long long)kz * H + kyc;
-      ov[r] = (long long)kz * HW + (long long)kyc * W + kxc;
-      mcv[r] = Mc[ov[r]];
-      mpv[r] = Mp[ov[r]];
-#pragma unroll
-      for (int i = 0; i < 3; ++i) {
-        gxs[r][i] = Gx[i * DH + gxo];
-        gxd[r][i] = Gx[(3 + i) * DH + gxo];
-        eyv[r][i] = Ey[kyc * 3 + i];
-        ryv[r][i] = Ry[kyc * 3 + i];
-      }
-      wyv[r] = Wy2[kyc];
-    }
-#pragma unroll
-    for (int r = 0; r < RPW; ++r) {
-      const int ky = ky0 + 4 * r;
-      const bool rok = ky < H;  // uniform across the wave
-      const bool ok = rok && xok;
-      // modal coefficients (ARD/dct_partition.cpp:24-33,68) from the separable
-      // w^2 = wx2 + wy2 + wz2 (1-based wavenumbers):
-      //   a = .999 * 2 cos(w dt) = .999 * 2 (1 - 2 s^2),  g = .999 * 4 s^2 / w^2,  s = sin(w dt / 2)
-      const float w2 = wxz2 + wyv[r];
-      const float sn = sin_small(hdt * __builtin_amdgcn_sqrtf(w2));
-      const float s2 = sn * sn;
-      const float a = fmaf(-3.996f, s2, 1.998f);
-      const float g = gsc * s2 * __builtin_amdgcn_rcpf(w2);
-      float fh = 0.f;
-#pragma unroll
-      for (int i = 0; i < 3; ++i) fh = fmaf(ex[i], px ? gxd[r][i] : gxs[r][i], fh);
-      fh = fmaf(eyv[r][0], gy[0], fh);
-      fh = fmaf(eyv[r][1], gy[1], fh);
-      fh = fmaf(eyv[r][2], gy[2], fh);
-      fh = fmaf(ez0, s_gz[r][PZ * 3 + 0][tid], fh);
-      fh = fmaf(ez1, s_gz[r][PZ * 3 + 1][tid], fh);
-      fh = fmaf(ez2, s_gz[r][PZ * 3 + 2][tid], fh);
+      const int PY = r & 1;
+      // modal coefficients (ARD/dct_partition.cpp:24-33,68) from u = (w dt / 2)^2,
+      // separable in the 1-based wavenumbers:
+      //   s^2 = sin^2(w dt / 2) = u S(u),  a = .999 * 2 cos(w dt) = 1.998 - 3.996 s^2,
+      //   g = .999 * 4 s^2 / w^2 * s_f s_i = gscale * S(u)
+      const float u = uxz + uy[r];
+      const float S = sinc2_poly(u);
+      const float a = fmaf(-3.996f, u * S, 1.998f);
+      const float g = gsc * S;
+      float fh = exs[0] * gxr[0][r];
+      fh = fmaf(exs[1], gxr[1][r], fh);
+      fh = fmaf(exs[2], gxr[2][r], fh);
+      fh = fmaf(exd[0], gxr[3][r], fh);
+      fh = fmaf(exd[1], gxr[4][r], fh);
+      fh = fmaf(exd[2], gxr[5][r], fh);
+      fh = fmaf(ey[r][0], in.gy[PY * 3 + 0], fh);
+      fh = fmaf(ey[r][1], in.gy[PY * 3 + 1], fh);
+      fh = fmaf(ey[r][2], in.gy[PY * 3 + 2], fh);
+      fh = fmaf(ez0, gz[r][PZ * 3 + 0], fh);
+      fh = fmaf(ez1, gz[r][PZ * 3 + 1], fh);
+      fh = fmaf(ez2, gz[r][PZ * 3 + 2], fh);
       if (PTS) {
-        const int kyc = min(ky, H - 1);
 #pragma unroll
         for (int q = 0; q < ARD_MAX_SRC_PER_PART; ++q)
-          if (q < nsrc) fh = fmaf(sfx[q], P.src[q].fy[kyc] * sz[q], fh);
+          if (q < nsrc) fh = fmaf(sfx[q], sfy[r][q] * sz[q], fh);
       }
-      const float mn = ok ? (a * mcv[r] - 0.999f * mpv[r] + g * fh) : 0.f;
-      if (ok) Mp[ov[r]] = mn;
+      const float mv = fmaf(a, in.mc[r], fmaf(-0.999f, in.mp[r], g * fh));
+      const bool ok = rok[r] && xok;
+      if (ok) Mp[po + unsigned(rof[r]) + unsigned(kx)] = mv;
+      const float mn = ok ? mv : 0.f;
+      s_m[kl][wave * RPW + r][lane] = mn;
       tz[r][PZ * 3 + 0] = fmaf(rz0, mn, tz[r][PZ * 3 + 0]);
       tz[r][PZ * 3 + 1] = fmaf(rz1, mn, tz[r][PZ * 3 + 1]);
       tz[r][PZ * 3 + 2] = fmaf(rz2, mn, tz[r][PZ * 3 + 2]);
-      ty[0] = fmaf(ryv[r][0], mn, ty[0]);
-      ty[1] = fmaf(ryv[r][1], mn, ty[1]);
-      ty[2] = fmaf(ryv[r][2], mn, ty[2]);
+      ty[PY * 3 + 0] = fmaf(ry[r][0], mn, ty[PY * 3 + 0]);
+      ty[PY * 3 + 1] = fmaf(ry[r][1], mn, ty[PY * 3 + 1]);
+      ty[PY * 3 + 2] = fmaf(ry[r][2], mn, ty[PY * 3 + 2]);
       if (PTS) {
-        const int kyc = min(ky, H - 1);
 #pragma unroll
         for (int q = 0; q < ARD_MAX_REC_PER_PART; ++q)
-          if (q < nrec) rec_acc[q] = fmaf(rfx[q] * P.rec[q].fy[kyc] * rzp[q], mn, rec_acc[q]);
-      }
-      // x reduction: parity sums inside each 16-lane row, then the four rows
-      // meet in LDS
-      const float v0 = row_parity_sum(rx[0] * mn);
-      const float v1 = row_parity_sum(rx[1] * mn);
-      const float v2 = row_parity_sum(rx[2] * mn);
-      if (rok && (lane & 14) == 0) {
-        float* t = s_tx[kl][wave + 4 * r][lane & 1];
-        atomicAdd(t + 0, v0);
-        atomicAdd(t + 1, v1);
-        atomicAdd(t + 2, v2);
+          if (q < nrec) rec_acc[q] = fmaf(rfx[q] * rfy[r][q] * rzp[q], mn, rec_acc[q]);
       }
     }
-    // y reduction: waves 0,2 (even ky) and 1,3 (odd ky) meet in LDS
 #pragma unroll
-    for (int j = 0; j < 3; ++j) atomicAdd(&s_ty[kl][py][j][lane], ty[j]);
+    for (int q = 0; q < 6; ++q) s_ty[kl][q][wave][lane] = ty[q];
   };
-  // write the LDS accumulators of planes kb .. kb+n-1 to the partial
-  // reductions and clear them
-  auto flush = [&](const int kb, const int n) {
-    __syncthreads();
-    gfloat* __restrict__ Txp = ARD_G(P.Tx) + (long long)bx * 6 * DH;
-    gfloat* __restrict__ Typ = ARD_G(P.Ty) + (long long)by * 6 * DW;
-    for (int e = tid; e < n * TY * 6; e += 256) {
-      const int q = e % 6, row = (e / 6) % TY, kl = e / (6 * TY);  // q = parity * 3 + i
+
+  // x and y sums of a finished batch (planes kb .. kb+n-1):
+  // x: KB * TY (plane, row) slots, four threads per slot; thread qtr of a slot
+  // takes kx = 16 j + 4 qtr + t (j, t < 4) and the four meet through quad
+  // permutes. The threads of a slot share its trip count, so every quad is whole.
+  // y: the four waves' sums of each (plane, j, kx), four kx per thread.
+  auto reduce = [&](const int kb, const int n) {
+    const int qtr = tid & 3;
+    for (int c = tid >> 2; c < KB * TY; c += 64) {
+      const int kl = c / TY, row = c % TY;
+      float ev0 = 0.f, ev1 = 0.f, ev2 = 0.f, od0 = 0.f, od1 = 0.f, od2 = 0.f;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int k0 = 16 * j + 4 * qtr;  // even
+        const float4 m = *reinterpret_cast<const float4*>(&s_m[kl][row][k0]);
+        const float4 ra = *reinterpret_cast<const float4*>(&s_rx[k0 * 3]);      // kx k0: x y z, k0+1: x
+        const float4 rb = *reinterpret_cast<const float4*>(&s_rx[k0 * 3 + 4]);  // k0+1: y z, k0+2: x y
+        const float4 rc = *reinterpret_cast<const float4*>(&s_rx[k0 * 3 + 8]);  // k0+2: z, k0+3: x y z
+        ev0 = fmaf(ra.x, m.x, ev0); ev1 = fmaf(ra.y, m.x, ev1); ev2 = fmaf(ra.z, m.x, ev2);
+        od0 = fmaf(ra.w, m.y, od0); od1 = fmaf(rb.x, m.y, od1); od2 = fmaf(rb.y, m.y, od2);
+        ev0 = fmaf(rb.z, m.z, ev0); ev1 = fmaf(rb.w, m.z, ev1); ev2 = fmaf(rc.x, m.z, ev2);
+        od0 = fmaf(rc.y, m.w, od0); od1 = fmaf(rc.z, m.w, od1); od2 = fmaf(rc.w, m.w, od2);
+      }
+#define ARD_QUAD_SUM(v)                                                                    \
+  v += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0xB1, 0xF, 0xF, false)); \
+  v += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x4E, 0xF, 0xF, false));
+      ARD_QUAD_SUM(ev0) ARD_QUAD_SUM(ev1) ARD_QUAD_SUM(ev2) ARD_QUAD_SUM(od0) ARD_QUAD_SUM(od1) ARD_QUAD_SUM(od2)
+#undef ARD_QUAD_SUM
       const int ky = kyb + row;
-      float* src = &s_tx[kl][row][q / 3][q % 3];
-      if (ky < H) Txp[q * DH + (long long)(kb + kl) * H + ky] = *src;
-      *src = 0.f;
+      if (kl < n && ky < H && qtr < 2) {
+        const int o = (kb + kl) * H + ky;
+        if (qtr == 0) {
+          Txp[0 * DH + o] = ev0; Txp[1 * DH + o] = ev1; Txp[2 * DH + o] = ev2;
+        } else {
+          Txp[3 * DH + o] = od0; Txp[4 * DH + o] = od1; Txp[5 * DH + o] = od2;
+        }
+      }
     }
-    for (int e = tid; e < n * 6 * 64; e += 256) {
-      const int l = e & 63, q = (e >> 6) % 6, kl = e / 384;
-      const int kxx = bx * 64 + l;
-      float* src = &s_ty[kl][q / 3][q % 3][l];
-      if (kxx < W) Typ[q * DW + (long long)(kb + kl) * W + kxx] = *src;
-      *src = 0.f;
+    for (int e = tid; e < KB * 6 * 16; e += 256) {
+      const int x4 = e & 15, q = (e >> 4) % 6, pl = e / 96;
+      if (pl >= n) continue;
+      float4 v = *reinterpret_cast<const float4*>(&s_ty[pl][q][0][x4 * 4]);
+#pragma unroll
+      for (int w = 1; w < 4; ++w) {
+        const float4 t = *reinterpret_cast<const float4*>(&s_ty[pl][q][w][x4 * 4]);
+        v.x += t.x; v.y += t.y; v.z += t.z; v.w += t.w;
+      }
+      const int kxx = bx * 64 + x4 * 4;
+      gfloat* dst = Typ + q * DW + (kb + pl) * W + kxx;
+      if (kxx + 0 < W) dst[0] = v.x;
+      if (kxx + 1 < W) dst[1] = v.y;
+      if (kxx + 2 < W) dst[2] = v.z;
+      if (kxx + 3 < W) dst[3] = v.w;
     }
-    __syncthreads();
   };
-  int kb = kz0;
-  for (int kz = kz0; kz < kz1; kz += 2) {
-    body(kz, kz - kb, std::integral_constant<int, 0>());
-    if (kz + 1 < kz1) body(kz + 1, kz + 1 - kb, std::integral_constant<int, 1>());
-    const int next = kz + 2;
-    if (next - kb >= KB || next >= kz1) {
-      flush(kb, min(next, kz1) - kb);
-      kb = next;
+
+  PlaneIn cur, nxt;
+  load(cur, kz0);
+  __syncthreads();  // s_rx
+  for (int kb = kz0; kb < kz1; kb += KB) {
+    const int n = min(KB, kz1 - kb);
+    for (int kl = 0; kl < n; kl += 2) {
+      const int kz = kb + kl;
+      // the next plane's loads go out before this plane's arithmetic
+      // (past the chunk's end they re-read its last plane)
+      load(nxt, min(kz + 1, kz1 - 1));
+      plane(cur, kz, kl, std::integral_constant<int, 0>());
+      if (kl + 1 < n) {
+        load(cur, min(kz + 2, kz1 - 1));
+        plane(nxt, kz + 1, kl + 1, std::integral_constant<int, 1>());
+      }
     }
+    __syncthreads();  // batch in LDS
+    reduce(kb, n);
+    __syncthreads();  // LDS free for the next batch
   }
-  // z reduction partials of this kz chunk
+  // z sums of this kz chunk
   gfloat* tzp = ARD_G(P.Tz) + (long long)bz * 6 * HW;
 #pragma unroll
   for (int r = 0; r < RPW; ++r) {
-    const int ky = ky0 + 4 * r;
-    if (ky >= H || !xok) continue;
+    if (!rok[r] || !xok) continue;
 #pragma unroll
-    for (int q = 0; q < 6; ++q) tzp[q * HW + (long long)ky * W + kx] = tz[r][q];
+    for (int q = 0; q < 6; ++q) tzp[q * HW + rof[r] + kx] = tz[r][q];
   }
   // recorder partials of this block
   if (PTS && nrec > 0) {
@@ -414,7 +493,7 @@ int launch_modal(int rpw, bool pts, const ModalDesc* d, const long long* starts,
     break;
   switch (rpw) {
     ARD_MODAL_CASE(4)
-    ARD_MODAL_CASE(5)
+    ARD_MODAL_CASE(6)
     ARD_MODAL_CASE(8)
     default:
       return -1;

@@ -13,6 +13,9 @@ namespace ard {
 typedef __attribute__((address_space(1))) float gfloat;
 #define ARD_G(p) ((gfloat*)(p))
 #define ARD_CG(p) ((const gfloat*)(p))
+// read-only tables: constant address space (uniform reads become scalar loads)
+typedef __attribute__((address_space(4))) const float cfloat;
+#define ARD_CC(p) ((cfloat*)(p))
 
 // One problem of a grouped, strided GEMM  C = alpha * A * B  (A: M x K row-major).
 // B(k, n) = B[k*bK + (n % N1)*b1 + (n / N1)*b2], C(m, n) likewise with cM, c1, c2.
