@@ -68,6 +68,8 @@ struct ModalDesc {
   float* Tx;        // [nbx][6][D][H] partial reductions
   float* Ty;        // [nby][6][D][W]
   float* Tz;        // [nbz][6][H][W]
+  const float* pty;  // [H][8] y factors of the point forces (4) and recorders (4); zero past the counts
+  const float* ptz;  // [D][8] likewise along z
   int nsrc;
   const SrcTab* src;
   int nrec;

@@ -271,7 +271,36 @@ int build_modal(Ctx& c) {
       d.rec = dr;
     }
     d.rec_part = P.rec_part;
-    groups[std::make_pair(P.rpw, (d.nsrc + d.nrec) > 0 ? 1 : 0)].push_back(d);
+    if (d.nsrc + d.nrec > 0) {
+      // y and z factors of the points, 8 per wavenumber (sources, then recorders)
+      std::vector<float> ty(size_t(p.h) * 8, 0.f), tz(size_t(p.d) * 8, 0.f), col(size_t(std::max(p.h, p.d)));
+      int q = 0;
+      for (auto& s : B.srcs) {
+        if (s.part != i || s.in_shell) continue;
+        point_table(p.h, s.y, true, col.data());
+        for (int k = 0; k < p.h; ++k) ty[size_t(k) * 8 + q] = col[size_t(k)];
+        point_table(p.d, s.z, true, col.data());
+        for (int k = 0; k < p.d; ++k) tz[size_t(k) * 8 + q] = col[size_t(k)];
+        ++q;
+      }
+      q = 0;
+      for (int rid : P.recs) {
+        const RecorderDef& rd = c.topo.recorders[rid];
+        point_table(p.h, rd.ly, false, col.data());
+        for (int k = 0; k < p.h; ++k) ty[size_t(k) * 8 + 4 + q] = col[size_t(k)];
+        point_table(p.d, rd.lz, false, col.data());
+        for (int k = 0; k < p.d; ++k) tz[size_t(k) * 8 + 4 + q] = col[size_t(k)];
+        ++q;
+      }
+      float *dy, *dz;
+      if ((r = upload_vec(c, (void**)&dy, ty.data(), ty.size() * 4))) return r;
+      if ((r = upload_vec(c, (void**)&dz, tz.data(), tz.size() * 4))) return r;
+      d.pty = dy;
+      d.ptz = dz;
+    }
+    // one launch for all partitions: the point terms are uniform branches, and
+    // one grid avoids the tail of a second, smaller launch
+    groups[std::make_pair(P.rpw, 1)].push_back(d);
   }
   B.modal.clear();
   for (auto& kv : groups) {

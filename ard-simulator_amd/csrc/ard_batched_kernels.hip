@@ -235,24 +235,17 @@ __global__ __launch_bounds__(256) void k_modal(const ModalDesc* __restrict__ des
       tz[r][q] = 0.f;
     }
   }
+  // point forces and recorders: x factors per lane; y and z factors from the
+  // partition's point tables ([H][8] and [D][8]: 4 sources, then 4 recorders)
   const int nsrc = PTS ? min(P.nsrc, ARD_MAX_SRC_PER_PART) : 0;
   const int nrec = PTS ? P.nrec : 0;
   float sfx[ARD_MAX_SRC_PER_PART], rfx[ARD_MAX_REC_PER_PART], rec_acc[ARD_MAX_REC_PER_PART];
-  float sfy[PTS ? RPW : 1][ARD_MAX_SRC_PER_PART], rfy[PTS ? RPW : 1][ARD_MAX_REC_PER_PART];
 #pragma unroll
-  for (int q = 0; q < ARD_MAX_SRC_PER_PART; ++q) {
-    sfx[q] = PTS && q < nsrc ? P.src[q].fx[kxc] * *P.src[q].val : 0.f;
-#pragma unroll
-    for (int r = 0; r < (PTS ? RPW : 1); ++r)
-      sfy[r][q] = PTS && q < nsrc ? P.src[q].fy[min(ky0 + r, H - 1)] : 0.f;
-  }
+  for (int q = 0; q < ARD_MAX_SRC_PER_PART; ++q) sfx[q] = PTS && q < nsrc ? P.src[q].fx[kxc] * *P.src[q].val : 0.f;
 #pragma unroll
   for (int q = 0; q < ARD_MAX_REC_PER_PART; ++q) {
     rfx[q] = PTS && q < nrec && xok ? P.rec[q].fx[kxc] : 0.f;
     rec_acc[q] = 0.f;
-#pragma unroll
-    for (int r = 0; r < (PTS ? RPW : 1); ++r)
-      rfy[r][q] = PTS && q < nrec ? P.rec[q].fy[min(ky0 + r, H - 1)] : 0.f;
   }
 
   // streamed operands of one plane: modes of the rows (clamped rows) and both
@@ -287,11 +280,7 @@ __global__ __launch_bounds__(256) void k_modal(const ModalDesc* __restrict__ des
     for (int q = 0; q < 6; ++q)
 #pragma unroll
       for (int r = 0; r < RPW; ++r) gxr[q][r] = gxp[r * 6 + q];
-    float sz[ARD_MAX_SRC_PER_PART], rzp[ARD_MAX_REC_PER_PART];
-#pragma unroll
-    for (int q = 0; q < ARD_MAX_SRC_PER_PART; ++q) sz[q] = PTS && q < nsrc ? P.src[q].fz[kz] : 0.f;
-#pragma unroll
-    for (int q = 0; q < ARD_MAX_REC_PER_PART; ++q) rzp[q] = PTS && q < nrec ? P.rec[q].fz[kz] : 0.f;
+    const cfloat* ptz = ARD_CC(P.ptz) + kz * 8;
     float ty[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
 #pragma unroll
     for (int r = 0; r < RPW; ++r) {
@@ -316,10 +305,11 @@ __global__ __launch_bounds__(256) void k_modal(const ModalDesc* __restrict__ des
       fh = fmaf(ez0, gz[r][PZ * 3 + 0], fh);
       fh = fmaf(ez1, gz[r][PZ * 3 + 1], fh);
       fh = fmaf(ez2, gz[r][PZ * 3 + 2], fh);
+      const cfloat* pty = ARD_CC(P.pty) + min(ky0 + r, H - 1) * 8;
       if (PTS) {
 #pragma unroll
         for (int q = 0; q < ARD_MAX_SRC_PER_PART; ++q)
-          if (q < nsrc) fh = fmaf(sfx[q], sfy[r][q] * sz[q], fh);
+          if (q < nsrc) fh = fmaf(sfx[q], pty[q] * ptz[q], fh);
       }
       const float mv = fmaf(a, in.mc[r], fmaf(-0.999f, in.mp[r], g * fh));
       const bool ok = rok[r] && xok;
@@ -335,7 +325,7 @@ __global__ __launch_bounds__(256) void k_modal(const ModalDesc* __restrict__ des
       if (PTS) {
 #pragma unroll
         for (int q = 0; q < ARD_MAX_REC_PER_PART; ++q)
-          if (q < nrec) rec_acc[q] = fmaf(rfx[q] * rfy[r][q] * rzp[q], mn, rec_acc[q]);
+          if (q < nrec) rec_acc[q] = fmaf(rfx[q] * (pty[4 + q] * ptz[4 + q]), mn, rec_acc[q]);
       }
     }
 #pragma unroll
