@@ -20,6 +20,7 @@ Workloads (all cell coordinates are global):
 from __future__ import annotations
 
 import dataclasses
+import os
 from typing import List, Optional, Tuple
 
 import numpy as np
@@ -245,3 +246,32 @@ def rooms64(seed=5, steps=1500, h=0.05, n_rooms=64):
         x0 += W + 2  # 2 m gap: PML slabs (5 cells) never touch the next room
     return Scene("rooms64", h, DT_FOR_H[0.05], 343.0, boxes, srcs, recs, steps, recorder_mode=REC_LOCAL,
                  room_of_box=room_of)
+
+
+def write_cell_files(scene, directory):
+    """Writes scene.txt / sources.txt / recorders.txt in the reference's
+    whitespace format (ARD/partition.cpp:151-176, ARD/sound_source.cpp:24-48,
+    ARD/recorder.cpp:67-89) with values in cells, as read by
+    ``ard_run --cells``. Gaussian sources only."""
+    os.makedirs(directory, exist_ok=True)
+    with open(os.path.join(directory, "scene.txt"), "w") as f:
+        for b in scene.boxes:
+            f.write(" ".join(str(int(v)) for v in b) + "\n")
+    with open(os.path.join(directory, "sources.txt"), "w") as f:
+        for s in scene.sources:
+            if s.wave is not None:
+                raise ValueError("waveform sources have no file form")
+            f.write("%d %d %d\n" % (s.x, s.y, s.z))
+    with open(os.path.join(directory, "recorders.txt"), "w") as f:
+        for r in scene.recorders:
+            f.write("%d %d %d\n" % tuple(r))
+
+
+def read_cell_files(directory):
+    """Boxes, source points and recorder points of write_cell_files' format."""
+    def rows(name, n):
+        with open(os.path.join(directory, name)) as f:
+            v = [int(t) for t in f.read().split()]
+        return [tuple(v[i : i + n]) for i in range(0, len(v) - len(v) % n, n)]
+
+    return rows("scene.txt", 6), rows("sources.txt", 3), rows("recorders.txt", 3)

@@ -15,7 +15,8 @@ HIP_SOURCES = ["ard_kernels.hip", "ard_batched_kernels.hip", "ard_batched.hip", 
 CXX_SOURCES = ["ard_topology.cpp"]
 HOST_SOURCES = ["host/ard_host.cpp"]
 
-HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
+HIPCC = os.environ.get("HIPCC", os.path.join(ROCM, "bin", "hipcc"))
 ARCH = "gfx950"
 # Packed FP32 instructions are disabled for all device code.
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-parameter", "-Wno-unused-variable"]
@@ -51,8 +52,9 @@ def build(verbose=False):
     run([HIPCC, "-shared", "-o", tmp] + objs + ["--offload-arch=" + ARCH, "-fPIC"])
     os.replace(tmp, OUT)
     # command-line driver (the reference's main loop without the window)
-    run(["g++", "-O2", "-std=c++17", "-o", os.path.join(HERE, "ard_run"), os.path.join(HERE, "host", "ard_run.cpp"),
-         "-L" + HERE, "-lard", "-Wl,-rpath,$ORIGIN"])
+    run(["g++", "-O2", "-std=c++17", "-D__HIP_PLATFORM_AMD__", "-I" + os.path.join(ROCM, "include"),
+         "-o", os.path.join(HERE, "ard_run"), os.path.join(HERE, "host", "ard_run.cpp"),
+         "-L" + HERE, "-lard", "-L" + os.path.join(ROCM, "lib"), "-lamdhip64", "-Wl,-rpath,$ORIGIN"])
     if verbose:
         print("built", OUT)
     return OUT

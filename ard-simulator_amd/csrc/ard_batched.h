@@ -53,7 +53,8 @@ struct ModalDesc {
   const float* wx2;   // [W] u_x = (c0 pi (kx+1) / Lx)^2 dt^2 / 4, and likewise for y, z
   const float* wy2;
   const float* wz2;
-  float* M[2];
+  float* M[2];       // [D][H][Wp]
+  int Wp;            // mode row pitch (W rounded up to 32)
   const float* Ex;  // [W][3]  2 c_W(kx, x)
   const float* Rx;  // [W][3]  e(kx) c_W(kx, x)
   const float* Ey;

@@ -60,7 +60,8 @@ struct BPart {
   float* G[3] = {};   // folded force spectra
   float* P[3] = {};   // face-slab pressure
   float* T[3] = {};   // partial reductions [nchunk][6][plane]
-  float* M[2] = {};
+  float* M[2] = {};   // modes [D][H][wp], rows padded to whole 128-byte lines
+  int wp = 0;
   float* w2[3] = {};     // per-axis squared wavenumbers (c0 pi (k+1) / L)^2
   float gscale = 0.f;
   float* E[3] = {};   // Ex, Ey, Ez
@@ -116,6 +117,15 @@ struct BatchedState {
   bool graphs_valid = false;
   // tiling of the modal pass and transform kernel, chosen at finalize
   int rpw = 4, kz_chunk = 48, gemm_variant = 1, tuned = 0;
+  // second stream and events: in the step graph the PML slabs run beside the
+  // DCT chain (fold .. inverse transforms), which they share no data with
+  hipStream_t side = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  ~BatchedState() {
+    if (ev_fork) hipEventDestroy(ev_fork);
+    if (ev_join) hipEventDestroy(ev_join);
+    if (side) hipStreamDestroy(side);
+  }
 };
 
 namespace {
@@ -238,6 +248,7 @@ int build_modal(Ctx& c) {
     d.gscale = P.gscale;
     d.wx2 = P.w2[0]; d.wy2 = P.w2[1]; d.wz2 = P.w2[2];
     d.M[0] = P.M[0]; d.M[1] = P.M[1];
+    d.Wp = P.wp;
     d.Ex = P.E[0]; d.Rx = P.R[0]; d.Ey = P.E[1]; d.Ry = P.R[1]; d.Ez = P.E[2]; d.Rz = P.R[2];
     d.Gx = P.G[0]; d.Gy = P.G[1]; d.Gz = P.G[2];
     d.yt = P.yz[0]; d.zt = P.yz[1];
@@ -271,8 +282,9 @@ int build_modal(Ctx& c) {
       d.rec = dr;
     }
     d.rec_part = P.rec_part;
-    if (d.nsrc + d.nrec > 0) {
-      // y and z factors of the points, 8 per wavenumber (sources, then recorders)
+    {
+      // y and z factors of the points, 8 per wavenumber (sources, then
+      // recorders); all zero for a partition without points
       std::vector<float> ty(size_t(p.h) * 8, 0.f), tz(size_t(p.d) * 8, 0.f), col(size_t(std::max(p.h, p.d)));
       int q = 0;
       for (auto& s : B.srcs) {
@@ -447,6 +459,12 @@ int finalize_batched(Ctx& c) {
   if ((r = setup_common(c))) return r;
   c.bs = new BatchedState();
   BatchedState& B = *c.bs;
+  {
+    hipError_t e = hipStreamCreateWithFlags(&B.side, hipStreamNonBlocking);  // default priority
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&B.ev_fork, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&B.ev_join, hipEventDisableTiming);
+    if (e != hipSuccess) return hip_fail(e, "side stream");
+  }
   const int np = int(c.topo.parts.size());
   B.parts.assign(np, BPart());
   for (size_t i = 0; i < c.topo.recorders.size(); ++i) B.parts[c.topo.recorders[i].part].recs.push_back(int(i));
@@ -460,7 +478,7 @@ int finalize_batched(Ctx& c) {
     if (int(P.recs.size()) > ARD_MAX_REC_PER_PART)
       return set_error(ARD_ERR_UNSUPPORTED, "more than 4 recorders in one partition");
     const long long W = p.w, H = p.h, D = p.d, N = W * H * D;
-    if (N >= (1LL << 30))  // 32-bit cell and byte offsets in the modal pass
+    if (D * H * ceil_div(W, 32) * 32 >= (1LL << 30))  // 32-bit cell and byte offsets in the modal pass
       return set_error(ARD_ERR_UNSUPPORTED, "a DCT partition of 2^30 or more cells (split it)");
     P.plane[0] = D * H; P.plane[1] = D * W; P.plane[2] = H * W;
     // partial-reduction buffers are sized for the finest tiling the
@@ -472,17 +490,21 @@ int finalize_batched(Ctx& c) {
     P.nbz = int(ceil_div(D, 16));
     const long long shell = 6 * (P.plane[0] + P.plane[1] + P.plane[2]);
     const long long tsz = 6 * (P.nbx * P.plane[0] + P.nby * P.plane[1] + P.nbz * P.plane[2]);
+    P.wp = int(ceil_div(W, 32) * 32);
+    const long long NP = D * H * P.wp;  // padded mode array
+    const long long total = 5 * shell + tsz + 64 + 2 * NP;
     float* blk;
-    if ((r = c.dalloc((void**)&blk, size_t(5 * shell + tsz + 2 * N) * 4))) return r;
-    if ((r = c.dzero(blk, size_t(5 * shell + tsz + 2 * N) * 4))) return r;
+    if ((r = c.dalloc((void**)&blk, size_t(total) * 4))) return r;
+    if ((r = c.dzero(blk, size_t(total) * 4))) return r;
     float* q = blk;
     for (float** arr : {P.F, P.S, P.U, P.G, P.P})
       for (int w = 0; w < 3; ++w) { arr[w] = q; q += 6 * P.plane[w]; }
     P.T[0] = q; q += 6 * P.nbx * P.plane[0];
     P.T[1] = q; q += 6 * P.nby * P.plane[1];
     P.T[2] = q; q += 6 * P.nbz * P.plane[2];
-    P.M[0] = q; q += N;
-    P.M[1] = q; q += N;
+    q = blk + ceil_div(q - blk, 64) * 64;  // 256-byte aligned
+    P.M[0] = q; q += NP;
+    P.M[1] = q; q += NP;
     // modal coefficients are formed in the kernel from per-axis terms
     // u = (w dt / 2)^2 of the 1-based wavenumbers (ARD/dct_partition.cpp:24-33),
     // tabulated in FP64; the kernel's series for sin^2 holds for u <= 3, past
@@ -636,7 +658,16 @@ int enqueue_step(Ctx& c, int par, PhaseTimer* T) {
     if (T) T->end(st, ev);
   };
   phase(ARD_PH_SOURCES, [&] { return launch_bsources(B.d_src, B.nsrc, c.d_step, c.courant, st); });
-  phase(ARD_PH_PML, [&] { return launch_pml(B.d_pml, B.npml, B.pml_blocks, par, c.prm.h, c.prm.dt, c.prm.c0, st); });
+  // captured: PML on the side stream, joined before the interface, which reads
+  // the new PML pressure and writes PML forces; timed: all in order on one stream
+  const bool fork = !T && B.side;
+  if (fork) {
+    r |= hipEventRecord(B.ev_fork, st) != hipSuccess;
+    r |= hipStreamWaitEvent(B.side, B.ev_fork, 0) != hipSuccess;
+  }
+  hipStream_t sp = fork ? B.side : st;
+  phase(ARD_PH_PML, [&] { return launch_pml(B.d_pml, B.npml, B.pml_blocks, par, c.prm.h, c.prm.dt, c.prm.c0, sp); });
+  if (fork) r |= hipEventRecord(B.ev_join, B.side) != hipSuccess;
   phase(ARD_PH_FOLD, [&] { return launch_fold(B.d_fold, B.d_fold_start, B.nfold, B.fold_total, st); });
   phase(ARD_PH_FACE_FWD1, [&] { return launch_gemm(B.d_gemm[0], B.ngemm[0], B.gemm_tiles[0], true, true, 0, st, B.gemm_variant); });
   phase(ARD_PH_FACE_FWD2, [&] { return launch_gemm(B.d_gemm[1], B.ngemm[1], B.gemm_tiles[1], false, false, 0, st, B.gemm_variant); });
@@ -645,6 +676,7 @@ int enqueue_step(Ctx& c, int par, PhaseTimer* T) {
   phase(ARD_PH_UNFOLD, [&] { return launch_unfold(B.d_unfold, B.d_unfold_start, B.nunfold, B.unfold_total, st); });
   phase(ARD_PH_FACE_INV1, [&] { return launch_gemm(B.d_gemm[2], B.ngemm[2], B.gemm_tiles[2], true, true, 0, st, B.gemm_variant); });
   phase(ARD_PH_FACE_INV2, [&] { return launch_gemm(B.d_gemm[3], B.ngemm[3], B.gemm_tiles[3], false, false, 0, st, B.gemm_variant); });
+  if (fork) r |= hipStreamWaitEvent(st, B.ev_join, 0) != hipSuccess;
   // the interface reads the PML pressure just written: buffer par ^ 1
   phase(ARD_PH_INTERFACE, [&] { return launch_interface(B.d_bnd, B.nbnd, B.d_parts, par ^ 1, B.bnd_total, st); });
   phase(ARD_PH_RECORD, [&] {
@@ -780,6 +812,7 @@ int batched_pressure_field(Ctx& c, int part, const float** out) {
   const float* M = B.parts[part].M[c.time_step & 1];
   GemmDesc g[3];
   g[0] = gemm_k_contig(iw, p.w, M, B.dense[0], int(H * D), W);
+  g[0].b1 = B.parts[part].wp;  // padded mode rows
   // along y per z-plane: n = (x, z)
   g[1] = GemmDesc{};
   g[1].A = ih; g[1].B = B.dense[0]; g[1].C = B.dense[1];

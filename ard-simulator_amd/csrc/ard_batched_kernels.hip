@@ -132,6 +132,14 @@ __device__ __forceinline__ float sinc2_poly(float u) {
   return fmaf(p, u, 1.0f);
 }
 
+// Workgroups are dealt to the 8 XCDs in turn (block b to XCD b % 8). This
+// renumbering gives each XCD one contiguous range of logical blocks, so tiles
+// that share face spectra and cache lines share that XCD's L2.
+__device__ __forceinline__ int xcd_block(int bid, int n) {
+  const int x = bid & 7, i = bid >> 3, q = n >> 3, r = n & 7;
+  return x * q + min(x, r) + i;
+}
+
 typedef __amdgpu_buffer_rsrc_t brsrc;
 // raw buffer resource over n bytes (gfx9 word 3: 32-bit float data, no swizzle)
 __device__ __forceinline__ brsrc make_rsrc(const void* p, unsigned nbytes) {
@@ -149,10 +157,15 @@ __device__ __forceinline__ float bload(brsrc r, unsigned voff, unsigned soff) {
 // Wave w owns rows ky0 = kyb + w RPW .. ky0 + RPW - 1 (parity r & 1 known at
 // compile time) and walks the chunk's kz planes in pairs (parity known), with
 // the next plane's loads in flight during the current one. z sums stay in
-// registers. Per batch of KB planes, each wave leaves its y sums and the new
-// modes in LDS; after a barrier the block forms the batch's x sums from the
-// mode tile and adds the four waves' y sums, then a second barrier frees LDS.
-// (No LDS atomics: float atomics on LDS serialise.)
+// registers. The chunk goes in batches of KB planes:
+//   planes   each wave: new modes of its rows -> global, LDS mode tile, its y sums -> LDS
+//   barrier
+//   reduce   x sums (and recorder sums) of the batch from the mode tile; the
+//            four waves' y sums added; the next batch's per-row and per-plane
+//            factors (x-face spectra, point-force factors, z tables) staged in LDS
+//   barrier
+// so the plane loop reads no scalar memory and no LDS atomics (float atomics
+// on LDS serialise).
 template <int RPW, bool PTS>
 __global__ __launch_bounds__(256) void k_modal(const ModalDesc* __restrict__ descs,
                                                const long long* __restrict__ starts, int ndesc, int par) {
@@ -160,11 +173,18 @@ __global__ __launch_bounds__(256) void k_modal(const ModalDesc* __restrict__ des
   constexpr int TY = 4 * RPW;  // rows of the block tile
   constexpr int KB = 4;        // kz planes per batch of the x and y reductions
   constexpr int LD = 68;       // padded row of the mode tile
+  constexpr int NPT = ARD_MAX_SRC_PER_PART;
+  constexpr int NRC = ARD_MAX_REC_PER_PART;
+  static_assert(NPT == 4 && NRC == 4, "point tables hold 4 + 4 factors");
   __shared__ __attribute__((aligned(16))) float s_m[KB][TY][LD];     // new modes of a batch
   __shared__ __attribute__((aligned(16))) float s_ty[KB][6][4][64];  // y sums: [plane][ky parity * 3 + j][wave][lane]
   __shared__ __attribute__((aligned(16))) float s_rx[64 * 3];        // x reduction weights of the tile's kx
-  __shared__ float s_rec[4][ARD_MAX_REC_PER_PART];
-  const int bid = blockIdx.x;
+  __shared__ __attribute__((aligned(16))) float s_gx[KB][TY][8];     // x-face spectra of the batch's rows (6 used)
+  __shared__ __attribute__((aligned(16))) float s_zt[KB][8];         // z factors of the batch's planes
+  __shared__ __attribute__((aligned(16))) float s_pc[PTS ? KB : 1][TY][NPT];  // point-force y*z factors
+  __shared__ __attribute__((aligned(16))) float s_rfx[PTS ? NRC : 1][64];     // recorder x factors
+  __shared__ float s_rec[4][NRC];
+  const int bid = xcd_block(int(blockIdx.x), int(gridDim.x));
   const int di = find_desc_ll(starts, ndesc, bid);
   const ModalDesc& P = descs[di];
   const int lb = bid - int(starts[di]);
@@ -184,16 +204,17 @@ __global__ __launch_bounds__(256) void k_modal(const ModalDesc* __restrict__ des
   const int kz0 = bz * P.kz_chunk;   // kz_chunk is even, so kz0 is even
   const int kz1 = min(D, kz0 + P.kz_chunk);
   const int HW = H * W, DH = D * H, DW = D * W;  // partitions hold < 2^30 cells
-  const cfloat* __restrict__ Gxw = ARD_CC(P.Gx) + ky0 * 6;  // [D][H][6]: this wave's rows of a plane adjacent
-  const cfloat* __restrict__ Zt = ARD_CC(P.zt);
-  const brsrc rMc = make_rsrc(P.M[par], unsigned(D * HW) * 4u);
-  const brsrc rMp = make_rsrc(P.M[par ^ 1], unsigned(D * HW) * 4u);
+  const int Wp = P.Wp, HWp = H * Wp;            // mode arrays: rows padded to 128-byte lines
+  const brsrc rMc = make_rsrc(P.M[par], unsigned(D * HWp) * 4u);
+  const brsrc rMp = make_rsrc(P.M[par ^ 1], unsigned(D * HWp) * 4u);
   const brsrc rGy = make_rsrc(P.Gy, unsigned(6 * DW) * 4u);
   gfloat* __restrict__ Mp = ARD_G(P.M[par ^ 1]);
   gfloat* __restrict__ Txp = ARD_G(P.Tx) + (long long)bx * 6 * DH;
   gfloat* __restrict__ Typ = ARD_G(P.Ty) + (long long)by * 6 * DW;
   const float gsc = P.gscale;  // .999 * 4 * s_f * s_i * dt^2 / 4
   const unsigned vx = unsigned(kxc) * 4u;
+  const int nsrc = PTS ? min(P.nsrc, NPT) : 0;
+  const int nrec = PTS ? min(P.nrec, NRC) : 0;
 
   // x factors, split by parity so that the folded spectra need no select
   float exs[3], exd[3];
@@ -207,16 +228,21 @@ __global__ __launch_bounds__(256) void k_modal(const ModalDesc* __restrict__ des
 #pragma unroll
     for (int i = 0; i < 3; ++i) s_rx[lane * 3 + i] = xok ? ARD_CG(P.Rx)[kxc * 3 + i] : 0.f;
   }
+  if (PTS && wave == 1) {
+#pragma unroll
+    for (int q = 0; q < NRC; ++q) s_rfx[q][lane] = q < nrec && xok ? P.rec[q].fx[kxc] : 0.f;
+  }
   const float ux = ARD_CG(P.wx2)[kxc];
   // per-row factors of this wave's rows (wave-uniform)
   float ey[RPW][3], ry[RPW][3], uy[RPW];
   bool rok[RPW];
-  int rof[RPW];  // row offset ky * W (clamped)
+  int rof[RPW], rom[RPW];  // row offsets ky * W of the face tables and ky * Wp of the modes (clamped)
 #pragma unroll
   for (int r = 0; r < RPW; ++r) {
     const int kyc = min(ky0 + r, H - 1);
     rok[r] = ky0 + r < H;
     rof[r] = kyc * W;
+    rom[r] = kyc * Wp;
     const cfloat* yt = ARD_CC(P.yt) + kyc * 8;
     uy[r] = yt[6];
 #pragma unroll
@@ -235,18 +261,49 @@ __global__ __launch_bounds__(256) void k_modal(const ModalDesc* __restrict__ des
       tz[r][q] = 0.f;
     }
   }
-  // point forces and recorders: x factors per lane; y and z factors from the
-  // partition's point tables ([H][8] and [D][8]: 4 sources, then 4 recorders)
-  const int nsrc = PTS ? min(P.nsrc, ARD_MAX_SRC_PER_PART) : 0;
-  const int nrec = PTS ? P.nrec : 0;
-  float sfx[ARD_MAX_SRC_PER_PART], rfx[ARD_MAX_REC_PER_PART], rec_acc[ARD_MAX_REC_PER_PART];
+  // point forces: x factor (times the source value) per lane
+  float sfx[NPT];
 #pragma unroll
-  for (int q = 0; q < ARD_MAX_SRC_PER_PART; ++q) sfx[q] = PTS && q < nsrc ? P.src[q].fx[kxc] * *P.src[q].val : 0.f;
+  for (int q = 0; q < NPT; ++q) sfx[q] = PTS && q < nsrc ? P.src[q].fx[kxc] * *P.src[q].val : 0.f;
+  float rec_acc[NRC];  // recorder sums of this thread's (plane, row) slots
 #pragma unroll
-  for (int q = 0; q < ARD_MAX_REC_PER_PART; ++q) {
-    rfx[q] = PTS && q < nrec && xok ? P.rec[q].fx[kxc] : 0.f;
-    rec_acc[q] = 0.f;
-  }
+  for (int q = 0; q < NRC; ++q) rec_acc[q] = 0.f;
+
+  // factors of one batch staged in LDS: x-face spectra (6 of 8 per row),
+  // z tables (8 per plane), point-force y*z factors (4 per row). Loaded into
+  // registers one batch ahead, stored after the batch's first barrier.
+  constexpr int NGX = KB * TY * 8, NZT = KB * 8, NPC = PTS ? KB * TY * NPT : 0;
+  constexpr int NSTG = (NGX + NZT + NPC + 255) / 256;
+  float stg[NSTG];
+  auto stage_load = [&](const int kb) {
+#pragma unroll
+    for (int t = 0; t < NSTG; ++t) {
+      const int e = tid + 256 * t;
+      float v = 0.f;
+      if (e < NGX) {
+        const int q = e & 7, row = (e >> 3) % TY, kz = min(kb + e / (8 * TY), kz1 - 1);
+        const int ky = min(kyb + row, H - 1);
+        v = q < 6 ? ARD_CG(P.Gx)[(kz * H + ky) * 6 + q] : 0.f;
+      } else if (e < NGX + NZT) {
+        const int f = e - NGX, kz = min(kb + (f >> 3), kz1 - 1);
+        v = ARD_CG(P.zt)[kz * 8 + (f & 7)];
+      } else if (PTS && nsrc > 0 && e < NGX + NZT + NPC) {  // point tables exist only with points
+        const int f = e - NGX - NZT, q = f % NPT, row = (f / NPT) % TY, kz = min(kb + f / (NPT * TY), kz1 - 1);
+        const int ky = min(kyb + row, H - 1);
+        v = ARD_CG(P.pty)[ky * 8 + q] * ARD_CG(P.ptz)[kz * 8 + q];
+      }
+      stg[t] = v;
+    }
+  };
+  auto stage_store = [&]() {
+#pragma unroll
+    for (int t = 0; t < NSTG; ++t) {
+      const int e = tid + 256 * t;
+      if (e < NGX) (&s_gx[0][0][0])[e] = stg[t];
+      else if (e < NGX + NZT) (&s_zt[0][0])[e - NGX] = stg[t];
+      else if (PTS && e < NGX + NZT + NPC) (&s_pc[0][0][0])[e - NGX - NZT] = stg[t];
+    }
+  };
 
   // streamed operands of one plane: modes of the rows (clamped rows) and both
   // y-parity spectra
@@ -254,11 +311,11 @@ __global__ __launch_bounds__(256) void k_modal(const ModalDesc* __restrict__ des
     float mc[RPW], mp[RPW], gy[6];
   };
   auto load = [&](PlaneIn& in, const int kz) {
-    const unsigned po = unsigned(kz * HW);
+    const unsigned po = unsigned(kz * HWp);
 #pragma unroll
     for (int r = 0; r < RPW; ++r) {
-      in.mc[r] = bload(rMc, vx, (po + unsigned(rof[r])) * 4u);
-      in.mp[r] = bload(rMp, vx, (po + unsigned(rof[r])) * 4u);
+      in.mc[r] = bload(rMc, vx, (po + unsigned(rom[r])) * 4u);
+      in.mp[r] = bload(rMp, vx, (po + unsigned(rom[r])) * 4u);
     }
 #pragma unroll
     for (int q = 0; q < 6; ++q) in.gy[q] = bload(rGy, vx, unsigned(q * DW + kz * W) * 4u);
@@ -267,24 +324,17 @@ __global__ __launch_bounds__(256) void k_modal(const ModalDesc* __restrict__ des
   // one kz plane of this wave's rows
   auto plane = [&](const PlaneIn& in, const int kz, const int kl, auto pzc) {
     constexpr int PZ = decltype(pzc)::value;
-    const cfloat* zt = Zt + kz * 8;
-    const float ez0 = zt[0], ez1 = zt[1], ez2 = zt[2];
-    const float rz0 = zt[3], rz1 = zt[4], rz2 = zt[5];
-    const float uxz = ux + zt[6];
-    const unsigned po = unsigned(kz * HW);
-    // folded x spectra of the rows: Gx[kz][ky0 + r][q], 6 RPW adjacent floats
-    // (rows past H read the next plane's values, inside the allocation; unused)
-    const cfloat* gxp = Gxw + kz * H * 6;
-    float gxr[6][RPW];
-#pragma unroll
-    for (int q = 0; q < 6; ++q)
-#pragma unroll
-      for (int r = 0; r < RPW; ++r) gxr[q][r] = gxp[r * 6 + q];
-    const cfloat* ptz = ARD_CC(P.ptz) + kz * 8;
+    const float4 za = *reinterpret_cast<const float4*>(&s_zt[kl][0]);  // Ez[kz][0..2], Rz[kz][0]
+    const float4 zb = *reinterpret_cast<const float4*>(&s_zt[kl][4]);  // Rz[kz][1..2], u_z, 0
+    const float uxz = ux + zb.z;
+    const unsigned po = unsigned(kz * HWp);
     float ty[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
 #pragma unroll
     for (int r = 0; r < RPW; ++r) {
       const int PY = r & 1;
+      const int row = wave * RPW + r;
+      const float4 ga = *reinterpret_cast<const float4*>(&s_gx[kl][row][0]);  // folded x spectra q 0..3
+      const float4 gb = *reinterpret_cast<const float4*>(&s_gx[kl][row][4]);  // q 4..5
       // modal coefficients (ARD/dct_partition.cpp:24-33,68) from u = (w dt / 2)^2,
       // separable in the 1-based wavenumbers:
       //   s^2 = sin^2(w dt / 2) = u S(u),  a = .999 * 2 cos(w dt) = 1.998 - 3.996 s^2,
@@ -293,40 +343,36 @@ __global__ __launch_bounds__(256) void k_modal(const ModalDesc* __restrict__ des
       const float S = sinc2_poly(u);
       const float a = fmaf(-3.996f, u * S, 1.998f);
       const float g = gsc * S;
-      float fh = exs[0] * gxr[0][r];
-      fh = fmaf(exs[1], gxr[1][r], fh);
-      fh = fmaf(exs[2], gxr[2][r], fh);
-      fh = fmaf(exd[0], gxr[3][r], fh);
-      fh = fmaf(exd[1], gxr[4][r], fh);
-      fh = fmaf(exd[2], gxr[5][r], fh);
+      float fh = exs[0] * ga.x;
+      fh = fmaf(exs[1], ga.y, fh);
+      fh = fmaf(exs[2], ga.z, fh);
+      fh = fmaf(exd[0], ga.w, fh);
+      fh = fmaf(exd[1], gb.x, fh);
+      fh = fmaf(exd[2], gb.y, fh);
       fh = fmaf(ey[r][0], in.gy[PY * 3 + 0], fh);
       fh = fmaf(ey[r][1], in.gy[PY * 3 + 1], fh);
       fh = fmaf(ey[r][2], in.gy[PY * 3 + 2], fh);
-      fh = fmaf(ez0, gz[r][PZ * 3 + 0], fh);
-      fh = fmaf(ez1, gz[r][PZ * 3 + 1], fh);
-      fh = fmaf(ez2, gz[r][PZ * 3 + 2], fh);
-      const cfloat* pty = ARD_CC(P.pty) + min(ky0 + r, H - 1) * 8;
-      if (PTS) {
-#pragma unroll
-        for (int q = 0; q < ARD_MAX_SRC_PER_PART; ++q)
-          if (q < nsrc) fh = fmaf(sfx[q], pty[q] * ptz[q], fh);
+      fh = fmaf(za.x, gz[r][PZ * 3 + 0], fh);
+      fh = fmaf(za.y, gz[r][PZ * 3 + 1], fh);
+      fh = fmaf(za.z, gz[r][PZ * 3 + 2], fh);
+      if (PTS && nsrc > 0) {
+        const float4 pc = *reinterpret_cast<const float4*>(&s_pc[PTS ? kl : 0][row][0]);
+        fh = fmaf(sfx[0], pc.x, fh);
+        fh = fmaf(sfx[1], pc.y, fh);
+        fh = fmaf(sfx[2], pc.z, fh);
+        fh = fmaf(sfx[3], pc.w, fh);
       }
       const float mv = fmaf(a, in.mc[r], fmaf(-0.999f, in.mp[r], g * fh));
       const bool ok = rok[r] && xok;
-      if (ok) Mp[po + unsigned(rof[r]) + unsigned(kx)] = mv;
+      if (ok) Mp[po + unsigned(rom[r]) + unsigned(kx)] = mv;
       const float mn = ok ? mv : 0.f;
-      s_m[kl][wave * RPW + r][lane] = mn;
-      tz[r][PZ * 3 + 0] = fmaf(rz0, mn, tz[r][PZ * 3 + 0]);
-      tz[r][PZ * 3 + 1] = fmaf(rz1, mn, tz[r][PZ * 3 + 1]);
-      tz[r][PZ * 3 + 2] = fmaf(rz2, mn, tz[r][PZ * 3 + 2]);
+      s_m[kl][row][lane] = mn;
+      tz[r][PZ * 3 + 0] = fmaf(za.w, mn, tz[r][PZ * 3 + 0]);
+      tz[r][PZ * 3 + 1] = fmaf(zb.x, mn, tz[r][PZ * 3 + 1]);
+      tz[r][PZ * 3 + 2] = fmaf(zb.y, mn, tz[r][PZ * 3 + 2]);
       ty[PY * 3 + 0] = fmaf(ry[r][0], mn, ty[PY * 3 + 0]);
       ty[PY * 3 + 1] = fmaf(ry[r][1], mn, ty[PY * 3 + 1]);
       ty[PY * 3 + 2] = fmaf(ry[r][2], mn, ty[PY * 3 + 2]);
-      if (PTS) {
-#pragma unroll
-        for (int q = 0; q < ARD_MAX_REC_PER_PART; ++q)
-          if (q < nrec) rec_acc[q] = fmaf(rfx[q] * (pty[4 + q] * ptz[4 + q]), mn, rec_acc[q]);
-      }
     }
 #pragma unroll
     for (int q = 0; q < 6; ++q) s_ty[kl][q][wave][lane] = ty[q];
@@ -335,14 +381,16 @@ __global__ __launch_bounds__(256) void k_modal(const ModalDesc* __restrict__ des
   // x and y sums of a finished batch (planes kb .. kb+n-1):
   // x: KB * TY (plane, row) slots, four threads per slot; thread qtr of a slot
   // takes kx = 16 j + 4 qtr + t (j, t < 4) and the four meet through quad
-  // permutes. The threads of a slot share its trip count, so every quad is whole.
+  // permutes. The threads of a slot share its trip count, so every quad is
+  // whole. The recorders' x sums ride along, weighted by their y*z factors.
   // y: the four waves' sums of each (plane, j, kx), four kx per thread.
   auto reduce = [&](const int kb, const int n) {
     const int qtr = tid & 3;
     for (int c = tid >> 2; c < KB * TY; c += 64) {
       const int kl = c / TY, row = c % TY;
       float ev0 = 0.f, ev1 = 0.f, ev2 = 0.f, od0 = 0.f, od1 = 0.f, od2 = 0.f;
-#pragma unroll
+      float rs[NRC] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll 1  // one group of 4 kx at a time: few registers live beside the prefetched plane
       for (int j = 0; j < 4; ++j) {
         const int k0 = 16 * j + 4 * qtr;  // even
         const float4 m = *reinterpret_cast<const float4*>(&s_m[kl][row][k0]);
@@ -353,13 +401,36 @@ __global__ __launch_bounds__(256) void k_modal(const ModalDesc* __restrict__ des
         od0 = fmaf(ra.w, m.y, od0); od1 = fmaf(rb.x, m.y, od1); od2 = fmaf(rb.y, m.y, od2);
         ev0 = fmaf(rb.z, m.z, ev0); ev1 = fmaf(rb.w, m.z, ev1); ev2 = fmaf(rc.x, m.z, ev2);
         od0 = fmaf(rc.y, m.w, od0); od1 = fmaf(rc.z, m.w, od1); od2 = fmaf(rc.w, m.w, od2);
+        if (PTS) {
+#pragma unroll
+          for (int q = 0; q < NRC; ++q) {
+            if (q < nrec) {
+              const float4 f = *reinterpret_cast<const float4*>(&s_rfx[PTS ? q : 0][k0]);
+              rs[q] = fmaf(f.x, m.x, fmaf(f.y, m.y, fmaf(f.z, m.z, fmaf(f.w, m.w, rs[q]))));
+            }
+          }
+        }
       }
 #define ARD_QUAD_SUM(v)                                                                    \
   v += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0xB1, 0xF, 0xF, false)); \
   v += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x4E, 0xF, 0xF, false));
       ARD_QUAD_SUM(ev0) ARD_QUAD_SUM(ev1) ARD_QUAD_SUM(ev2) ARD_QUAD_SUM(od0) ARD_QUAD_SUM(od1) ARD_QUAD_SUM(od2)
-#undef ARD_QUAD_SUM
       const int ky = kyb + row;
+      if (PTS && nrec > 0) {
+        const int kyc = min(ky, H - 1), kz = min(kb + kl, D - 1);
+#pragma unroll
+        for (int q = 0; q < NRC; ++q) {
+          if (q < nrec) {
+            ARD_QUAD_SUM(rs[q])
+            // rows past H and planes past the batch hold no modes of this step
+            const float w = (kl < n && ky < H && qtr == 0)
+                                ? ARD_CG(P.pty)[kyc * 8 + 4 + q] * ARD_CG(P.ptz)[kz * 8 + 4 + q]
+                                : 0.f;
+            rec_acc[q] = fmaf(w, rs[q], rec_acc[q]);
+          }
+        }
+      }
+#undef ARD_QUAD_SUM
       if (kl < n && ky < H && qtr < 2) {
         const int o = (kb + kl) * H + ky;
         if (qtr == 0) {
@@ -388,10 +459,13 @@ __global__ __launch_bounds__(256) void k_modal(const ModalDesc* __restrict__ des
   };
 
   PlaneIn cur, nxt;
+  stage_load(kz0);
   load(cur, kz0);
-  __syncthreads();  // s_rx
+  stage_store();
+  __syncthreads();  // s_rx, s_rfx, first batch's factors
   for (int kb = kz0; kb < kz1; kb += KB) {
     const int n = min(KB, kz1 - kb);
+    stage_load(kb + KB);  // next batch's factors (past the chunk: clamped, unused)
     for (int kl = 0; kl < n; kl += 2) {
       const int kz = kb + kl;
       // the next plane's loads go out before this plane's arithmetic
@@ -403,9 +477,10 @@ __global__ __launch_bounds__(256) void k_modal(const ModalDesc* __restrict__ des
         plane(nxt, kz + 1, kl + 1, std::integral_constant<int, 1>());
       }
     }
-    __syncthreads();  // batch in LDS
+    __syncthreads();  // batch in LDS; its factors no longer read
     reduce(kb, n);
-    __syncthreads();  // LDS free for the next batch
+    stage_store();
+    __syncthreads();  // LDS free for the next batch, its factors in place
   }
   // z sums of this kz chunk
   gfloat* tzp = ARD_G(P.Tz) + (long long)bz * 6 * HW;
@@ -418,7 +493,7 @@ __global__ __launch_bounds__(256) void k_modal(const ModalDesc* __restrict__ des
   // recorder partials of this block
   if (PTS && nrec > 0) {
 #pragma unroll
-    for (int q = 0; q < ARD_MAX_REC_PER_PART; ++q) {
+    for (int q = 0; q < NRC; ++q) {
       if (q < nrec) {
         float v = rec_acc[q];
 #pragma unroll
@@ -428,8 +503,7 @@ __global__ __launch_bounds__(256) void k_modal(const ModalDesc* __restrict__ des
     }
     __syncthreads();
     if (tid < nrec)
-      P.rec_part[(long long)lb * ARD_MAX_REC_PER_PART + tid] =
-          s_rec[0][tid] + s_rec[1][tid] + s_rec[2][tid] + s_rec[3][tid];
+      P.rec_part[(long long)lb * NRC + tid] = s_rec[0][tid] + s_rec[1][tid] + s_rec[2][tid] + s_rec[3][tid];
   }
 }
 

@@ -16,12 +16,15 @@ int Simulation::m_pml_layers = 5;
 int Simulation::m_form = ARD_FORM_BATCHED;
 int Simulation::m_recorder_mode = ARD_REC_REFERENCE;
 int Simulation::m_device = 0;
+int Simulation::m_autotune = 0;
 
 namespace {
 void check(int r, const char* what) {
   if (r < 0) throw Error(std::string(what) + ": " + ard_last_error());
 }
 int m2cell(int v) { return int(float(v) / Simulation::m_dh); }  // (int)(v / m_dh) in FP32
+// scene values in metres (the reference's files) or, with cell_units, in cells
+int to_cell(int v, bool cell_units) { return cell_units ? v : m2cell(v); }
 }  // namespace
 
 Partition::Partition(int xs, int ys, int zs, int w, int h, int d)
@@ -72,7 +75,7 @@ void Partition::Info() {
             << std::endl;
 }
 
-std::vector<std::shared_ptr<Partition>> Partition::ImportPartitions(const std::string& path) {
+std::vector<std::shared_ptr<Partition>> Partition::ImportPartitions(const std::string& path, bool cell_units) {
   std::vector<std::shared_ptr<Partition>> parts;
   std::ifstream file(path);
   if (!file) throw Error("cannot open " + path);
@@ -80,8 +83,9 @@ std::vector<std::shared_ptr<Partition>> Partition::ImportPartitions(const std::s
     int xs, ys, zs, w, h, d;
     file >> xs >> ys >> zs >> w >> h >> d;
     if (file.fail()) break;
-    parts.push_back(std::make_shared<DctPartition>(m2cell(xs), m2cell(ys), m2cell(zs), m2cell(w), m2cell(h),
-                                                   m2cell(d)));
+    parts.push_back(std::make_shared<DctPartition>(to_cell(xs, cell_units), to_cell(ys, cell_units),
+                                                   to_cell(zs, cell_units), to_cell(w, cell_units),
+                                                   to_cell(h, cell_units), to_cell(d, cell_units)));
   }
   return parts;
 }
@@ -92,7 +96,7 @@ real_t GaussianSource::SampleValue(real_t t) {
   return 1e9f * std::exp(-arg);
 }
 
-std::vector<std::shared_ptr<SoundSource>> SoundSource::ImportSources(const std::string& path) {
+std::vector<std::shared_ptr<SoundSource>> SoundSource::ImportSources(const std::string& path, bool cell_units) {
   std::vector<std::shared_ptr<SoundSource>> out;
   std::ifstream file(path);
   if (!file) throw Error("cannot open " + path);
@@ -100,7 +104,8 @@ std::vector<std::shared_ptr<SoundSource>> SoundSource::ImportSources(const std::
     int x, y, z;
     file >> x >> y >> z;
     if (file.fail()) break;
-    out.push_back(std::make_shared<GaussianSource>(m2cell(x), m2cell(y), m2cell(z)));
+    out.push_back(std::make_shared<GaussianSource>(to_cell(x, cell_units), to_cell(y, cell_units),
+                                                   to_cell(z, cell_units)));
   }
   return out;
 }
@@ -142,7 +147,7 @@ void Recorder::RecordField(int time_step) {
   if (response_.is_open()) response_ << v << std::endl;
 }
 
-std::vector<std::shared_ptr<Recorder>> Recorder::ImportRecorders(const std::string& path) {
+std::vector<std::shared_ptr<Recorder>> Recorder::ImportRecorders(const std::string& path, bool cell_units) {
   std::vector<std::shared_ptr<Recorder>> out;
   std::ifstream file(path);
   if (!file) throw Error("cannot open " + path);
@@ -150,7 +155,7 @@ std::vector<std::shared_ptr<Recorder>> Recorder::ImportRecorders(const std::stri
     int x, y, z;
     file >> x >> y >> z;
     if (file.fail()) break;
-    out.push_back(std::make_shared<Recorder>(m2cell(x), m2cell(y), m2cell(z),
+    out.push_back(std::make_shared<Recorder>(to_cell(x, cell_units), to_cell(y, cell_units), to_cell(z, cell_units),
                                              int(Simulation::m_duration / Simulation::m_dt)));
   }
   return out;
@@ -162,7 +167,7 @@ Simulation::Simulation(std::vector<std::shared_ptr<Partition>>& partitions,
     : recorders_(std::move(recorders)) {
   ard_params p{};
   p.h = m_dh; p.dt = m_dt; p.c0 = m_c0; p.pml_layers = m_pml_layers; p.absorption = Partition::m_absorption;
-  p.form = m_form; p.recorder_mode = m_recorder_mode;
+  p.form = m_form; p.recorder_mode = m_recorder_mode; p.autotune = m_autotune;
   check(ard_ctx_create(m_device, &p, &ctx_), "ard_ctx_create");
   try {
     for (auto& part : partitions) {

@@ -54,7 +54,9 @@ class Partition {
   void Info();
 
   // ARD/partition.cpp:151-176
-  static std::vector<std::shared_ptr<Partition>> ImportPartitions(const std::string& path);
+  // ARD/partition.cpp:151-176: boxes "x y z w h d" in metres; cell_units reads
+  // them as cell indices (seeded scenes whose points are not whole metres)
+  static std::vector<std::shared_ptr<Partition>> ImportPartitions(const std::string& path, bool cell_units = false);
 
   int x_start() const { return x_start_; }
   int y_start() const { return y_start_; }
@@ -98,7 +100,7 @@ class SoundSource {
   virtual ~SoundSource() = default;
   virtual real_t SampleValue(real_t t) = 0;
   // ARD/sound_source.cpp:24-48 (without writing source_<id>.txt)
-  static std::vector<std::shared_ptr<SoundSource>> ImportSources(const std::string& path);
+  static std::vector<std::shared_ptr<SoundSource>> ImportSources(const std::string& path, bool cell_units = false);
   int x() const { return x_; }
   int y() const { return y_; }
   int z() const { return z_; }
@@ -139,7 +141,7 @@ class Recorder {
   // response stream (the 10^3 cube of out_<id>.txt is not written)
   void RecordField(int time_step = 0);
   void OpenResponse(const std::string& path);
-  static std::vector<std::shared_ptr<Recorder>> ImportRecorders(const std::string& path);
+  static std::vector<std::shared_ptr<Recorder>> ImportRecorders(const std::string& path, bool cell_units = false);
   int x() const { return x_; }
   int y() const { return y_; }
   int z() const { return z_; }
@@ -164,6 +166,7 @@ class Simulation {
   static int m_form;           // ARD_FORM_PLAIN or ARD_FORM_BATCHED
   static int m_recorder_mode;  // ARD_REC_REFERENCE or ARD_REC_LOCAL
   static int m_device;
+  static int m_autotune;       // batched form: time the tiling candidates at construction
 
   // ARD/simulation.cpp:14-294. `partitions` receives the PML slabs, like the
   // reference's member list. Recorders given here are sampled on the device

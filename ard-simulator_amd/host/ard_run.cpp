@@ -2,47 +2,112 @@
 // (ARD/main.cpp:47-193) without the SDL window: imports a scene, builds the
 // Simulation, steps it and writes response_<id>.txt for each recorder.
 //
-//   ard_run <scene.txt> <sources.txt> [recorders.txt] [steps] [h] [out_dir]
+//   ard_run [options] <scene.txt> <sources.txt> [recorders.txt] [steps] [h] [out_dir]
+//
+// options:
+//   --cells       scene, source and recorder files hold cell indices, not metres
+//   --rec-local   recorder coordinates are local to their partition (default:
+//                 the reference's addressing, global coordinates into the
+//                 partition's array)
+//   --plain       per-partition form (default: batched)
+//   --autotune    batched form: time the tiling candidates before stepping
+//   --timing      launch each phase directly and print per-phase kernel times
+//   --no-output   step without writing response files (no per-step device sync)
+//   --copy-mib N  first run one device-to-device copy of N MiB: a kernel of known
+//                 traffic, the scale for hardware byte counters
+#include <hip/hip_runtime_api.h>
+
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
 #include <iostream>
+#include <string>
+#include <vector>
 
 #include "ard_host.h"
 
 using namespace ard::host;
 
 int main(int argc, char** argv) {
-  if (argc < 3) {
-    std::fprintf(stderr, "usage: %s scene.txt sources.txt [recorders.txt] [steps] [h] [out_dir]\n", argv[0]);
+  bool cells = false, timing = false, output = true;
+  long long copy_mib = 0;
+  std::vector<const char*> pos;
+  for (int i = 1; i < argc; ++i) {
+    if (!std::strcmp(argv[i], "--cells")) cells = true;
+    else if (!std::strcmp(argv[i], "--rec-local")) Simulation::m_recorder_mode = ARD_REC_LOCAL;
+    else if (!std::strcmp(argv[i], "--plain")) Simulation::m_form = ARD_FORM_PLAIN;
+    else if (!std::strcmp(argv[i], "--autotune")) Simulation::m_autotune = 1;
+    else if (!std::strcmp(argv[i], "--timing")) timing = true;
+    else if (!std::strcmp(argv[i], "--no-output")) output = false;
+    else if (!std::strcmp(argv[i], "--copy-mib") && i + 1 < argc) copy_mib = std::atoll(argv[++i]);
+    else if (argv[i][0] == '-' && argv[i][1] == '-') {
+      std::fprintf(stderr, "unknown option %s\n", argv[i]);
+      return 2;
+    } else {
+      pos.push_back(argv[i]);
+    }
+  }
+  if (pos.size() < 2) {
+    std::fprintf(stderr,
+                 "usage: %s [--cells] [--rec-local] [--plain] [--autotune] [--timing] [--no-output] [--copy-mib N] "
+                 "scene.txt sources.txt [recorders.txt] [steps] [h] [out_dir]\n",
+                 argv[0]);
     return 2;
   }
-  if (argc > 5) {
-    Simulation::m_dh = float(std::atof(argv[5]));
+  if (pos.size() > 4) {
+    Simulation::m_dh = float(std::atof(pos[4]));
     Simulation::m_dt = Simulation::m_dh < 0.075f ? 0.625e-4f : (Simulation::m_dh < 0.15f ? 1.0f / 8000.0f : 2e-4f);
   }
-  const std::string out_dir = argc > 6 ? argv[6] : ".";
+  const std::string out_dir = pos.size() > 5 ? pos[5] : ".";
+  if (copy_mib > 0) {
+    const size_t n = size_t(copy_mib) << 20;
+    void *a = nullptr, *b = nullptr;
+    if (hipSetDevice(Simulation::m_device) != hipSuccess || hipMalloc(&a, n) != hipSuccess ||
+        hipMalloc(&b, n) != hipSuccess || hipMemset(a, 1, n) != hipSuccess ||
+        hipMemcpy(b, a, n, hipMemcpyDeviceToDevice) != hipSuccess || hipDeviceSynchronize() != hipSuccess) {
+      std::fprintf(stderr, "calibration copy failed\n");
+      return 1;
+    }
+    hipFree(a);
+    hipFree(b);
+    std::printf("copied %lld MiB device to device\n", copy_mib);
+  }
   try {
     auto t0 = std::chrono::steady_clock::now();
-    auto partitions = Partition::ImportPartitions(argv[1]);
-    auto sources = SoundSource::ImportSources(argv[2]);
+    auto partitions = Partition::ImportPartitions(pos[0], cells);
+    auto sources = SoundSource::ImportSources(pos[1], cells);
     std::vector<std::shared_ptr<Recorder>> recorders;
-    if (argc > 3) recorders = Recorder::ImportRecorders(argv[3]);
+    if (pos.size() > 2) recorders = Recorder::ImportRecorders(pos[2], cells);
     for (auto& r : recorders) r->FindPartition(partitions);
     Simulation sim(partitions, sources, recorders);
-    for (size_t i = 0; i < recorders.size(); ++i)
-      recorders[i]->OpenResponse(out_dir + "/response_" + std::to_string(i) + ".txt");
-    const int steps = argc > 4 ? std::atoi(argv[4]) : int(Simulation::m_duration / Simulation::m_dt);
+    if (output)
+      for (size_t i = 0; i < recorders.size(); ++i)
+        recorders[i]->OpenResponse(out_dir + "/response_" + std::to_string(i) + ".txt");
+    const int steps = pos.size() > 3 ? std::atoi(pos[3]) : int(Simulation::m_duration / Simulation::m_dt);
+    if (timing && ard_set_kernel_timing(sim.context(), 1) != ARD_OK) throw Error(ard_last_error());
     auto t1 = std::chrono::steady_clock::now();
     std::cout << "Initialization finished. (" << std::chrono::duration<double>(t1 - t0).count() << " s)" << std::endl;
     sim.Info();
     for (int s = 0; s < steps; ++s) {
       const int t = sim.Update();
-      for (auto& r : recorders) r->RecordField(t);
+      if (output)
+        for (auto& r : recorders) r->RecordField(t);
     }
+    if (ard_synchronize(sim.context()) != ARD_OK) throw Error(ard_last_error());
     auto t2 = std::chrono::steady_clock::now();
-    std::cout << "Simulation finished. (" << std::chrono::duration<double>(t2 - t1).count() << " s, " << steps
-              << " steps)" << std::endl;
+    const double secs = std::chrono::duration<double>(t2 - t1).count();
+    std::cout << "Simulation finished. (" << secs << " s, " << steps << " steps)" << std::endl;
+    if (timing) {
+      static const char* names[ARD_NUM_PHASES] = {"sources", "pml", "fold", "face_fwd1", "face_fwd2", "modal", "unfold",
+                                              "face_inv1", "face_inv2", "interface", "record", "dct_fwd", "dct_inv"};
+      float ms[ARD_NUM_PHASES] = {};
+      int launches[ARD_NUM_PHASES] = {};
+      if (ard_kernel_times(sim.context(), ms, launches, ARD_NUM_PHASES) != ARD_OK) throw Error(ard_last_error());
+      for (int p = 0; p < ARD_NUM_PHASES; ++p)
+        if (launches[p])
+          std::printf("phase %-10s %8.4f ms per launch, %d launches\n", names[p], ms[p], launches[p]);
+    }
   } catch (const Error& e) {
     std::fprintf(stderr, "error: %s\n", e.what());
     return 1;

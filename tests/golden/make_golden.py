@@ -21,6 +21,8 @@ OUT = os.path.dirname(os.path.abspath(__file__))
 
 
 def main():
+    # the seeded hall in cell units (ard_run --cells)
+    S.write_cell_files(S.hall(), os.path.join(OUT, "hall_cells"))
     for sc in (S.two_rooms(), S.scene3()):
         topo = None
         for double in (False, True):

@@ -76,3 +76,13 @@ def test_bad_arguments_rejected():
         A._check(sim.L.ard_add_dct_partition(sim.ctx, 0, 0, 0, 30, 30, 30))  # topology already built
     with pytest.raises(A.ArdError):
         A._check(sim.L.ard_step(sim.ctx, 1))  # not finalized
+
+
+def test_hall_cell_fixture_matches_generator():
+    """The committed hall scene files (cell units) are the seeded generator's output."""
+    sc = S.hall()
+    boxes, srcs, recs = S.read_cell_files(os.path.join(os.path.dirname(__file__), "golden", "hall_cells"))
+    assert boxes == [tuple(b) for b in sc.boxes]
+    assert srcs == [(s.x, s.y, s.z) for s in sc.sources]
+    assert recs == [tuple(r) for r in sc.recorders]
+    assert sum(b[3] * b[4] * b[5] for b in boxes) == 74_000_000
