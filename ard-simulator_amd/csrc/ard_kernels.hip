@@ -17,12 +17,13 @@ namespace ard {
 
 // ---------------------------------------------------------------------------
 // Grouped strided GEMM: 64x64 output tile per 256-thread block, K in chunks
-// of 16 staged in LDS, 4x4 outputs per thread.
+// of 16 staged in LDS (two buffers; the next chunk is loaded into registers
+// during the current one), 4x4 outputs per thread.
 // ---------------------------------------------------------------------------
 template <bool B_K_CONTIG, bool C_M_CONTIG, int EP>
 __global__ __launch_bounds__(256) void k_gemm(const GemmDesc* __restrict__ descs, int ndesc) {
-  __shared__ float As[16][68];
-  __shared__ float Bs[16][68];
+  __shared__ float As[2][16][68];
+  __shared__ float Bs[2][16][68];
   const int tile = blockIdx.x;
   int lo = 0, hi = ndesc - 1;
   while (lo < hi) {
@@ -63,37 +64,48 @@ __global__ __launch_bounds__(256) void k_gemm(const GemmDesc* __restrict__ descs
 
   const gfloat* __restrict__ A = ARD_CG(g.A);
   const gfloat* __restrict__ B = ARD_CG(g.B);
-  for (int k0 = 0; k0 < K; k0 += 16) {
-    // A tile: As[k][m] = A[m0+m][k0+k]
+  // the next K chunk is loaded into registers while the current one is
+  // multiplied from LDS (two LDS buffers, one barrier per chunk)
+  float ra[4], rb[4];
+  auto gload = [&](const int k0) {
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       const int k = tid & 15, m = (tid >> 4) + 16 * r;
-      const int gm = m0 + m, gk = k0 + k;
-      const float v = A[(long long)min(gm, M - 1) * K + min(gk, K - 1)];  // clamped: loads stay unconditional
-      As[k][m] = (gm < M && gk < K) ? v : 0.f;
+      ra[r] = A[(long long)min(m0 + m, M - 1) * K + min(k0 + k, K - 1)];  // clamped: loads stay unconditional
     }
-    if (B_K_CONTIG) {
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
+    for (int r = 0; r < 4; ++r) {
+      const int gk = B_K_CONTIG ? k0 + (tid & 15) : k0 + (tid >> 6) + 4 * r;
+      rb[r] = B[(long long)min(gk, K - 1) * g.bK + boff[B_K_CONTIG ? r : 0]];
+    }
+  };
+  auto sstore = [&](const int k0, const int buf) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int k = tid & 15, m = (tid >> 4) + 16 * r;
+      As[buf][k][m] = (m0 + m < M && k0 + k < K) ? ra[r] : 0.f;
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      if (B_K_CONTIG) {
         const int k = tid & 15, n = (tid >> 4) + 16 * r;
-        const int gk = k0 + k;
-        const float v = B[(long long)min(gk, K - 1) * g.bK + boff[r]];
-        Bs[k][n] = (bn[r] < N && gk < K) ? v : 0.f;
-      }
-    } else {
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
+        Bs[buf][k][n] = (bn[r] < N && k0 + k < K) ? rb[r] : 0.f;
+      } else {
         const int k = (tid >> 6) + 4 * r, n = tid & 63;
-        const int gk = k0 + k;
-        const float v = B[(long long)min(gk, K - 1) * g.bK + boff[0]];
-        Bs[k][n] = (bn[0] < N && gk < K) ? v : 0.f;
+        Bs[buf][k][n] = (bn[0] < N && k0 + k < K) ? rb[r] : 0.f;
       }
     }
+  };
+  gload(0);
+  int buf = 0;
+  for (int k0 = 0; k0 < K; k0 += 16, buf ^= 1) {
+    sstore(k0, buf);
     __syncthreads();
+    if (k0 + 16 < K) gload(k0 + 16);
 #pragma unroll
     for (int kk = 0; kk < 16; ++kk) {
-      const float4 a = *reinterpret_cast<const float4*>(&As[kk][4 * tm]);
-      const float4 b = *reinterpret_cast<const float4*>(&Bs[kk][4 * tn]);
+      const float4 a = *reinterpret_cast<const float4*>(&As[buf][kk][4 * tm]);
+      const float4 b = *reinterpret_cast<const float4*>(&Bs[buf][kk][4 * tn]);
       const float av[4] = {a.x, a.y, a.z, a.w};
       const float bv[4] = {b.x, b.y, b.z, b.w};
 #pragma unroll
@@ -101,7 +113,6 @@ __global__ __launch_bounds__(256) void k_gemm(const GemmDesc* __restrict__ descs
 #pragma unroll
         for (int j = 0; j < 4; ++j) acc[i][j] = fmaf(av[i], bv[j], acc[i][j]);
     }
-    __syncthreads();
   }
 
   long long coff[4];
@@ -143,8 +154,8 @@ typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 template <bool B_K_CONTIG, int EP>
 __global__ __launch_bounds__(256) void k_gemm_mfma(const GemmDesc* __restrict__ descs, int ndesc) {
-  __shared__ float As[16][68];
-  __shared__ float Bs[16][68];
+  __shared__ float As[2][16][68];
+  __shared__ float Bs[2][16][68];
   const int tile = blockIdx.x;
   int lo = 0, hi = ndesc - 1;
   while (lo < hi) {
@@ -182,39 +193,48 @@ __global__ __launch_bounds__(256) void k_gemm_mfma(const GemmDesc* __restrict__ 
 
   const gfloat* __restrict__ A = ARD_CG(g.A);
   const gfloat* __restrict__ B = ARD_CG(g.B);
-  for (int k0 = 0; k0 < K; k0 += 16) {
+  float ra[4], rb[4];
+  auto gload = [&](const int k0) {
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       const int k = tid & 15, m = (tid >> 4) + 16 * r;
-      const int gm = m0 + m, gk = k0 + k;
-      const float v = A[(long long)min(gm, M - 1) * K + min(gk, K - 1)];  // clamped: loads stay unconditional
-      As[k][m] = (gm < M && gk < K) ? v : 0.f;
+      ra[r] = A[(long long)min(m0 + m, M - 1) * K + min(k0 + k, K - 1)];  // clamped: loads stay unconditional
     }
-    if (B_K_CONTIG) {
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
+    for (int r = 0; r < 4; ++r) {
+      const int gk = B_K_CONTIG ? k0 + (tid & 15) : k0 + (tid >> 6) + 4 * r;
+      rb[r] = B[(long long)min(gk, K - 1) * g.bK + boff[B_K_CONTIG ? r : 0]];
+    }
+  };
+  auto sstore = [&](const int k0, const int buf) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int k = tid & 15, m = (tid >> 4) + 16 * r;
+      As[buf][k][m] = (m0 + m < M && k0 + k < K) ? ra[r] : 0.f;
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      if (B_K_CONTIG) {
         const int k = tid & 15, n = (tid >> 4) + 16 * r;
-        const int gk = k0 + k;
-        const float v = B[(long long)min(gk, K - 1) * g.bK + boff[r]];
-        Bs[k][n] = (bn[r] < N && gk < K) ? v : 0.f;
-      }
-    } else {
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
+        Bs[buf][k][n] = (bn[r] < N && k0 + k < K) ? rb[r] : 0.f;
+      } else {
         const int k = (tid >> 6) + 4 * r, n = tid & 63;
-        const int gk = k0 + k;
-        const float v = B[(long long)min(gk, K - 1) * g.bK + boff[0]];
-        Bs[k][n] = (bn[0] < N && gk < K) ? v : 0.f;
+        Bs[buf][k][n] = (bn[0] < N && k0 + k < K) ? rb[r] : 0.f;
       }
     }
+  };
+  gload(0);
+  int buf = 0;
+  for (int k0 = 0; k0 < K; k0 += 16, buf ^= 1) {
+    sstore(k0, buf);
     __syncthreads();
+    if (k0 + 16 < K) gload(k0 + 16);
 #pragma unroll
     for (int kk = 0; kk < 16; kk += 2) {
-      const float a = As[kk + (lane >> 5)][wm + (lane & 31)];
-      const float b = Bs[kk + (lane >> 5)][wn + (lane & 31)];
+      const float a = As[buf][kk + (lane >> 5)][wm + (lane & 31)];
+      const float b = Bs[buf][kk + (lane >> 5)][wn + (lane & 31)];
       acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, acc, 0, 0, 0);
     }
-    __syncthreads();
   }
   const int j = n0 + wn + (lane & 31);
   if (j >= N) return;
