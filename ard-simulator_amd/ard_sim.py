@@ -27,6 +27,7 @@ EXPORTED = [
     "ard_boundary_info", "ard_recorder_partition", "ard_step", "ard_time_step", "ard_fetch_recorders",
     "ard_copy_pressure_field", "ard_get_pressure", "ard_set_force", "ard_synchronize", "ard_set_kernel_timing",
     "ard_kernel_times", "ard_selftest_transform", "ard_set_transform_variant", "ard_batched_config",
+    "ard_adjoint",
 ]
 PHASES = ["sources", "pml", "fold", "face_fwd1", "face_fwd2", "modal", "unfold", "face_inv1", "face_inv2",
           "interface", "record", "dct_fwd", "dct_inv"]
@@ -83,6 +84,7 @@ def lib():
         L.ard_selftest_transform.argtypes = [i, i, i, i, i, i, fp, fp, fp]
         L.ard_set_transform_variant.argtypes = [i]
         L.ard_batched_config.argtypes = [vp, ip]
+        L.ard_adjoint.argtypes = [vp, i, fp, fp, fp, i]
         _lib = L
     return _lib
 
@@ -220,6 +222,21 @@ class Simulation:
         cnt = np.zeros(n, dtype=np.int32)
         _check(self.L.ard_kernel_times(self.ctx, _fp(ms), cnt.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), n))
         return {PHASES[i]: (float(ms[i]), int(cnt[i])) for i in range(n) if cnt[i]}
+
+    def adjoint(self, dl_drec, checkpoint_stride=0):
+        """Gradients of a recorder loss by the discrete adjoint (plain form).
+
+        dl_drec: [nsteps][nrec] array of dL/dy. Re-runs nsteps steps from the
+        zero state with checkpoints, then the reverse sweep. Returns
+        (dL/dwave [nsources][nsteps] float32, dL/dscale [6] float32)."""
+        g = np.ascontiguousarray(dl_drec, dtype=np.float32)
+        nsteps = g.shape[0]
+        if g.ndim != 2 or g.shape[1] != self.nrec:
+            raise ValueError("dl_drec must be [nsteps][%d]" % self.nrec)
+        dw = np.zeros((len(self.scene.sources), nsteps), dtype=np.float32)
+        ds = np.zeros(6, dtype=np.float32)
+        _check(self.L.ard_adjoint(self.ctx, nsteps, _fp(g), _fp(dw), _fp(ds), int(checkpoint_stride)))
+        return dw, ds
 
     def batched_config(self):
         """(rows per wave, kz planes per block, transform variant, candidates timed)."""

@@ -11,7 +11,8 @@ CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libard.so")
 OBJ = os.path.join(HERE, "_build")
 
-HIP_SOURCES = ["ard_kernels.hip", "ard_batched_kernels.hip", "ard_batched.hip", "ard_ctx.cpp", "ard_plain.cpp"]
+HIP_SOURCES = ["ard_kernels.hip", "ard_batched_kernels.hip", "ard_batched.hip", "ard_ctx.cpp", "ard_plain.cpp",
+               "ard_adjoint_kernels.hip", "ard_adjoint.cpp"]
 CXX_SOURCES = ["ard_topology.cpp"]
 HOST_SOURCES = ["host/ard_host.cpp"]
 

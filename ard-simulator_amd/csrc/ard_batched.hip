@@ -26,26 +26,6 @@ namespace {
 long long ceil_div(long long a, long long b) { return (a + b - 1) / b; }
 const double kPi = 3.14159265358979323846;
 
-// Host twin of shell_index (ard_kernels.hip): which face slab stores cell
-// (x,y,z) of a W x H x D partition, and where.
-int shell_of(int W, int H, int D, int x, int y, int z, long long* off) {
-  if (x < 3 || x >= W - 3) {
-    const int q = x < 3 ? x : x - W + 6;
-    *off = ((long long)q * D + z) * H + y;
-    return 0;
-  }
-  if (y < 3 || y >= H - 3) {
-    const int q = y < 3 ? y : y - H + 6;
-    *off = ((long long)q * D + z) * W + x;
-    return 1;
-  }
-  if (z < 3 || z >= D - 3) {
-    const int q = z < 3 ? z : z - D + 6;
-    *off = ((long long)q * H + y) * W + x;
-    return 2;
-  }
-  return -1;
-}
 }  // namespace
 
 int apply_tiling(Ctx& c, int rpw, int kzc);
@@ -335,68 +315,13 @@ int build_modal(Ctx& c) {
   return ARD_OK;
 }
 
-// Last-writer masks: replays the boundaries in list order over the written
-// cells (ARD/simulation.cpp:313-317 run serially) and keeps, for each face
-// element, the outputs that no later boundary overwrites.
+// Last-writer masks of the grouped interface launch (final_writer_masks),
+// uploaded with each boundary's element offset.
 int build_masks(Ctx& c, std::vector<BndDesc>& descs, const std::vector<long long>& elems) {
-  const int np = int(c.topo.parts.size());
-  std::vector<std::vector<int>> owner[3];
-  for (int w = 0; w < 3; ++w) owner[w].resize(np);
-  std::vector<std::vector<int>> powner(np);
-  for (int i = 0; i < np; ++i) {
-    const PartGeom& p = c.topo.parts[i];
-    if (p.is_pml) powner[i].assign(size_t(p.cells()), -1);
-    else {
-      owner[0][i].assign(size_t(6) * p.d * p.h, -1);
-      owner[1][i].assign(size_t(6) * p.d * p.w, -1);
-      owner[2][i].assign(size_t(6) * p.h * p.w, -1);
-    }
-  }
-  auto target = [&](const BndDesc& d, int side, int u, int v, int m, int** slot) {
-    const PartGeom& p = c.topo.parts[d.side[side]];
-    const int uu = u - d.org[side][0], vv = v - d.org[side][1];
-    const int n0 = d.type == 0 ? p.w : (d.type == 1 ? p.h : p.d);
-    const int nrm = side == 0 ? n0 + m : m;
-    int x, y, z;
-    if (d.type == 0) { x = nrm; y = uu; z = vv; }
-    else if (d.type == 1) { x = uu; y = nrm; z = vv; }
-    else { x = uu; y = vv; z = nrm; }
-    if (p.is_pml) {
-      *slot = &powner[d.side[side]][(size_t(z) * p.h + y) * p.w + x];
-    } else {
-      long long off;
-      const int w = shell_of(p.w, p.h, p.d, x, y, z, &off);
-      *slot = w >= 0 ? &owner[w][d.side[side]][size_t(off)] : nullptr;
-    }
-  };
-  for (int b = 0; b < int(descs.size()); ++b) {
-    const BndDesc& d = descs[b];
-    for (long long e = 0; e < elems[b]; ++e) {
-      const int u = d.t0s + int(e % d.t0n), v = d.t1s + int(e / d.t0n);  // k_interface's element order
-      for (int r = 0; r < 6; ++r) {
-        int* s;
-        target(d, r < 3 ? 0 : 1, u, v, r - 3, &s);
-        if (s) *s = b;
-      }
-    }
-  }
   BatchedState& B = *c.bs;
   std::vector<uint8_t> mask;
-  std::vector<long long> mstart(descs.size());
-  for (int b = 0; b < int(descs.size()); ++b) {
-    const BndDesc& d = descs[b];
-    mstart[b] = (long long)mask.size();
-    for (long long e = 0; e < elems[b]; ++e) {
-      const int u = d.t0s + int(e % d.t0n), v = d.t1s + int(e / d.t0n);
-      uint8_t m = 0;
-      for (int r = 0; r < 6; ++r) {
-        int* s;
-        target(d, r < 3 ? 0 : 1, u, v, r - 3, &s);
-        if (s && *s == b) m |= uint8_t(1u << r);
-      }
-      mask.push_back(m);
-    }
-  }
+  std::vector<long long> mstart;
+  final_writer_masks(c, descs, elems, mask, mstart);
   uint8_t* dm = nullptr;
   int r = upload_vec(c, (void**)&dm, mask.data(), mask.size());
   if (r) return r;

@@ -86,6 +86,7 @@ Ctx::~Ctx() {
   if (graph_exec[0]) hipGraphExecDestroy(graph_exec[0]);
   if (graph_exec[1]) hipGraphExecDestroy(graph_exec[1]);
   destroy_batched(*this);
+  destroy_adjoint(*this);
   for (void* p : allocs) hipFree(p);
   if (stream) hipStreamDestroy(stream);
 }

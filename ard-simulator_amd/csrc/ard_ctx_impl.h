@@ -14,7 +14,8 @@
 
 namespace ard {
 
-struct BatchedState;  // ard_batched.hip
+struct BatchedState;
+struct AdjState;  // ard_batched.hip
 
 // Dense fields of one DCT partition in the plain form (ARD/dct_partition.h,
 // ARD/dct_volume.h): force F and pressure p in the value domain, modes M[2]
@@ -94,6 +95,8 @@ struct Ctx {
 
   // batched form
   BatchedState* bs = nullptr;
+  // adjoint of the plain form (ard_adjoint.cpp), set up on first use
+  AdjState* adj = nullptr;
   hipGraphExec_t graph_exec[2] = {nullptr, nullptr};
 
   ~Ctx();
@@ -110,6 +113,11 @@ void modal_tables(int W, int H, int D, double h, double dt, double c0, std::vect
 // shared set-up (ard_plain.cpp)
 int setup_common(Ctx& c);               // step counter, PML slabs, sources/recorders tables
 int build_boundary_descs(Ctx& c, std::vector<BndDesc>& out, std::vector<long long>& elems);
+int shell_of(int W, int H, int D, int x, int y, int z, long long* off);
+std::vector<float> pml_profile(const PartGeom& p, int layers, double h, double c0, double scale);
+void destroy_adjoint(Ctx& c);
+void final_writer_masks(const Ctx& c, const std::vector<BndDesc>& descs, const std::vector<long long>& elems,
+                        std::vector<uint8_t>& mask, std::vector<long long>& mstart);
 int ensure_trace(Ctx& c, int steps_needed);
 
 // plain form (ard_plain.cpp)

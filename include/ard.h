@@ -167,6 +167,22 @@ int ard_set_transform_variant(int variant);
  * transform variant, number of candidates timed. No counterpart in the reference. */
 int ard_batched_config(ard_ctx* ctx, int* out4);
 
+/* Discrete adjoint of the plain form (no counterpart in the reference, which
+ * has no gradients; it transposes the step of ARD/simulation.cpp:300-317).
+ * Runs `nsteps` steps from the zero state with a checkpoint every
+ * checkpoint_stride steps (<= 0: ceil(sqrt(nsteps))), then the reverse sweep,
+ * and returns the gradients of a loss L(y) of the recorder traces given
+ * dl_drec[t][r] = dL/dy(t, r) (the layout of ard_fetch_recorders):
+ *   dl_dwave[s][t]  dL/d(force sample of source s injected at step t), for every
+ *                   source in registration order (zero rows for sources outside
+ *                   every DCT partition); may be NULL
+ *   dl_dscale[6]    dL/d(damping scale of each PML face, ard_set_pml_scale); may be NULL
+ * Plain form only (ARD_ERR_UNSUPPORTED otherwise). The context is left in its
+ * forward state after nsteps (time step nsteps, the traces of that run).
+ * Allocates its checkpoints for the call. */
+int ard_adjoint(ard_ctx* ctx, int nsteps, const float* dl_drec, float* dl_dwave, float* dl_dscale,
+                int checkpoint_stride);
+
 #ifdef __cplusplus
 }
 #endif
