@@ -248,6 +248,25 @@ def rooms64(seed=5, steps=1500, h=0.05, n_rooms=64):
                  room_of_box=room_of)
 
 
+def split_rooms(scene):
+    """The independent rooms of a scene with ``room_of_box`` as separate
+    scenes (boxes, sources and recorders by room), in room order."""
+    rooms = sorted(set(scene.room_of_box))
+    out = []
+    for r in rooms:
+        boxes = [b for b, q in zip(scene.boxes, scene.room_of_box) if q == r]
+
+        def inside(pt):
+            return any(b[0] <= pt[0] < b[0] + b[3] and b[1] <= pt[1] < b[1] + b[4] and b[2] <= pt[2] < b[2] + b[5]
+                       for b in boxes)
+
+        srcs = [s for s in scene.sources if inside((s.x, s.y, s.z))]
+        recs = [q for q in scene.recorders if inside(q)]
+        out.append(dataclasses.replace(scene, name="%s_room%d" % (scene.name, r), boxes=boxes, sources=srcs,
+                                       recorders=recs, room_of_box=None))
+    return out
+
+
 def write_cell_files(scene, directory):
     """Writes scene.txt / sources.txt / recorders.txt in the reference's
     whitespace format (ARD/partition.cpp:151-176, ARD/sound_source.cpp:24-48,

@@ -27,7 +27,8 @@ EXPORTED = [
     "ard_boundary_info", "ard_recorder_partition", "ard_step", "ard_time_step", "ard_fetch_recorders",
     "ard_copy_pressure_field", "ard_get_pressure", "ard_set_force", "ard_synchronize", "ard_set_kernel_timing",
     "ard_kernel_times", "ard_selftest_transform", "ard_set_transform_variant", "ard_batched_config",
-    "ard_adjoint",
+    "ard_adjoint", "ard_batch_create", "ard_batch_step", "ard_batch_fetch_recorders", "ard_batch_context",
+    "ard_batch_room_partition", "ard_batch_destroy",
 ]
 PHASES = ["sources", "pml", "fold", "face_fwd1", "face_fwd2", "modal", "unfold", "face_inv1", "face_inv2",
           "interface", "record", "dct_fwd", "dct_inv"]
@@ -85,6 +86,12 @@ def lib():
         L.ard_set_transform_variant.argtypes = [i]
         L.ard_batched_config.argtypes = [vp, ip]
         L.ard_adjoint.argtypes = [vp, i, fp, fp, fp, i]
+        L.ard_batch_create.argtypes = [ctypes.POINTER(vp), i, ctypes.POINTER(vp)]
+        L.ard_batch_step.argtypes = [vp, i]
+        L.ard_batch_fetch_recorders.argtypes = [vp, i, fp, i, i]
+        L.ard_batch_context.argtypes = [vp, ctypes.POINTER(vp)]
+        L.ard_batch_room_partition.argtypes = [vp, i, i]
+        L.ard_batch_destroy.argtypes = [vp]
         _lib = L
     return _lib
 
@@ -243,3 +250,63 @@ class Simulation:
         out = (ctypes.c_int * 4)()
         _check(self.L.ard_batched_config(self.ctx, out))
         return tuple(out)
+
+
+class Batch:
+    """A batch of independent rooms advanced together (ard_batch_create).
+
+    ``scenes``: ard_scenes.Scene objects sharing h, dt, c0, PML and form.
+    Each is registered in a member context (topology built, not finalized);
+    the batch owns one merged context and frees the members at once."""
+
+    def __init__(self, scenes, form=FORM_BATCHED, device=0, autotune=False):
+        self.L = lib()
+        self.members = [Simulation(sc, form=form, device=device, finalize=False, autotune=autotune)
+                        for sc in scenes]
+        arr = (ctypes.c_void_p * len(scenes))(*[m.ctx for m in self.members])
+        h = ctypes.c_void_p()
+        _check(self.L.ard_batch_create(arr, len(scenes), ctypes.byref(h)))
+        self.batch = h
+        self.nrec = [len(sc.recorders) for sc in scenes]
+        for m in self.members:
+            m.close()
+        self.members = []
+        c = ctypes.c_void_p()
+        _check(self.L.ard_batch_context(self.batch, ctypes.byref(c)))
+        self.ctx = c
+
+    def close(self):
+        if getattr(self, "batch", None):
+            self.L.ard_batch_destroy(self.batch)
+            self.batch = None
+
+    def __del__(self):
+        self.close()
+
+    def update(self, n=1):
+        _check(self.L.ard_batch_step(self.batch, n))
+
+    def synchronize(self):
+        _check(self.L.ard_synchronize(self.ctx))
+
+    def recorders(self, room, t0, nt):
+        out = np.zeros((nt, max(self.nrec[room], 1)), dtype=np.float32)
+        _check(self.L.ard_batch_fetch_recorders(self.batch, room, _fp(out), t0, nt))
+        return out[:, : self.nrec[room]]
+
+    def partitions(self):
+        out, buf = [], (ctypes.c_int * 9)()
+        for i in range(_check(self.L.ard_num_partitions(self.ctx))):
+            _check(self.L.ard_partition_info(self.ctx, i, buf))
+            out.append(tuple(buf))
+        return out
+
+    def boundaries(self):
+        out, buf = [], (ctypes.c_int * 9)()
+        for i in range(_check(self.L.ard_num_boundaries(self.ctx))):
+            _check(self.L.ard_boundary_info(self.ctx, i, buf))
+            out.append(tuple(buf))
+        return out
+
+    def room_partition(self, room, part):
+        return _check(self.L.ard_batch_room_partition(self.batch, room, part))

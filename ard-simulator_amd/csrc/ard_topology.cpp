@@ -150,6 +150,7 @@ std::string Topology::resolve(bool recorder_local) {
   }
   for (size_t r = 0; r < recorders.size(); ++r) {
     RecorderDef& rc = recorders[r];
+    if (rc.preset) continue;
     rc.part = -1;
     for (int i = 0; i < int(parts.size()); ++i) {
       const PartGeom& p = parts[i];

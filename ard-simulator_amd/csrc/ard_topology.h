@@ -46,6 +46,7 @@ struct RecorderDef {
   int x = 0, y = 0, z = 0;  // global cells
   int part = -1;            // owning DCT partition
   int lx = 0, ly = 0, lz = 0;  // partition-local cell actually sampled
+  bool preset = false;      // part and local cell given (a batch's copy of a resolved recorder)
 };
 
 struct Topology {

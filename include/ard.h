@@ -183,6 +183,27 @@ int ard_batched_config(ard_ctx* ctx, int* out4);
 int ard_adjoint(ard_ctx* ctx, int nsteps, const float* dl_drec, float* dl_dwave, float* dl_dscale,
                 int checkpoint_stride);
 
+/* A batch of independent rooms advanced together (no counterpart in the
+ * reference, whose Simulation holds one scene, ARD/simulation.cpp:14-294).
+ * Each member is a context whose partitions, sources and recorders are
+ * registered and whose topology is built (ard_build_topology), not finalized;
+ * all members share device, h, dt, c0, PML layers and scales, absorption and
+ * form. The batch lays the rooms side by side in one context of its own (far
+ * enough apart that no two touch, so every room keeps its own topology and
+ * recorder addressing) and finalizes it: one step graph per step for all
+ * rooms in the batched form. Members are left unchanged and stay owned by
+ * the caller. A batch of host-only members is built but not finalized. */
+typedef struct ard_batch ard_batch;
+int ard_batch_create(ard_ctx* const* members, int n, ard_batch** out);
+int ard_batch_step(ard_batch* batch, int nsteps);
+/* Room `room`'s recorders, [nt][its recorder count], steps t0 .. t0+nt-1. */
+int ard_batch_fetch_recorders(ard_batch* batch, int room, float* out, int t0, int nt);
+/* The batch's own context (partition info, kernel timing, fields); owned by the batch. */
+int ard_batch_context(ard_batch* batch, ard_ctx** out);
+/* Id in the batch's context of the room's DCT partition `part` (the room's own id). */
+int ard_batch_room_partition(ard_batch* batch, int room, int part);
+int ard_batch_destroy(ard_batch* batch);
+
 #ifdef __cplusplus
 }
 #endif
