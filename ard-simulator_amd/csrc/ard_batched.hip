@@ -384,6 +384,7 @@ int finalize_batched(Ctx& c) {
   if ((r = setup_common(c))) return r;
   c.bs = new BatchedState();
   BatchedState& B = *c.bs;
+  B.gemm_variant = g_gemm_variant;
   {
     hipError_t e = hipStreamCreateWithFlags(&B.side, hipStreamNonBlocking);  // default priority
     if (e == hipSuccess) e = hipEventCreateWithFlags(&B.ev_fork, hipEventDisableTiming);
@@ -667,8 +668,11 @@ int autotune(Ctx& c) {
     }
     if (r) break;
   }
+  // the FP32 transform kernels compete on speed; BF16 split precision, when
+  // requested, is kept (a precision choice, not a speed one)
   float bestg = 1e30f;
   for (int v : {0, 1}) {
+    if (B.gemm_variant == 2) break;
     const float ms = time_ms([&] {
       for (int k = 0; k < 4; ++k)
         launch_gemm(B.d_gemm[k], B.ngemm[k], B.gemm_tiles[k], k % 2 == 0, k % 2 == 0, 0, c.stream, v);

@@ -470,7 +470,7 @@ int ard_batched_config(ard_ctx* ctx, int* out4) {
 }
 
 int ard_set_transform_variant(int variant) {
-  if (variant < 0 || variant > 1) return fail(ARD_ERR_INVALID, "unknown transform variant");
+  if (variant < 0 || variant > 2) return fail(ARD_ERR_INVALID, "unknown transform variant");
   g_gemm_variant = variant;
   return ARD_OK;
 }
@@ -478,7 +478,7 @@ int ard_set_transform_variant(int variant) {
 int ard_selftest_transform(int device, int variant, int b_layout, int M, int N, int K, const float* A, const float* B,
                            float* C) {
   if (!A || !B || !C || M <= 0 || N <= 0 || K <= 0) return fail(ARD_ERR_INVALID, "bad arguments");
-  if (variant < 0 || variant > 1 || b_layout < 0 || b_layout > 1) return fail(ARD_ERR_INVALID, "bad variant");
+  if (variant < 0 || variant > 2 || b_layout < 0 || b_layout > 1) return fail(ARD_ERR_INVALID, "bad variant");
   hipError_t e = hipSetDevice(device);
   if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
   float *dA = nullptr, *dB = nullptr, *dC = nullptr;

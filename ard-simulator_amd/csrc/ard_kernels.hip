@@ -251,7 +251,135 @@ __global__ __launch_bounds__(256) void k_gemm_mfma(const GemmDesc* __restrict__ 
   }
 }
 
-// variant: 0 vector FMA, 1 matrix cores (FP32)
+// ---------------------------------------------------------------------------
+// The same grouped GEMM in BF16 split precision on the matrix cores: every
+// FP32 operand is split into x = hi + lo (hi = bf16(x), lo = bf16(x - hi)) and
+//   A B ~ A_hi B_hi + A_hi B_lo + A_lo B_hi
+// with v_mfma_f32_32x32x8_bf16 and FP32 accumulation (relative error of a
+// product about 2^-16). Operand layout of the instruction: lane l holds
+// A[l % 32][4 (l / 32) + i] and B[4 (l / 32) + i][l % 32], i < 4; the
+// accumulator layout is that of v_mfma_f32_32x32x2_f32.
+// ---------------------------------------------------------------------------
+typedef short s16x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ short bf16_rn(float x) {  // round to nearest even (finite inputs)
+  unsigned u = __float_as_uint(x);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return short(u >> 16);
+}
+__device__ __forceinline__ float bf16_f(short h) { return __uint_as_float(unsigned(static_cast<unsigned short>(h)) << 16); }
+
+template <bool B_K_CONTIG, int EP>
+__global__ __launch_bounds__(256) void k_gemm_bf16x3(const GemmDesc* __restrict__ descs, int ndesc) {
+  __shared__ float As[2][16][68];
+  __shared__ float Bs[2][16][68];
+  const int tile = blockIdx.x;
+  int lo = 0, hi = ndesc - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (descs[mid].tile0 <= tile) lo = mid;
+    else hi = mid - 1;
+  }
+  const GemmDesc& g = descs[lo];
+  const int M = g.M, N = g.N, K = g.K, N1 = g.N1;
+  const int t = tile - g.tile0;
+  const int m0 = (t / g.tiles_n) * 64;
+  const int n0 = (t % g.tiles_n) * 64;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+
+  long long boff[4];
+  int bn[4];
+  if (B_K_CONTIG) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int n = n0 + tid / 16 + 16 * r;
+      bn[r] = n;
+      boff[r] = n < N ? (long long)(n % N1) * g.b1 + (long long)(n / N1) * g.b2 : 0;
+    }
+  } else {
+    const int n = n0 + (tid & 63);
+    bn[0] = n;
+    boff[0] = n < N ? (long long)(n % N1) * g.b1 + (long long)(n / N1) * g.b2 : 0;
+  }
+  f32x16 acc;
+#pragma unroll
+  for (int r = 0; r < 16; ++r) acc[r] = 0.f;
+
+  const gfloat* __restrict__ A = ARD_CG(g.A);
+  const gfloat* __restrict__ B = ARD_CG(g.B);
+  float ra[4], rb[4];
+  auto gload = [&](const int k0) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int k = tid & 15, m = (tid >> 4) + 16 * r;
+      ra[r] = A[(long long)min(m0 + m, M - 1) * K + min(k0 + k, K - 1)];  // clamped: loads stay unconditional
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int gk = B_K_CONTIG ? k0 + (tid & 15) : k0 + (tid >> 6) + 4 * r;
+      rb[r] = B[(long long)min(gk, K - 1) * g.bK + boff[B_K_CONTIG ? r : 0]];
+    }
+  };
+  auto sstore = [&](const int k0, const int buf) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int k = tid & 15, m = (tid >> 4) + 16 * r;
+      As[buf][k][m] = (m0 + m < M && k0 + k < K) ? ra[r] : 0.f;
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      if (B_K_CONTIG) {
+        const int k = tid & 15, n = (tid >> 4) + 16 * r;
+        Bs[buf][k][n] = (bn[r] < N && k0 + k < K) ? rb[r] : 0.f;
+      } else {
+        const int k = (tid >> 6) + 4 * r, n = tid & 63;
+        Bs[buf][k][n] = (bn[0] < N && k0 + k < K) ? rb[r] : 0.f;
+      }
+    }
+  };
+  gload(0);
+  int buf = 0;
+  for (int k0 = 0; k0 < K; k0 += 16, buf ^= 1) {
+    sstore(k0, buf);
+    __syncthreads();
+    if (k0 + 16 < K) gload(k0 + 16);
+#pragma unroll
+    for (int ks = 0; ks < 16; ks += 8) {
+      const int kb = ks + 4 * (lane >> 5);
+      s16x4 ah, al, bh, bl;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const float a = As[buf][kb + i][wm + (lane & 31)];
+        const float b = Bs[buf][kb + i][wn + (lane & 31)];
+        ah[i] = bf16_rn(a);
+        al[i] = bf16_rn(a - bf16_f(ah[i]));
+        bh[i] = bf16_rn(b);
+        bl[i] = bf16_rn(b - bf16_f(bh[i]));
+      }
+      acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(al, bh, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(ah, bl, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(ah, bh, acc, 0, 0, 0);
+    }
+  }
+  const int j = n0 + wn + (lane & 31);
+  if (j >= N) return;
+  const long long cn = (long long)(j % N1) * g.c1 + (long long)(j / N1) * g.c2;
+  gfloat* __restrict__ C = ARD_G(g.C);
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    const int i = m0 + wm + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+    if (i >= M) continue;
+    const long long o = (long long)i * g.cM + cn;
+    const float v = g.alpha * acc[r];
+    if (EP == 0) C[o] = v;
+    else C[o] = ARD_CG(g.ep_a)[o] * ARD_CG(g.ep_cur)[o] - g.ep_b * C[o] + ARD_CG(g.ep_g)[o] * v;
+  }
+}
+
+// variant: 0 vector FMA, 1 matrix cores (FP32), 2 matrix cores (BF16 split, FP32 accumulate)
 int g_gemm_variant = 1;
 
 int launch_gemm(const GemmDesc* d_descs, int ndesc, int total_tiles, bool b_k_contig, bool c_m_contig, int ep,
@@ -259,6 +387,13 @@ int launch_gemm(const GemmDesc* d_descs, int ndesc, int total_tiles, bool b_k_co
   if (total_tiles <= 0) return 0;
   dim3 grid(total_tiles), block(256);
   if (variant < 0) variant = g_gemm_variant;
+  if (variant == 2) {
+    if (b_k_contig && ep == 0) hipLaunchKernelGGL((k_gemm_bf16x3<true, 0>), grid, block, 0, s, d_descs, ndesc);
+    else if (!b_k_contig && ep == 0) hipLaunchKernelGGL((k_gemm_bf16x3<false, 0>), grid, block, 0, s, d_descs, ndesc);
+    else if (!b_k_contig && ep == 1) hipLaunchKernelGGL((k_gemm_bf16x3<false, 1>), grid, block, 0, s, d_descs, ndesc);
+    else return -1;
+    return hipGetLastError() == hipSuccess ? 0 : -2;
+  }
   if (variant == 1) {
     if (b_k_contig && ep == 0) hipLaunchKernelGGL((k_gemm_mfma<true, 0>), grid, block, 0, s, d_descs, ndesc);
     else if (!b_k_contig && ep == 0) hipLaunchKernelGGL((k_gemm_mfma<false, 0>), grid, block, 0, s, d_descs, ndesc);

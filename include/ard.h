@@ -156,11 +156,15 @@ int ard_kernel_times(ard_ctx* ctx, float* avg_ms, int* launches, int nphase);
 /* Self-test of the dense transform kernels (no counterpart in the reference):
  * C[M][N] = A[M][K] * B on device `device`, where B is K x N row-major
  * (b_layout 0) or N x K row-major (b_layout 1). variant: 0 vector FMA,
- * 1 matrix cores FP32. Synchronous; allocates and frees its own buffers. */
+ * 1 matrix cores FP32, 2 matrix cores BF16 split precision (each operand
+ * x = hi + lo in BF16, three products, FP32 accumulate). Synchronous;
+ * allocates and frees its own buffers. */
 int ard_selftest_transform(int device, int variant, int b_layout, int M, int N, int K, const float* A, const float* B,
                            float* C);
 /* Selects the dense transform kernels used by contexts finalized afterwards
- * (0 vector FMA, 1 matrix cores FP32). Default 1. */
+ * (0 vector FMA, 1 matrix cores FP32, 2 matrix cores BF16 split). Default 1.
+ * The batched form's autotuner may switch between 0 and 1 (same precision);
+ * it keeps 2. */
 int ard_set_transform_variant(int variant);
 /* Tiling of the batched form after ard_finalize (chosen by the autotuner when
  * ard_params.autotune is set): out[4] = rows per wave, kz planes per block,

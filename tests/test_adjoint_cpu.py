@@ -80,3 +80,13 @@ def test_adjoint_rejects_bad_contexts():
     with pytest.raises(ValueError):
         host.adjoint(np.zeros((10, 3), np.float32))  # wrong recorder count
     host.close()
+
+
+@pytest.mark.parametrize("n", [5, 8, 13])
+def test_fft_style_dct_matches_the_matrix_transforms(n):
+    """The FFT-style transform of the naive reference against the definitions."""
+    rng = np.random.default_rng(n)
+    v = torch.tensor(rng.standard_normal((3, 4, n)), dtype=torch.float64)
+    c2, c3 = torch_ref._cos_mats(n, torch.float64, "cpu")
+    assert torch.allclose(torch_ref._dct_fft(v, 2, False), v @ c2.t(), rtol=0, atol=1e-12)
+    assert torch.allclose(torch_ref._dct_fft(v, 2, True), v @ c3.t(), rtol=0, atol=1e-12)

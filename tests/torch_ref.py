@@ -57,10 +57,29 @@ def _shift(x, axis, s):
     return Fn.pad(y, pad)
 
 
+def _dct_fft(v, dim, inverse):
+    """REDFT10 (inverse: REDFT01) along dim through a complex FFT of length 2n:
+    REDFT10  Y_k = Re(e^{-i pi k / 2n} FFT_2n([x, reversed x])_k)
+    REDFT01  Y_j = Re(2n IFFT_2n(Z)_j),  Z_k = c_k X_k e^{i pi k / 2n} (k < n), c_0 = 1, c_k = 2"""
+    n = v.shape[dim]
+    k = torch.arange(n, device=v.device, dtype=v.dtype)
+    shape = [1] * v.dim()
+    shape[dim] = n
+    ph = torch.polar(torch.ones_like(k), (math.pi / (2 * n)) * k).reshape(shape)
+    if not inverse:
+        ext = torch.cat([v, torch.flip(v, [dim])], dim=dim)
+        f = torch.fft.fft(ext, dim=dim)
+        return (torch.narrow(f, dim, 0, n) * ph.conj()).real
+    c = torch.where(k == 0, torch.ones_like(k), 2 * torch.ones_like(k)).reshape(shape)
+    z = torch.cat([v * c * ph, torch.zeros_like(v, dtype=ph.dtype)], dim=dim)
+    return (2 * n) * torch.narrow(torch.fft.ifft(z, dim=dim), dim, 0, n).real
+
+
 class TorchARD:
-    def __init__(self, scene, device="cpu", dtype=torch.float64):
+    def __init__(self, scene, device="cpu", dtype=torch.float64, dct="matrix"):
         assert scene.recorder_mode == S.REC_LOCAL
-        self.sc, self.device, self.dtype = scene, device, dtype
+        assert dct in ("matrix", "fft")
+        self.sc, self.device, self.dtype, self.dct = scene, device, dtype, dct
         host = A.Simulation(scene, device=-1, finalize=False)
         self.parts = host.partitions()
         self.bnds = host.boundaries()
@@ -98,6 +117,10 @@ class TorchARD:
                 self.src.append(None)
 
     def _dct3(self, v, inverse):
+        if self.dct == "fft":
+            for dim in (2, 1, 0):
+                v = _dct_fft(v, dim, inverse)
+            return v
         d, hh, w = v.shape
         c2w, c3w = self.mats[w]
         c2h, c3h = self.mats[hh]
