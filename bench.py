@@ -87,7 +87,7 @@ def traffic_from_profile(kernel):
         return None
     with open(path) as f:
         d = json.load(f)
-    return d.get(kernel)
+    return d.get("kernels", {}).get(kernel)
 
 
 def cpu_baseline(sc):
