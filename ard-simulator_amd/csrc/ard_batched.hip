@@ -73,6 +73,8 @@ struct BatchedState {
   UnfoldDesc* d_unfold = nullptr; long long* d_unfold_start = nullptr; int nunfold = 0; long long unfold_total = 0;
   GemmDesc* d_gemm[4] = {};   // FT1, FT2, IT1, IT2
   int ngemm[4] = {}, gemm_tiles[4] = {};
+  GemmDesc* d_gemm_fold[4] = {};  // the same passes even/odd folded (transform variant 3)
+  int gemm_fold_tiles[4] = {};
   struct ModalGroup {
     int rpw;
     bool pts;
@@ -158,6 +160,38 @@ GemmDesc gemm_n_contig(const float* A, int L, const float* B, float* C, int inne
   g.alpha = 1.f;
   g.tiles_n = int(ceil_div(g.N, 64));
   return g;
+}
+
+// Even/odd folded cosine matrices of an axis of length L (k_gemm_mfma_fold):
+// forward [Mh + floor(L/2)][ceil(L/2)]: rows m < ceil(L/2) = REDFT10 row 2m,
+// zero rows up to Mh (a multiple of 64), then the odd rows 2m+1; inverse
+// [2][ceil(L/2)][ceil(L/2)]: REDFT01 columns 2k' (E) and 2k'+1 (O) of rows j.
+// FP64, stored FP32; cached per length in the context.
+int fold_tables(Ctx& c, int L, const float** fwd, const float** inv) {
+  auto it = c.fold_tables.find(L);
+  if (it == c.fold_tables.end()) {
+    const int Kh = (L + 1) / 2, Me = (L + 1) / 2, Mo = L / 2;
+    const int Mh = int(ceil_div(Me, 64) * 64);
+    std::vector<float> f(size_t(Mh + Mo) * Kh, 0.f), v(size_t(2) * Kh * Kh, 0.f);
+    auto c2 = [&](int k, int j) { return 2.0 * std::cos(kPi * k * (j + 0.5) / L); };
+    auto c3 = [&](int j, int k) { return k == 0 ? 1.0 : 2.0 * std::cos(kPi * k * (j + 0.5) / L); };
+    for (int j = 0; j < Kh; ++j) {
+      for (int m = 0; m < Me; ++m) f[size_t(m) * Kh + j] = float(c2(2 * m, j));
+      for (int m = 0; m < Mo; ++m) f[size_t(Mh + m) * Kh + j] = float(c2(2 * m + 1, j));
+      for (int k = 0; k < Kh; ++k) {
+        if (2 * k < L) v[size_t(j) * Kh + k] = float(c3(j, 2 * k));
+        if (2 * k + 1 < L) v[size_t(Kh + j) * Kh + k] = float(c3(j, 2 * k + 1));
+      }
+    }
+    float *df, *di;
+    int r;
+    if ((r = upload_vec(c, (void**)&df, f.data(), f.size() * 4)) || (r = upload_vec(c, (void**)&di, v.data(), v.size() * 4)))
+      return r;
+    it = c.fold_tables.emplace(L, std::make_pair(df, di)).first;
+  }
+  *fwd = it->second.first;
+  *inv = it->second.second;
+  return ARD_OK;
 }
 
 int upload_gemms(Ctx& c, std::vector<GemmDesc>& v, GemmDesc** d, int* tiles) {
@@ -505,7 +539,27 @@ int finalize_batched(Ctx& c) {
   }
   for (int k = 0; k < 4; ++k) {
     B.ngemm[k] = int(gm[k].size());
+    // the folded twin of each pass: forward passes (k = 0, 1) fold the input,
+    // inverse passes (k = 2, 3) produce the output pair
+    std::vector<GemmDesc> gf = gm[k];
+    for (GemmDesc& g : gf) {
+      const int L = g.M;
+      const float *ff, *fi;
+      if ((r = fold_tables(c, L, &ff, &fi))) return r;
+      g.L = L;
+      g.K = (L + 1) / 2;
+      if (k < 2) {
+        g.A = ff;
+        g.Mh = int(ceil_div((L + 1) / 2, 64) * 64);
+        g.M = g.Mh + L / 2;
+      } else {
+        g.A = fi;
+        g.M = (L + 1) / 2;
+      }
+      g.tiles_n = int(ceil_div(g.N, 64));
+    }
     if ((r = upload_gemms(c, gm[k], &B.d_gemm[k], &B.gemm_tiles[k]))) return r;
+    if ((r = upload_gemms(c, gf, &B.d_gemm_fold[k], &B.gemm_fold_tiles[k]))) return r;
   }
   {
     std::vector<long long> st;
@@ -595,13 +649,18 @@ int enqueue_step(Ctx& c, int par, PhaseTimer* T) {
   phase(ARD_PH_PML, [&] { return launch_pml(B.d_pml, B.npml, B.pml_blocks, par, c.prm.h, c.prm.dt, c.prm.c0, sp); });
   if (fork) r |= hipEventRecord(B.ev_join, B.side) != hipSuccess;
   phase(ARD_PH_FOLD, [&] { return launch_fold(B.d_fold, B.d_fold_start, B.nfold, B.fold_total, st); });
-  phase(ARD_PH_FACE_FWD1, [&] { return launch_gemm(B.d_gemm[0], B.ngemm[0], B.gemm_tiles[0], true, true, 0, st, B.gemm_variant); });
-  phase(ARD_PH_FACE_FWD2, [&] { return launch_gemm(B.d_gemm[1], B.ngemm[1], B.gemm_tiles[1], false, false, 0, st, B.gemm_variant); });
+  auto face = [&](int k) {
+    if (B.gemm_variant == 3)
+      return launch_gemm_fold(B.d_gemm_fold[k], B.ngemm[k], B.gemm_fold_tiles[k], k % 2 == 0, k < 2 ? 1 : 2, st);
+    return launch_gemm(B.d_gemm[k], B.ngemm[k], B.gemm_tiles[k], k % 2 == 0, k % 2 == 0, 0, st, B.gemm_variant);
+  };
+  phase(ARD_PH_FACE_FWD1, [&] { return face(0); });
+  phase(ARD_PH_FACE_FWD2, [&] { return face(1); });
   for (auto& g : B.modal)
     phase(ARD_PH_MODAL, [&] { return launch_modal(g.rpw, g.pts, g.d_desc[par], g.d_start, g.n, g.blocks, par, st); });
   phase(ARD_PH_UNFOLD, [&] { return launch_unfold(B.d_unfold, B.d_unfold_start, B.nunfold, B.unfold_total, st); });
-  phase(ARD_PH_FACE_INV1, [&] { return launch_gemm(B.d_gemm[2], B.ngemm[2], B.gemm_tiles[2], true, true, 0, st, B.gemm_variant); });
-  phase(ARD_PH_FACE_INV2, [&] { return launch_gemm(B.d_gemm[3], B.ngemm[3], B.gemm_tiles[3], false, false, 0, st, B.gemm_variant); });
+  phase(ARD_PH_FACE_INV1, [&] { return face(2); });
+  phase(ARD_PH_FACE_INV2, [&] { return face(3); });
   if (fork) r |= hipStreamWaitEvent(st, B.ev_join, 0) != hipSuccess;
   // the interface reads the PML pressure just written: buffer par ^ 1
   phase(ARD_PH_INTERFACE, [&] { return launch_interface(B.d_bnd, B.nbnd, B.d_parts, par ^ 1, B.bnd_total, st); });
@@ -671,11 +730,15 @@ int autotune(Ctx& c) {
   // the FP32 transform kernels compete on speed; BF16 split precision, when
   // requested, is kept (a precision choice, not a speed one)
   float bestg = 1e30f;
-  for (int v : {0, 1}) {
+  for (int v : {0, 1, 3}) {
     if (B.gemm_variant == 2) break;
     const float ms = time_ms([&] {
-      for (int k = 0; k < 4; ++k)
-        launch_gemm(B.d_gemm[k], B.ngemm[k], B.gemm_tiles[k], k % 2 == 0, k % 2 == 0, 0, c.stream, v);
+      for (int k = 0; k < 4; ++k) {
+        if (v == 3)
+          launch_gemm_fold(B.d_gemm_fold[k], B.ngemm[k], B.gemm_fold_tiles[k], k % 2 == 0, k < 2 ? 1 : 2, c.stream);
+        else
+          launch_gemm(B.d_gemm[k], B.ngemm[k], B.gemm_tiles[k], k % 2 == 0, k % 2 == 0, 0, c.stream, v);
+      }
     }, 3);
     tried++;
     if (ms < bestg) { bestg = ms; B.gemm_variant = v; }

@@ -510,14 +510,17 @@ __global__ __launch_bounds__(256) void k_modal(const ModalDesc* __restrict__ des
 // ---------------------------------------------------------------------------
 // Recorder samples from the block partials, then the step counter.
 // ---------------------------------------------------------------------------
-__global__ void k_brecord(const BRecDesc* __restrict__ r, int n, float* __restrict__ trace, int cap,
-                          int* __restrict__ d_step) {
+__global__ __launch_bounds__(256) void k_brecord(const BRecDesc* __restrict__ r, int n, float* __restrict__ trace,
+                                                 int cap, int* __restrict__ d_step) {
+  // one wave per recorder: its lanes sum the block partials in FP64
   const int t = *d_step;
-  __syncthreads();
-  for (int i = threadIdx.x; i < n; i += blockDim.x) {
-    float s = 0.f;
-    for (int b = 0; b < r[i].nblocks; ++b) s += r[i].part[(long long)b * ARD_MAX_REC_PER_PART];
-    if (t < cap) trace[(long long)t * n + i] = s;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int i = wave; i < n; i += 4) {
+    double s = 0.0;
+    for (int b = lane; b < r[i].nblocks; b += 64) s += double(r[i].part[(long long)b * ARD_MAX_REC_PER_PART]);
+#pragma unroll
+    for (int m = 1; m < 64; m <<= 1) s += __shfl_xor(s, m);
+    if (lane == 0 && t < cap) trace[(long long)t * n + i] = float(s);
   }
   __syncthreads();
   if (threadIdx.x == 0) *d_step = t + 1;

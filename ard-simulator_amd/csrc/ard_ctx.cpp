@@ -470,7 +470,7 @@ int ard_batched_config(ard_ctx* ctx, int* out4) {
 }
 
 int ard_set_transform_variant(int variant) {
-  if (variant < 0 || variant > 2) return fail(ARD_ERR_INVALID, "unknown transform variant");
+  if (variant < 0 || variant > 3) return fail(ARD_ERR_INVALID, "unknown transform variant");
   g_gemm_variant = variant;
   return ARD_OK;
 }

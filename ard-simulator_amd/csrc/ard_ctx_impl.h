@@ -67,6 +67,7 @@ struct Ctx {
 
   std::vector<void*> allocs;
   std::map<int, std::pair<float*, float*>> tables;  // axis length -> (REDFT10, REDFT01) matrices
+  std::map<int, std::pair<float*, float*>> fold_tables;  // axis length -> even/odd folded (forward, inverse)
 
   // step counter and recorder trace, device-resident so captured graphs replay correctly
   int* d_step = nullptr;

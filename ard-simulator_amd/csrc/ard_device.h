@@ -38,6 +38,12 @@ struct GemmDesc {
   const float* ep_a;
   const float* ep_g;
   float ep_b;
+  // Even/odd folded transforms along an axis of length L (k_gemm_mfma_fold):
+  // forward, rows [0, Mh) give the even outputs 2m from x[j] + x[L-1-j], rows
+  // Mh.. the odd outputs from x[j] - x[L-1-j]; inverse, row j gives
+  // E = A_E X_even and O = A_O X_odd and stores x[j] = E + O, x[L-1-j] = E - O.
+  int L;
+  int Mh;
 };
 
 // A PML slab (ARD/pml_partition.h). Buffers are double-buffered by step parity:

@@ -379,7 +379,155 @@ __global__ __launch_bounds__(256) void k_gemm_bf16x3(const GemmDesc* __restrict_
   }
 }
 
-// variant: 0 vector FMA, 1 matrix cores (FP32), 2 matrix cores (BF16 split, FP32 accumulate)
+// ---------------------------------------------------------------------------
+// Even/odd folded per-axis transforms on the matrix cores (FP32). With
+// c_L(k, L-1-j) = (-1)^k c_L(k, j):
+//   forward (REDFT10)  X[2m]   = sum_{j<ceil(L/2)} C[2m][j]   (x[j] + x[L-1-j])
+//                      X[2m+1] = sum_{j<floor(L/2)} C[2m+1][j] (x[j] - x[L-1-j])
+//   inverse (REDFT01)  E[j] = sum_k' Ci[j][2k'] X[2k'],  O[j] = sum_k' Ci[j][2k'+1] X[2k'+1]
+//                      x[j] = E + O,  x[L-1-j] = E - O
+// (the middle element of an odd L counts once). Half the multiply-adds of the
+// dense pass. Forward: rows [0, Mh) of the folded matrix are the even outputs
+// (Mh a multiple of 64, so a 64-row tile is all even or all odd), the sum or
+// difference is formed as B is staged. Inverse: A holds [A_E; A_O] (M rows
+// each); every tile accumulates both and writes the pair.
+// ---------------------------------------------------------------------------
+template <bool B_K_CONTIG, int FOLD>
+__global__ __launch_bounds__(256) void k_gemm_mfma_fold(const GemmDesc* __restrict__ descs, int ndesc) {
+  constexpr int NS = FOLD == 2 ? 2 : 1;  // operand sets (E, O)
+  __shared__ float As[2][NS][16][68];
+  __shared__ float Bs[2][NS][16][68];
+  const int tile = blockIdx.x;
+  int lo = 0, hi = ndesc - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (descs[mid].tile0 <= tile) lo = mid;
+    else hi = mid - 1;
+  }
+  const GemmDesc& g = descs[lo];
+  const int M = g.M, N = g.N, K = g.K, N1 = g.N1, L = g.L;
+  const int t = tile - g.tile0;
+  const int m0 = (t / g.tiles_n) * 64;
+  const int n0 = (t % g.tiles_n) * 64;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+  const bool odd_tile = FOLD == 1 && m0 >= g.Mh;  // forward: uniform per tile
+
+  long long boff[4];
+  int bn[4];
+  if (B_K_CONTIG) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int n = n0 + tid / 16 + 16 * r;
+      bn[r] = n;
+      boff[r] = n < N ? (long long)(n % N1) * g.b1 + (long long)(n / N1) * g.b2 : 0;
+    }
+  } else {
+    const int n = n0 + (tid & 63);
+    bn[0] = n;
+    boff[0] = n < N ? (long long)(n % N1) * g.b1 + (long long)(n / N1) * g.b2 : 0;
+  }
+  f32x16 acc[NS];
+#pragma unroll
+  for (int q = 0; q < NS; ++q)
+#pragma unroll
+    for (int r = 0; r < 16; ++r) acc[q][r] = 0.f;
+
+  const gfloat* __restrict__ A = ARD_CG(g.A);
+  const gfloat* __restrict__ B = ARD_CG(g.B);
+  float ra[NS][4], rb[2][4];
+  // B rows of K index k: forward x[k] and x[L-1-k]; inverse X[2k] and X[2k+1]
+  auto brow = [&](int k, int w) { return FOLD == 1 ? (w == 0 ? k : L - 1 - k) : 2 * k + w; };
+  auto gload = [&](const int k0) {
+#pragma unroll
+    for (int q = 0; q < NS; ++q)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int k = tid & 15, m = (tid >> 4) + 16 * r;
+        ra[q][r] = A[(long long)q * M * K + (long long)min(m0 + m, M - 1) * K + min(k0 + k, K - 1)];
+      }
+#pragma unroll
+    for (int w = 0; w < 2; ++w)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int gk = min(B_K_CONTIG ? k0 + (tid & 15) : k0 + (tid >> 6) + 4 * r, K - 1);
+        rb[w][r] = B[(long long)min(brow(gk, w), L - 1) * g.bK + boff[B_K_CONTIG ? r : 0]];
+      }
+  };
+  auto sstore = [&](const int k0, const int buf) {
+#pragma unroll
+    for (int q = 0; q < NS; ++q)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int k = tid & 15, m = (tid >> 4) + 16 * r;
+        As[buf][q][k][m] = (m0 + m < M && k0 + k < K) ? ra[q][r] : 0.f;
+      }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int k = B_K_CONTIG ? (tid & 15) : (tid >> 6) + 4 * r;
+      const int n = B_K_CONTIG ? (tid >> 4) + 16 * r : (tid & 63);
+      const int gk = k0 + k;
+      const bool ok = (B_K_CONTIG ? bn[r] : bn[0]) < N && gk < K;
+      if (FOLD == 1) {
+        const bool mid = gk == L - 1 - gk;  // odd L: the middle element, counted once
+        const float v = odd_tile ? (mid ? 0.f : rb[0][r] - rb[1][r]) : (mid ? rb[0][r] : rb[0][r] + rb[1][r]);
+        Bs[buf][0][k][n] = ok ? v : 0.f;
+      } else {
+        Bs[buf][0][k][n] = ok && 2 * gk < L ? rb[0][r] : 0.f;
+        Bs[buf][1][k][n] = ok && 2 * gk + 1 < L ? rb[1][r] : 0.f;
+      }
+    }
+  };
+  gload(0);
+  int buf = 0;
+  for (int k0 = 0; k0 < K; k0 += 16, buf ^= 1) {
+    sstore(k0, buf);
+    __syncthreads();
+    if (k0 + 16 < K) gload(k0 + 16);
+#pragma unroll
+    for (int kk = 0; kk < 16; kk += 2) {
+#pragma unroll
+      for (int q = 0; q < NS; ++q) {
+        const float a = As[buf][q][kk + (lane >> 5)][wm + (lane & 31)];
+        const float b = Bs[buf][q][kk + (lane >> 5)][wn + (lane & 31)];
+        acc[q] = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, acc[q], 0, 0, 0);
+      }
+    }
+  }
+  const int j = n0 + wn + (lane & 31);
+  if (j >= N) return;
+  const long long cn = (long long)(j % N1) * g.c1 + (long long)(j / N1) * g.c2;
+  gfloat* __restrict__ C = ARD_G(g.C);
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    const int i = m0 + wm + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+    if (i >= M) continue;
+    if (FOLD == 1) {
+      const int k = i < g.Mh ? 2 * i : 2 * (i - g.Mh) + 1;
+      if (k < L) C[(long long)k * g.cM + cn] = g.alpha * acc[0][r];
+    } else {
+      const float e = acc[0][r], o = acc[NS - 1][r];
+      C[(long long)i * g.cM + cn] = g.alpha * (e + o);
+      if (L - 1 - i != i) C[(long long)(L - 1 - i) * g.cM + cn] = g.alpha * (e - o);
+    }
+  }
+}
+
+int launch_gemm_fold(const GemmDesc* d_descs, int ndesc, int total_tiles, bool b_k_contig, int fold, hipStream_t s) {
+  if (total_tiles <= 0) return 0;
+  dim3 grid(total_tiles), block(256);
+  if (fold == 1 && b_k_contig) hipLaunchKernelGGL((k_gemm_mfma_fold<true, 1>), grid, block, 0, s, d_descs, ndesc);
+  else if (fold == 1) hipLaunchKernelGGL((k_gemm_mfma_fold<false, 1>), grid, block, 0, s, d_descs, ndesc);
+  else if (fold == 2 && b_k_contig) hipLaunchKernelGGL((k_gemm_mfma_fold<true, 2>), grid, block, 0, s, d_descs, ndesc);
+  else if (fold == 2) hipLaunchKernelGGL((k_gemm_mfma_fold<false, 2>), grid, block, 0, s, d_descs, ndesc);
+  else return -1;
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+// variant: 0 vector FMA, 1 matrix cores (FP32), 2 matrix cores (BF16 split, FP32 accumulate),
+// 3 matrix cores FP32 with even/odd folding (batched face passes; others use 1)
 int g_gemm_variant = 1;
 
 int launch_gemm(const GemmDesc* d_descs, int ndesc, int total_tiles, bool b_k_contig, bool c_m_contig, int ep,
@@ -387,6 +535,7 @@ int launch_gemm(const GemmDesc* d_descs, int ndesc, int total_tiles, bool b_k_co
   if (total_tiles <= 0) return 0;
   dim3 grid(total_tiles), block(256);
   if (variant < 0) variant = g_gemm_variant;
+  if (variant == 3) variant = 1;  // folding applies to the batched form's face passes; dense passes use 1
   if (variant == 2) {
     if (b_k_contig && ep == 0) hipLaunchKernelGGL((k_gemm_bf16x3<true, 0>), grid, block, 0, s, d_descs, ndesc);
     else if (!b_k_contig && ep == 0) hipLaunchKernelGGL((k_gemm_bf16x3<false, 0>), grid, block, 0, s, d_descs, ndesc);

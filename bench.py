@@ -120,7 +120,8 @@ def main():
     ap.add_argument("--workload", default="hall")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--transform-variant", type=int, default=1, help="0 vector FMA, 1 matrix cores (FP32)")
+    ap.add_argument("--transform-variant", type=int, default=1,
+                    help="0 vector FMA, 1 matrix cores FP32, 2 matrix cores BF16 split, 3 matrix cores FP32 folded")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--direct", action="store_true",
                     help="launch each phase directly instead of replaying the step graph (counter collection)")
@@ -189,7 +190,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": "%s: batched face-shell form, hipGraph per step" % sc.name,
                    "tiling": {"rows_per_wave": tiling[0], "kz_per_block": tiling[1],
-                              "transform": ["vector-fma", "mfma-fp32"][tiling[2]], "autotune_candidates": tiling[3]}, "dct_cells": n_dct,
+                              "transform": ["vector-fma", "mfma-fp32", "mfma-bf16x3", "mfma-fp32-folded"][tiling[2]], "autotune_candidates": tiling[3]}, "dct_cells": n_dct,
                    "pml_cells": n_pml, "face_elements": faces, "boxes": len(sc.boxes), "h": sc.h, "dt": sc.dt,
                    "sources": len(sc.sources), "recorders": len(sc.recorders)},
         "roofline": roof,

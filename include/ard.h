@@ -162,9 +162,11 @@ int ard_kernel_times(ard_ctx* ctx, float* avg_ms, int* launches, int nphase);
 int ard_selftest_transform(int device, int variant, int b_layout, int M, int N, int K, const float* A, const float* B,
                            float* C);
 /* Selects the dense transform kernels used by contexts finalized afterwards
- * (0 vector FMA, 1 matrix cores FP32, 2 matrix cores BF16 split). Default 1.
- * The batched form's autotuner may switch between 0 and 1 (same precision);
- * it keeps 2. */
+ * (0 vector FMA, 1 matrix cores FP32, 2 matrix cores BF16 split, 3 matrix
+ * cores FP32 with the even/odd folding c(k, n-1-j) = (-1)^k c(k, j), half the
+ * multiply-adds; 3 applies to the batched form's face passes, other passes
+ * use 1). Default 1. The batched form's autotuner may choose among 0, 1 and 3
+ * (same precision); it keeps 2. */
 int ard_set_transform_variant(int variant);
 /* Tiling of the batched form after ard_finalize (chosen by the autotuner when
  * ard_params.autotune is set): out[4] = rows per wave, kz planes per block,

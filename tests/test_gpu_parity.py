@@ -55,6 +55,25 @@ def test_scene3_trace_matches_golden(form):
     assert e32 <= TOL and e64 <= TOL
 
 
+@pytest.mark.parametrize("variant", [0, 2, 3])
+def test_scene3_batched_transform_variants(variant):
+    """The batched form with each transform kernel against the goldens
+    (1 is covered above): vector FP32, BF16 split, folded matrix-core FP32."""
+    sc = S.scene3()
+    print("case scene3 batched transform variant", variant, flush=True)
+    A.set_transform_variant(variant)
+    try:
+        sim = A.Simulation(sc, form=A.FORM_BATCHED)
+    finally:
+        A.set_transform_variant(1)
+    assert sim.batched_config()[2] == variant
+    sim.update(sc.steps)
+    tr = sim.recorders()
+    e64 = rel_l2(tr, np.load(os.path.join(GOLD, "scene3_fp64.npy")))
+    print("scene3 variant %d rel-L2 vs fp64 %.3e" % (variant, e64))
+    assert e64 <= TOL
+
+
 def small_scene():
     """Three boxes with partial faces (PML runs, edge cells written twice)."""
     sc = S.two_rooms()
