@@ -1,7 +1,8 @@
 """Precision of the transform kernels on the batch-of-rooms workload: 64
 independent rooms (sides U[2,6] m, 1-4 boxes each, h = 0.05 m) advanced in one
 batched context for 1500 steps, with the per-axis transforms in FP32 on
-vector FMA, FP32 on matrix cores and BF16 split precision on matrix cores;
+vector FMA, FP32 on matrix cores (plain and even/odd folded) and BF16 split
+precision on matrix cores;
 and, as the FFT-style point of comparison, the naive torch restatement with
 FFT-based DCTs in FP32. Each is compared with the FP64 torch restatement
 (on the GPU) on the first rooms; the errors are printed per result."""
@@ -33,7 +34,7 @@ def study():
             ref.append(R.run([None] * len(sc.sources), torch.ones(6, dtype=torch.float64, device="cuda"), STEPS)
                        .cpu().numpy())
     out = {}
-    for v, name in ((0, "fp32 vector"), (1, "fp32 mfma"), (2, "bf16x3 mfma")):
+    for v, name in ((0, "fp32 vector"), (1, "fp32 mfma"), (2, "bf16x3 mfma"), (3, "fp32 mfma folded")):
         A.set_transform_variant(v)
         b = A.Batch(rooms)
         b.update(STEPS)
@@ -56,6 +57,7 @@ def study():
 def test_fp32_transforms_within_tolerance(study):
     assert study["fp32 vector"] <= 1e-4
     assert study["fp32 mfma"] <= 1e-4
+    assert study["fp32 mfma folded"] <= 1e-4
 
 
 def test_bf16_split_error_is_bounded(study):
