@@ -28,7 +28,7 @@ EXPORTED = [
     "ard_copy_pressure_field", "ard_get_pressure", "ard_set_force", "ard_synchronize", "ard_set_kernel_timing",
     "ard_kernel_times", "ard_selftest_transform", "ard_set_transform_variant", "ard_batched_config",
     "ard_adjoint", "ard_batch_create", "ard_batch_step", "ard_batch_fetch_recorders", "ard_batch_context",
-    "ard_batch_room_partition", "ard_batch_destroy",
+    "ard_batch_room_partition", "ard_batch_destroy", "ard_set_recorder_cubes", "ard_fetch_recorder_cubes",
 ]
 PHASES = ["sources", "pml", "fold", "face_fwd1", "face_fwd2", "modal", "unfold", "face_inv1", "face_inv2",
           "interface", "record", "dct_fwd", "dct_inv"]
@@ -92,6 +92,8 @@ def lib():
         L.ard_batch_context.argtypes = [vp, ctypes.POINTER(vp)]
         L.ard_batch_room_partition.argtypes = [vp, i, i]
         L.ard_batch_destroy.argtypes = [vp]
+        L.ard_set_recorder_cubes.argtypes = [vp, i]
+        L.ard_fetch_recorder_cubes.argtypes = [vp, fp, i, i]
         _lib = L
     return _lib
 
@@ -129,7 +131,8 @@ class Simulation:
     whose topology can be inspected without a GPU.
     """
 
-    def __init__(self, scene, form=FORM_PLAIN, device=0, pml_scale=None, finalize=True, autotune=False):
+    def __init__(self, scene, form=FORM_PLAIN, device=0, pml_scale=None, finalize=True, autotune=False,
+                 cubes=False):
         self.L = lib()
         self.scene = scene
         self.form = form
@@ -155,6 +158,8 @@ class Simulation:
         for r in scene.recorders:
             _check(self.L.ard_add_recorder(self.ctx, *r))
         self.nrec = len(scene.recorders)
+        if cubes:
+            _check(self.L.ard_set_recorder_cubes(self.ctx, 1))
         if finalize:
             _check(self.L.ard_finalize(self.ctx))
 
@@ -203,6 +208,15 @@ class Simulation:
         out = np.zeros((nt, max(self.nrec, 1)), dtype=np.float32)
         _check(self.L.ard_fetch_recorders(self.ctx, _fp(out), t0, nt))
         return out[:, : self.nrec]
+
+    def recorder_cubes(self, t0=0, nt=None):
+        """Recorder cubes [nt][num_recorders][10][10][10] (z, y, x offsets -5..4)
+        of steps t0..t0+nt-1: the out_<id>.txt rows of Recorder::RecordField."""
+        if nt is None:
+            nt = self.time_step - t0
+        out = np.zeros((nt, self.nrec, 10, 10, 10), dtype=np.float32)
+        _check(self.L.ard_fetch_recorder_cubes(self.ctx, _fp(out), t0, nt))
+        return out
 
     def pressure_field(self, part):
         info = self.partitions()[part]

@@ -43,12 +43,15 @@ int ard_batch_create(ard_ctx* const* members, int n, ard_batch** out) {
     std::string err = c->topo.resolve(c->prm.recorder_mode == ARD_REC_LOCAL);
     if (!err.empty()) return set_error(ARD_ERR_RANGE, err);
   }
+  for (Ctx* c : m)
+    if (c->cubes != m[0]->cubes) return set_error(ARD_ERR_INVALID, "members differ in recorder cubes");
   ard_params p = m[0]->prm;
   p.recorder_mode = ARD_REC_LOCAL;  // recorders are copied resolved
   ard_ctx* h = nullptr;
   int r = ard_ctx_create(m[0]->device, &p, &h);
   if (r != ARD_OK) return r;
   Ctx* M = reinterpret_cast<Ctx*>(h);
+  M->cubes = m[0]->cubes;
   std::copy(m[0]->pml_scale, m[0]->pml_scale + 6, M->pml_scale);
   auto* b = new ard_batch();
   b->merged = h;

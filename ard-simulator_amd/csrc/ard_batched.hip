@@ -665,6 +665,7 @@ int enqueue_step(Ctx& c, int par, PhaseTimer* T) {
   // the interface reads the PML pressure just written: buffer par ^ 1
   phase(ARD_PH_INTERFACE, [&] { return launch_interface(B.d_bnd, B.nbnd, B.d_parts, par ^ 1, B.bnd_total, st); });
   phase(ARD_PH_RECORD, [&] {
+    if (enqueue_cubes(c, par, st)) return 1;
     return launch_brecord(B.d_rec, int(c.topo.recorders.size()), c.d_trace, c.trace_cap, c.d_step, st);
   });
   return r ? set_error(ARD_ERR_HIP, "batched step launch") : ARD_OK;
@@ -823,6 +824,15 @@ int batched_pressure_field(Ctx& c, int part, const float** out) {
   r |= launch_gemm(B.d_dense + 2, 1, int(ceil_div(g[2].M, 64)) * g[2].tiles_n, false, false, 0, c.stream);
   if (r) return set_error(ARD_ERR_HIP, "dense evaluation launch");
   *out = B.dense[2];
+  return ARD_OK;
+}
+
+// The mode buffers of a DCT partition (by step parity) and their padded row length.
+int batched_modes(Ctx& c, int part, const float* M[2], int* wp) {
+  const BPart& P = c.bs->parts[size_t(part)];
+  M[0] = P.M[0];
+  M[1] = P.M[1];
+  *wp = P.wp;
   return ARD_OK;
 }
 

@@ -87,6 +87,7 @@ Ctx::~Ctx() {
   if (graph_exec[1]) hipGraphExecDestroy(graph_exec[1]);
   destroy_batched(*this);
   destroy_adjoint(*this);
+  destroy_output(*this);
   for (void* p : allocs) hipFree(p);
   if (stream) hipStreamDestroy(stream);
 }
@@ -296,9 +297,11 @@ int ard_finalize(ard_ctx* ctx) {
   if (c->finalized) return fail(ARD_ERR_STATE, "already finalized");
   std::string err = c->topo.resolve(c->prm.recorder_mode == ARD_REC_LOCAL);
   if (!err.empty()) return fail(ARD_ERR_RANGE, err);
+  if (int r = check_cubes(*c)) return r;
   if (c->device < 0) return fail(ARD_ERR_STATE, "host-only context (device -1) cannot be finalized");
   hipSetDevice(c->device);
   int r = c->prm.form == ARD_FORM_PLAIN ? finalize_plain(*c) : finalize_batched(*c);
+  if (r == ARD_OK) r = setup_cubes(*c);
   if (r != ARD_OK) return r;
   c->finalized = true;
   return ARD_OK;
@@ -412,6 +415,30 @@ int ard_fetch_recorders(ard_ctx* ctx, float* out, int t0, int nt) {
   if (nt == 0 || nr == 0) return ARD_OK;
   hipSetDevice(c->device);
   hipError_t e = hipMemcpyAsync(out, c->d_trace + size_t(t0) * nr, size_t(nt) * nr * 4, hipMemcpyDeviceToHost,
+                                c->stream);
+  if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync D2H");
+  e = hipStreamSynchronize(c->stream);
+  return e == hipSuccess ? ARD_OK : hip_fail(e, "hipStreamSynchronize");
+}
+
+int ard_set_recorder_cubes(ard_ctx* ctx, int on) {
+  Ctx* c = reinterpret_cast<Ctx*>(ctx);
+  if (!c) return fail(ARD_ERR_INVALID, "null context");
+  if (c->finalized) return fail(ARD_ERR_STATE, "already finalized");
+  c->cubes = on != 0;
+  return ARD_OK;
+}
+
+int ard_fetch_recorder_cubes(ard_ctx* ctx, float* out, int t0, int nt) {
+  Ctx* c = reinterpret_cast<Ctx*>(ctx);
+  if (!c || (!out && nt > 0)) return fail(ARD_ERR_INVALID, "null argument");
+  if (!c->finalized) return fail(ARD_ERR_STATE, "not finalized");
+  if (!c->cubes) return fail(ARD_ERR_STATE, "recorder cubes were not switched on before ard_finalize");
+  const size_t row = c->topo.recorders.size() * 1000;
+  if (t0 < 0 || nt < 0 || t0 + nt > c->time_step) return fail(ARD_ERR_RANGE, "steps not taken yet");
+  if (nt == 0 || row == 0) return ARD_OK;
+  hipSetDevice(c->device);
+  hipError_t e = hipMemcpyAsync(out, c->d_cube + size_t(t0) * row, size_t(nt) * row * 4, hipMemcpyDeviceToHost,
                                 c->stream);
   if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync D2H");
   e = hipStreamSynchronize(c->stream);

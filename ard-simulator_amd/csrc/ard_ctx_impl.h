@@ -15,7 +15,8 @@
 namespace ard {
 
 struct BatchedState;
-struct AdjState;  // ard_batched.hip
+struct AdjState;      // ard_adjoint.cpp
+struct OutputState;   // ard_output.hip
 
 // Dense fields of one DCT partition in the plain form (ARD/dct_partition.h,
 // ARD/dct_volume.h): force F and pressure p in the value domain, modes M[2]
@@ -73,6 +74,10 @@ struct Ctx {
   int* d_step = nullptr;
   float* d_trace = nullptr;
   int trace_cap = 0;
+  // recorder cubes (ard_set_recorder_cubes): [trace_cap][recorders][1000]
+  bool cubes = false;
+  float* d_cube = nullptr;
+  OutputState* out = nullptr;
 
   // plain form
   std::vector<DctPlain> dct;       // by partition id
@@ -132,6 +137,13 @@ int batched_pressure_field(Ctx& c, int part, const float** out);
 int batched_set_force(Ctx& c, int part, int x, int y, int z, float f);
 void destroy_batched(Ctx& c);
 int batched_config(Ctx& c, int* out4);
+int batched_modes(Ctx& c, int part, const float* M[2], int* wp);
+
+// recorder cubes (ard_output.hip)
+int check_cubes(const Ctx& c);  // host-side range check
+int setup_cubes(Ctx& c);        // after the form's finalize
+int enqueue_cubes(Ctx& c, int par, hipStream_t st);
+void destroy_output(Ctx& c);
 
 // field access (either form)
 int pressure_field_ptr(Ctx& c, int part, const float** out);

@@ -53,6 +53,20 @@ int ensure_trace(Ctx& c, int steps_needed) {
     c.allocs.erase(std::find(c.allocs.begin(), c.allocs.end(), (void*)c.d_trace));
   }
   c.d_trace = nt;
+  if (c.cubes && nr > 0) {  // the cube rows grow with the point trace
+    const size_t row = size_t(nr) * 1000 * 4;
+    float* nc = nullptr;
+    if ((r = c.dalloc((void**)&nc, size_t(cap) * row))) return r;
+    if (c.d_cube) {
+      hipError_t e = hipMemcpyAsync(nc, c.d_cube, size_t(c.trace_cap) * row, hipMemcpyDeviceToDevice, c.stream);
+      if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync D2D");
+      e = hipStreamSynchronize(c.stream);
+      if (e != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
+      hipFree(c.d_cube);
+      c.allocs.erase(std::find(c.allocs.begin(), c.allocs.end(), (void*)c.d_cube));
+    }
+    c.d_cube = nc;
+  }
   c.trace_cap = cap;
   return ARD_OK;
 }
@@ -419,6 +433,7 @@ int step_plain(Ctx& c, int n) {
       T.end(c.stream, ev);
     }
     ev = T.begin(c.stream, ARD_PH_RECORD);
+    if (enqueue_cubes(c, par, c.stream)) return set_error(ARD_ERR_HIP, "recorder cube launch");
     if (launch_record(c.d_rec, nr, c.d_trace, c.trace_cap, c.d_step, c.stream))
       return set_error(ARD_ERR_HIP, "recorder launch");
     T.end(c.stream, ev);

@@ -17,6 +17,12 @@ int Simulation::m_form = ARD_FORM_BATCHED;
 int Simulation::m_recorder_mode = ARD_REC_REFERENCE;
 int Simulation::m_device = 0;
 int Simulation::m_autotune = 0;
+int Simulation::m_record_cubes = 1;
+std::string Simulation::m_output_root = "./output/";
+
+std::string Simulation::OutputDir() {
+  return m_output_root + std::to_string(m_dh) + "_" + std::to_string(Partition::m_absorption) + "/";
+}
 
 namespace {
 void check(int r, const char* what) {
@@ -96,6 +102,17 @@ real_t GaussianSource::SampleValue(real_t t) {
   return 1e9f * std::exp(-arg);
 }
 
+SoundSource::SoundSource(int x, int y, int z) : x_(x), y_(y), z_(z) {
+  static int id_generator = 0;
+  id_ = id_generator++;
+  source_.open(Simulation::OutputDir() + "source_" + std::to_string(id_) + ".txt");
+}
+
+void SoundSource::RecordSource() {
+  for (int t = 0; t < Simulation::m_duration / Simulation::m_dt; t++) source_ << SampleValue(real_t(t)) << std::endl;
+  source_.close();
+}
+
 std::vector<std::shared_ptr<SoundSource>> SoundSource::ImportSources(const std::string& path, bool cell_units) {
   std::vector<std::shared_ptr<SoundSource>> out;
   std::ifstream file(path);
@@ -107,10 +124,17 @@ std::vector<std::shared_ptr<SoundSource>> SoundSource::ImportSources(const std::
     out.push_back(std::make_shared<GaussianSource>(to_cell(x, cell_units), to_cell(y, cell_units),
                                                    to_cell(z, cell_units)));
   }
+  for (auto& s : out) s->RecordSource();
   return out;
 }
 
-Recorder::Recorder(int x, int y, int z, int total_steps) : x_(x), y_(y), z_(z), total_steps_(total_steps) {}
+Recorder::Recorder(int x, int y, int z, int total_steps) : x_(x), y_(y), z_(z), total_steps_(total_steps) {
+  static int id_generator = 0;
+  id_ = id_generator++;
+  const std::string dir = Simulation::OutputDir();
+  output_.open(dir + "out_" + std::to_string(id_) + ".txt");
+  response_.open(dir + "response_" + std::to_string(id_) + ".txt");
+}
 
 void Recorder::FindPartition(const std::vector<std::shared_ptr<Partition>>& partitions) {
   for (auto& p : partitions) {
@@ -124,27 +148,73 @@ void Recorder::FindPartition(const std::vector<std::shared_ptr<Partition>>& part
 }
 
 void Recorder::OpenResponse(const std::string& path) {
+  if (response_.is_open()) response_.close();
   response_.open(path);
   if (!response_) throw Error("cannot open " + path);
 }
 
+void Recorder::OpenOutput(const std::string& path) {
+  if (output_.is_open()) output_.close();
+  output_.open(path);
+  if (!output_) throw Error("cannot open " + path);
+}
+
+// The point sample and (if asked) the 10^3 cube of `time_step`: from the
+// device traces when registered with a Simulation, else read from the
+// partition's field after the step, with the reference's addressing (global
+// coordinates into the partition's array, ARD/dct_volume.cpp:70-73).
+bool Recorder::sample(int time_step, real_t* point, std::vector<real_t>* cube) {
+  if (sim_ && device_id_ >= 0) {
+    sim_->fetch_row(time_step);
+    *point = sim_->row_points_[size_t(device_id_)];
+    if (cube) {
+      if (!sim_->cubes_) return false;
+      cube->assign(sim_->row_cubes_.begin() + size_t(device_id_) * 1000,
+                   sim_->row_cubes_.begin() + size_t(device_id_ + 1) * 1000);
+    }
+    return true;
+  }
+  if (!part_) throw Error("recorder has no partition (FindPartition found none)");
+  const long long W = part_->width_, H = part_->height_;
+  long long f0;
+  if (Simulation::m_recorder_mode == ARD_REC_LOCAL)
+    f0 = ((long long)(z_ - part_->z_start_) * H + (y_ - part_->y_start_)) * W + (x_ - part_->x_start_);
+  else
+    f0 = ((long long)z_ * H + y_) * W + x_;
+  const std::vector<real_t> f = part_->get_pressure_field();
+  auto at = [&](long long idx) {
+    if (idx < 0 || idx >= (long long)f.size()) throw Error("recorder addresses a cell outside its partition's array");
+    return f[size_t(idx)];
+  };
+  *point = at(f0);
+  if (cube) {
+    cube->resize(1000);
+    for (int i = -5; i < 5; i++)
+      for (int j = -5; j < 5; j++)
+        for (int k = -5; k < 5; k++) (*cube)[size_t((i + 5) * 100 + (j + 5) * 10 + k + 5)] = at(f0 + (i * H + j) * W + k);
+  }
+  return true;
+}
+
 void Recorder::RecordField(int time_step) {
   if (time_step >= total_steps_) return;
-  float v;
-  if (sim_ && device_id_ >= 0) {
-    const int nr = ard_num_recorders(sim_->ctx_);
-    std::vector<float> row(static_cast<size_t>(nr));
-    check(ard_fetch_recorders(sim_->ctx_, row.data(), time_step, 1), "ard_fetch_recorders");
-    v = row[size_t(device_id_)];
-  } else {
-    if (!part_) throw Error("recorder has no partition (FindPartition found none)");
-    // the reference's addressing: global coordinates into the partition's array
-    const long long flat = (long long)z_ * part_->height_ * part_->width_ + (long long)y_ * part_->width_ + x_;
-    const int lx = int(flat % part_->width_), ly = int(flat / part_->width_ % part_->height_),
-              lz = int(flat / ((long long)part_->width_ * part_->height_));
-    v = part_->get_pressure(lx, ly, lz);
+  if (!output_.is_open() && !response_.is_open()) return;
+  real_t v;
+  std::vector<real_t> cube;
+  const bool has_cube = sample(time_step, &v, output_.is_open() ? &cube : nullptr);
+  if (output_.is_open()) {
+    if (!has_cube) throw Error("recorder cubes are off in this Simulation (Simulation::m_record_cubes)");
+    for (real_t c : cube) output_ << c << " ";
+    output_ << std::endl;
   }
   if (response_.is_open()) response_ << v << std::endl;
+}
+
+void Recorder::RecordResponse(int time_step) {
+  if (time_step > total_steps_ || !response_.is_open()) return;
+  real_t v;
+  sample(time_step, &v, nullptr);
+  response_ << v << std::endl;
 }
 
 std::vector<std::shared_ptr<Recorder>> Recorder::ImportRecorders(const std::string& path, bool cell_units) {
@@ -204,6 +274,8 @@ Simulation::Simulation(std::vector<std::shared_ptr<Partition>>& partitions,
       check(r->device_id_, "ard_add_recorder");
       r->sim_ = this;
     }
+    cubes_ = m_record_cubes && !recorders_.empty();
+    if (cubes_) check(ard_set_recorder_cubes(ctx_, 1), "ard_set_recorder_cubes");
     check(ard_finalize(ctx_), "ard_finalize");
   } catch (...) {
     ard_ctx_destroy(ctx_);
@@ -216,6 +288,18 @@ Simulation::~Simulation() {
   for (auto& r : recorders_) r->sim_ = nullptr;
   for (auto& p : partitions_) p->ctx_ = nullptr;
   if (ctx_) ard_ctx_destroy(ctx_);
+}
+
+void Simulation::fetch_row(int t) {
+  if (t == row_t_) return;
+  const int nr = ard_num_recorders(ctx_);
+  row_points_.resize(size_t(nr));
+  check(ard_fetch_recorders(ctx_, row_points_.data(), t, 1), "ard_fetch_recorders");
+  if (cubes_) {
+    row_cubes_.resize(size_t(nr) * 1000);
+    check(ard_fetch_recorder_cubes(ctx_, row_cubes_.data(), t, 1), "ard_fetch_recorder_cubes");
+  }
+  row_t_ = t;
 }
 
 int Simulation::Update() {

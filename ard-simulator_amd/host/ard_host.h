@@ -96,11 +96,16 @@ class PmlPartition : public Partition {
 
 class SoundSource {
  public:
-  SoundSource(int x, int y, int z) : x_(x), y_(y), z_(z) {}
+  // ARD/sound_source.cpp:8-17: numbered in construction order; opens
+  // source_<id>.txt in Simulation::OutputDir() (no file if that folder is missing)
+  SoundSource(int x, int y, int z);
   virtual ~SoundSource() = default;
   virtual real_t SampleValue(real_t t) = 0;
-  // ARD/sound_source.cpp:24-48 (without writing source_<id>.txt)
+  // ARD/sound_source.cpp:24-48: also writes each source's source_<id>.txt
   static std::vector<std::shared_ptr<SoundSource>> ImportSources(const std::string& path, bool cell_units = false);
+  // ARD/sound_source.cpp:50-57: SampleValue(t) for every step of m_duration, one per line
+  void RecordSource();
+  int id() const { return id_; }
   int x() const { return x_; }
   int y() const { return y_; }
   int z() const { return z_; }
@@ -110,6 +115,8 @@ class SoundSource {
 
  protected:
   int x_, y_, z_;
+  int id_;
+  std::ofstream source_;
 };
 
 class GaussianSource : public SoundSource {
@@ -134,13 +141,22 @@ class WaveformSource : public SoundSource {
 
 class Recorder {
  public:
+  // ARD/recorder.cpp:6-17: numbered in construction order; opens out_<id>.txt
+  // and response_<id>.txt in Simulation::OutputDir() (no files if that folder
+  // is missing, as in the reference)
   Recorder(int x, int y, int z, int total_steps = 1000);
   // ARD/recorder.cpp:26-38
   void FindPartition(const std::vector<std::shared_ptr<Partition>>& partitions);
-  // ARD/recorder.cpp:40-57: appends the point sample of `time_step` to the
-  // response stream (the 10^3 cube of out_<id>.txt is not written)
+  // ARD/recorder.cpp:40-57: for time_step < total_steps, appends the 10^3 cube
+  // around the recorder to out_<id>.txt and the point sample to
+  // response_<id>.txt, in the reference's text format (operator<< on float).
+  // Registered with a Simulation, both come from the device traces.
   void RecordField(int time_step = 0);
-  void OpenResponse(const std::string& path);
+  // ARD/recorder.cpp:59-65 (time_step <= total_steps, point sample only)
+  void RecordResponse(int time_step = 0);
+  void OpenResponse(const std::string& path);  // redirect response_<id>.txt
+  void OpenOutput(const std::string& path);    // redirect out_<id>.txt
+  int id() const { return id_; }
   static std::vector<std::shared_ptr<Recorder>> ImportRecorders(const std::string& path, bool cell_units = false);
   int x() const { return x_; }
   int y() const { return y_; }
@@ -148,10 +164,12 @@ class Recorder {
 
  private:
   int x_, y_, z_, total_steps_;
+  int id_;
   std::shared_ptr<Partition> part_;
   Simulation* sim_ = nullptr;
   int device_id_ = -1;  // recorder index in the context when registered
-  std::ofstream response_;
+  std::ofstream output_, response_;
+  bool sample(int time_step, real_t* point, std::vector<real_t>* cube);
   friend class Simulation;
 };
 
@@ -167,6 +185,11 @@ class Simulation {
   static int m_recorder_mode;  // ARD_REC_REFERENCE or ARD_REC_LOCAL
   static int m_device;
   static int m_autotune;       // batched form: time the tiling candidates at construction
+  static int m_record_cubes;   // registered recorders also sample their 10^3 cube on the device
+  static std::string m_output_root;  // "./output/" (ARD/recorder.cpp:13)
+  // m_output_root + std::to_string(m_dh) + "_" + std::to_string(m_absorption) + "/"
+  // (ARD/recorder.cpp:12-13, ARD/sound_source.cpp:13-14)
+  static std::string OutputDir();
 
   // ARD/simulation.cpp:14-294. `partitions` receives the PML slabs, like the
   // reference's member list. Recorders given here are sampled on the device
@@ -188,6 +211,11 @@ class Simulation {
   std::vector<std::shared_ptr<Partition>> partitions_;
   std::vector<std::shared_ptr<Recorder>> recorders_;
   int time_step_ = 0;
+  // one step's device samples, shared by the recorders' RecordField calls
+  int row_t_ = -1;
+  std::vector<real_t> row_points_, row_cubes_;
+  bool cubes_ = false;
+  void fetch_row(int t);
   friend class Recorder;
 };
 

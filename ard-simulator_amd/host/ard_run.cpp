@@ -1,8 +1,12 @@
 // Command-line driver in the shape of the reference's main loop
-// (ARD/main.cpp:47-193) without the SDL window: imports a scene, builds the
-// Simulation, steps it and writes response_<id>.txt for each recorder.
+// (ARD/main.cpp:47-193) without the SDL window: creates the output folder
+// <out_root>/<h>_<absorption>/ (ARD/main.cpp:51-52), imports a scene (the
+// import writes source_<id>.txt), builds the Simulation, steps it and, per
+// step, lets every recorder write its out_<id>.txt and response_<id>.txt rows.
 //
-//   ard_run [options] <scene.txt> <sources.txt> [recorders.txt] [steps] [h] [out_dir]
+//   ard_run [options] <scene.txt> <sources.txt> [recorders.txt] [steps] [h] [out_root]
+//
+// out_root defaults to ./output (the folder must exist, as in the reference).
 //
 // options:
 //   --cells       scene, source and recorder files hold cell indices, not metres
@@ -12,10 +16,16 @@
 //   --plain       per-partition form (default: batched)
 //   --autotune    batched form: time the tiling candidates before stepping
 //   --timing      launch each phase directly and print per-phase kernel times
-//   --no-output   step without writing response files (no per-step device sync)
+//   --no-output   step without writing recorder files (no per-step device sync)
+//   --no-cubes    recorders write response_<id>.txt only (no 10^3 cube sampling)
+//   --c0 V        Simulation::m_c0 in m/s (default 343.5)
+//   --duration S  Simulation::m_duration in seconds (default 2; recorders write
+//                 int(S / dt) rows, sources int(S / dt) samples)
 //   --copy-mib N  first run one device-to-device copy of N MiB: a kernel of known
 //                 traffic, the scale for hardware byte counters
 #include <hip/hip_runtime_api.h>
+
+#include <sys/stat.h>
 
 #include <chrono>
 #include <cstdio>
@@ -40,6 +50,9 @@ int main(int argc, char** argv) {
     else if (!std::strcmp(argv[i], "--autotune")) Simulation::m_autotune = 1;
     else if (!std::strcmp(argv[i], "--timing")) timing = true;
     else if (!std::strcmp(argv[i], "--no-output")) output = false;
+    else if (!std::strcmp(argv[i], "--no-cubes")) Simulation::m_record_cubes = 0;
+    else if (!std::strcmp(argv[i], "--c0") && i + 1 < argc) Simulation::m_c0 = float(std::atof(argv[++i]));
+    else if (!std::strcmp(argv[i], "--duration") && i + 1 < argc) Simulation::m_duration = float(std::atof(argv[++i]));
     else if (!std::strcmp(argv[i], "--copy-mib") && i + 1 < argc) copy_mib = std::atoll(argv[++i]);
     else if (argv[i][0] == '-' && argv[i][1] == '-') {
       std::fprintf(stderr, "unknown option %s\n", argv[i]);
@@ -50,8 +63,8 @@ int main(int argc, char** argv) {
   }
   if (pos.size() < 2) {
     std::fprintf(stderr,
-                 "usage: %s [--cells] [--rec-local] [--plain] [--autotune] [--timing] [--no-output] [--copy-mib N] "
-                 "scene.txt sources.txt [recorders.txt] [steps] [h] [out_dir]\n",
+                 "usage: %s [--cells] [--rec-local] [--plain] [--autotune] [--timing] [--no-output] [--no-cubes] "
+                 "[--c0 V] [--duration S] [--copy-mib N] scene.txt sources.txt [recorders.txt] [steps] [h] [out_root]\n",
                  argv[0]);
     return 2;
   }
@@ -59,7 +72,8 @@ int main(int argc, char** argv) {
     Simulation::m_dh = float(std::atof(pos[4]));
     Simulation::m_dt = Simulation::m_dh < 0.075f ? 0.625e-4f : (Simulation::m_dh < 0.15f ? 1.0f / 8000.0f : 2e-4f);
   }
-  const std::string out_dir = pos.size() > 5 ? pos[5] : ".";
+  if (pos.size() > 5) Simulation::m_output_root = std::string(pos[5]) + "/";
+  if (output) mkdir(Simulation::OutputDir().c_str(), 0755);  // the leaf only, like CreateDirectory
   if (copy_mib > 0) {
     const size_t n = size_t(copy_mib) << 20;
     void *a = nullptr, *b = nullptr;
@@ -80,10 +94,8 @@ int main(int argc, char** argv) {
     std::vector<std::shared_ptr<Recorder>> recorders;
     if (pos.size() > 2) recorders = Recorder::ImportRecorders(pos[2], cells);
     for (auto& r : recorders) r->FindPartition(partitions);
+    if (!output) Simulation::m_record_cubes = 0;
     Simulation sim(partitions, sources, recorders);
-    if (output)
-      for (size_t i = 0; i < recorders.size(); ++i)
-        recorders[i]->OpenResponse(out_dir + "/response_" + std::to_string(i) + ".txt");
     const int steps = pos.size() > 3 ? std::atoi(pos[3]) : int(Simulation::m_duration / Simulation::m_dt);
     if (timing && ard_set_kernel_timing(sim.context(), 1) != ARD_OK) throw Error(ard_last_error());
     auto t1 = std::chrono::steady_clock::now();

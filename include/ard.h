@@ -118,6 +118,16 @@ int ard_time_step(ard_ctx* ctx);
  * column of ARD/recorder.cpp:55). Synchronises. */
 int ard_fetch_recorders(ard_ctx* ctx, float* out, int t0, int nt);
 
+/* Recorder cubes: every step each recorder also samples the 10 x 10 x 10 cells (x+k, y+j, z+i),
+ * i, j, k in [-5, 5), that Recorder::RecordField writes to out_<id>.txt (ARD/recorder.cpp:44-54),
+ * addressed like the point sample (ard_params.recorder_mode). Before ard_finalize; ard_finalize
+ * fails with ARD_ERR_RANGE if a cube reaches outside its partition's array (undefined in the
+ * reference). The batched form evaluates the cubes from the modes on the device. */
+int ard_set_recorder_cubes(ard_ctx* ctx, int on);
+/* Cube samples of steps [t0, t0+nt) into out[nt][num_recorders][1000], element
+ * 100 (i+5) + 10 (j+5) + (k+5): the reference's write order. Synchronises. */
+int ard_fetch_recorder_cubes(ard_ctx* ctx, float* out, int t0, int nt);
+
 /* Partition::get_pressure_field (ARD/partition.h:52): the whole pressure volume, w*h*d floats.
  * Synchronises. The batched form evaluates DCT partitions from their modes on demand. */
 int ard_copy_pressure_field(ard_ctx* ctx, int part, float* out);
