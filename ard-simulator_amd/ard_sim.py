@@ -29,6 +29,7 @@ EXPORTED = [
     "ard_kernel_times", "ard_selftest_transform", "ard_set_transform_variant", "ard_batched_config",
     "ard_adjoint", "ard_batch_create", "ard_batch_step", "ard_batch_fetch_recorders", "ard_batch_context",
     "ard_batch_room_partition", "ard_batch_destroy", "ard_set_recorder_cubes", "ard_fetch_recorder_cubes",
+    "ard_render_view",
 ]
 PHASES = ["sources", "pml", "fold", "face_fwd1", "face_fwd2", "modal", "unfold", "face_inv1", "face_inv2",
           "interface", "record", "dct_fwd", "dct_inv"]
@@ -94,6 +95,7 @@ def lib():
         L.ard_batch_destroy.argtypes = [vp]
         L.ard_set_recorder_cubes.argtypes = [vp, i]
         L.ard_fetch_recorder_cubes.argtypes = [vp, fp, i, i]
+        L.ard_render_view.argtypes = [vp, i, i, ctypes.POINTER(ctypes.c_uint32), ip]
         _lib = L
     return _lib
 
@@ -216,6 +218,17 @@ class Simulation:
             nt = self.time_step - t0
         out = np.zeros((nt, self.nrec, 10, 10, 10), dtype=np.float32)
         _check(self.L.ard_fetch_recorder_cubes(self.ctx, _fp(out), t0, nt))
+        return out
+
+    def render_view(self, view=0, coord=-1):
+        """The visualisation plane of Simulation::Update (ARD/simulation.cpp:320-420):
+        uint32 pixels [rows][columns] of view 0 (xy at z = coord) or 1 (yz at
+        x = coord); coord < 0 places it at the first source."""
+        dims = (ctypes.c_int * 2)()
+        _check(self.L.ard_render_view(self.ctx, view, coord, None, dims))
+        out = np.zeros((dims[1], dims[0]), dtype=np.uint32)
+        _check(self.L.ard_render_view(self.ctx, view, coord,
+                                      out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), dims))
         return out
 
     def pressure_field(self, part):

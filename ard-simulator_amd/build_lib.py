@@ -12,7 +12,8 @@ OUT = os.path.join(HERE, "libard.so")
 OBJ = os.path.join(HERE, "_build")
 
 HIP_SOURCES = ["ard_kernels.hip", "ard_batched_kernels.hip", "ard_batched.hip", "ard_ctx.cpp", "ard_plain.cpp",
-               "ard_adjoint_kernels.hip", "ard_adjoint.cpp", "ard_batch.cpp", "ard_output.hip"]
+               "ard_adjoint_kernels.hip", "ard_adjoint.cpp", "ard_batch.cpp", "ard_output.hip",
+               "ard_view.hip"]
 CXX_SOURCES = ["ard_topology.cpp"]
 HOST_SOURCES = ["host/ard_host.cpp"]
 

@@ -88,6 +88,7 @@ Ctx::~Ctx() {
   destroy_batched(*this);
   destroy_adjoint(*this);
   destroy_output(*this);
+  destroy_view(*this);
   for (void* p : allocs) hipFree(p);
   if (stream) hipStreamDestroy(stream);
 }
@@ -443,6 +444,15 @@ int ard_fetch_recorder_cubes(ard_ctx* ctx, float* out, int t0, int nt) {
   if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync D2H");
   e = hipStreamSynchronize(c->stream);
   return e == hipSuccess ? ARD_OK : hip_fail(e, "hipStreamSynchronize");
+}
+
+int ard_render_view(ard_ctx* ctx, int view, int coord, unsigned int* pixels, int* dims) {
+  Ctx* c = reinterpret_cast<Ctx*>(ctx);
+  if (!c || !dims) return fail(ARD_ERR_INVALID, "null argument");
+  if (view != ARD_VIEW_XY && view != ARD_VIEW_YZ) return fail(ARD_ERR_INVALID, "view must be ARD_VIEW_XY or ARD_VIEW_YZ");
+  if (!c->finalized) return fail(ARD_ERR_STATE, "not finalized");
+  hipSetDevice(c->device);
+  return render_view(*c, view, coord, pixels, dims);
 }
 
 int ard_copy_pressure_field(ard_ctx* ctx, int part, float* out) {

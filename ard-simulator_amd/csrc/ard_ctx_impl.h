@@ -17,6 +17,7 @@ namespace ard {
 struct BatchedState;
 struct AdjState;      // ard_adjoint.cpp
 struct OutputState;   // ard_output.hip
+struct ViewState;     // ard_view.hip
 
 // Dense fields of one DCT partition in the plain form (ARD/dct_partition.h,
 // ARD/dct_volume.h): force F and pressure p in the value domain, modes M[2]
@@ -78,6 +79,7 @@ struct Ctx {
   bool cubes = false;
   float* d_cube = nullptr;
   OutputState* out = nullptr;
+  ViewState* view = nullptr;  // visualisation plane (ard_render_view)
 
   // plain form
   std::vector<DctPlain> dct;       // by partition id
@@ -144,6 +146,10 @@ int check_cubes(const Ctx& c);  // host-side range check
 int setup_cubes(Ctx& c);        // after the form's finalize
 int enqueue_cubes(Ctx& c, int par, hipStream_t st);
 void destroy_output(Ctx& c);
+
+// visualisation plane (ard_view.hip)
+int render_view(Ctx& c, int view, int coord, unsigned int* pixels, int* dims);
+void destroy_view(Ctx& c);
 
 // field access (either form)
 int pressure_field_ptr(Ctx& c, int part, const float** out);

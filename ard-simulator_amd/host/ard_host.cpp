@@ -308,6 +308,31 @@ int Simulation::Update() {
   return t;
 }
 
+int Simulation::size_x() {
+  int dims[2];
+  check(ard_render_view(ctx_, ARD_VIEW_XY, -1, nullptr, dims), "ard_render_view");
+  return dims[0];
+}
+int Simulation::size_y() {
+  int dims[2];
+  check(ard_render_view(ctx_, ARD_VIEW_XY, -1, nullptr, dims), "ard_render_view");
+  return dims[1];
+}
+int Simulation::size_z() {
+  int dims[2];
+  check(ard_render_view(ctx_, ARD_VIEW_YZ, -1, nullptr, dims), "ard_render_view");
+  return dims[1];
+}
+
+std::vector<uint32_t> Simulation::pixels() {
+  const int view = look_from_ == 1 ? ARD_VIEW_YZ : ARD_VIEW_XY;
+  int dims[2];
+  check(ard_render_view(ctx_, view, -1, nullptr, dims), "ard_render_view");
+  std::vector<uint32_t> px(size_t(dims[0]) * size_t(dims[1]));
+  check(ard_render_view(ctx_, view, -1, px.data(), dims), "ard_render_view");
+  return px;
+}
+
 void Simulation::Info() {
   std::cout << "Partitions: " << ard_num_partitions(ctx_) << ", boundaries: " << ard_num_boundaries(ctx_)
             << ", recorders: " << ard_num_recorders(ctx_) << std::endl;

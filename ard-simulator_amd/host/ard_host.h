@@ -16,6 +16,7 @@
 //  * errors throw ard::host::Error instead of asserting or crashing.
 #pragma once
 
+#include <cstdint>
 #include <fstream>
 #include <memory>
 #include <stdexcept>
@@ -203,6 +204,17 @@ class Simulation {
 
   int Update();  // ARD/simulation.cpp:300-317: returns the step index before the update
   void Info();
+  // The visualisation of ARD/simulation.cpp:320-420, drawn on the device when
+  // asked for (pixels()) rather than in every Update: look_from_ 0 is the xy
+  // plane at the first source's z, 1 the yz plane at its x (ARD/simulation.h:50).
+  int look_from_ = 0;
+  int size_x();  // ARD/simulation.h:62-73: bounding box of all partitions, PML slabs included
+  int size_y();
+  int size_z();
+  bool ready() const { return ctx_ != nullptr; }
+  // the pixel array after drawing the current state (ARD/simulation.h:78-81):
+  // size_x * size_y values (xy) or size_y * size_z (yz), RGBA8888 words
+  std::vector<uint32_t> pixels();
   ard_ctx* context() { return ctx_; }
   int time_step() const { return time_step_; }
 

@@ -21,6 +21,10 @@
 //   --c0 V        Simulation::m_c0 in m/s (default 343.5)
 //   --duration S  Simulation::m_duration in seconds (default 2; recorders write
 //                 int(S / dt) rows, sources int(S / dt) samples)
+//   --frame F     after the last step, write the visualisation plane
+//                 (Simulation::pixels(), the reference's window contents) to
+//                 F as a binary PPM image
+//   --look-from V 0: xy plane at the first source's z (default), 1: yz plane at its x
 //   --copy-mib N  first run one device-to-device copy of N MiB: a kernel of known
 //                 traffic, the scale for hardware byte counters
 #include <hip/hip_runtime_api.h>
@@ -42,6 +46,8 @@ using namespace ard::host;
 int main(int argc, char** argv) {
   bool cells = false, timing = false, output = true;
   long long copy_mib = 0;
+  const char* frame = nullptr;
+  int look_from = 0;
   std::vector<const char*> pos;
   for (int i = 1; i < argc; ++i) {
     if (!std::strcmp(argv[i], "--cells")) cells = true;
@@ -51,6 +57,8 @@ int main(int argc, char** argv) {
     else if (!std::strcmp(argv[i], "--timing")) timing = true;
     else if (!std::strcmp(argv[i], "--no-output")) output = false;
     else if (!std::strcmp(argv[i], "--no-cubes")) Simulation::m_record_cubes = 0;
+    else if (!std::strcmp(argv[i], "--frame") && i + 1 < argc) frame = argv[++i];
+    else if (!std::strcmp(argv[i], "--look-from") && i + 1 < argc) look_from = std::atoi(argv[++i]);
     else if (!std::strcmp(argv[i], "--c0") && i + 1 < argc) Simulation::m_c0 = float(std::atof(argv[++i]));
     else if (!std::strcmp(argv[i], "--duration") && i + 1 < argc) Simulation::m_duration = float(std::atof(argv[++i]));
     else if (!std::strcmp(argv[i], "--copy-mib") && i + 1 < argc) copy_mib = std::atoll(argv[++i]);
@@ -64,7 +72,7 @@ int main(int argc, char** argv) {
   if (pos.size() < 2) {
     std::fprintf(stderr,
                  "usage: %s [--cells] [--rec-local] [--plain] [--autotune] [--timing] [--no-output] [--no-cubes] "
-                 "[--c0 V] [--duration S] [--copy-mib N] scene.txt sources.txt [recorders.txt] [steps] [h] [out_root]\n",
+                 "[--c0 V] [--duration S] [--frame F] [--look-from V] [--copy-mib N] scene.txt sources.txt [recorders.txt] [steps] [h] [out_root]\n",
                  argv[0]);
     return 2;
   }
@@ -110,6 +118,22 @@ int main(int argc, char** argv) {
     auto t2 = std::chrono::steady_clock::now();
     const double secs = std::chrono::duration<double>(t2 - t1).count();
     std::cout << "Simulation finished. (" << secs << " s, " << steps << " steps)" << std::endl;
+    if (frame) {
+      sim.look_from_ = look_from;
+      const std::vector<uint32_t> px = sim.pixels();
+      const int cols = look_from == 1 ? sim.size_y() : sim.size_x();
+      const int rows = look_from == 1 ? sim.size_z() : sim.size_y();
+      std::FILE* f = std::fopen(frame, "wb");
+      if (!f) throw Error(std::string("cannot open ") + frame);
+      std::fprintf(f, "P6\n%d %d\n255\n", cols, rows);
+      for (uint32_t v : px) {
+        const unsigned char rgb[3] = {static_cast<unsigned char>(v >> 16), static_cast<unsigned char>(v >> 8),
+                                      static_cast<unsigned char>(v)};
+        std::fwrite(rgb, 1, 3, f);
+      }
+      std::fclose(f);
+      std::cout << "Frame written to " << frame << " (" << cols << " x " << rows << ")" << std::endl;
+    }
     if (timing) {
       static const char* names[ARD_NUM_PHASES] = {"sources", "pml", "fold", "face_fwd1", "face_fwd2", "modal", "unfold",
                                               "face_inv1", "face_inv2", "interface", "record", "dct_fwd", "dct_inv"};

@@ -128,6 +128,23 @@ int ard_set_recorder_cubes(ard_ctx* ctx, int on);
  * 100 (i+5) + 10 (j+5) + (k+5): the reference's write order. Synchronises. */
 int ard_fetch_recorder_cubes(ard_ctx* ctx, float* out, int t0, int nt);
 
+/* The visualisation of Simulation::Update (ARD/simulation.cpp:320-420) on the device: the pressure
+ * on one plane, colour-mapped (white at 0, blue for p >= 0.1, red for p <= -0.1) into the
+ * context's pixel array, which persists between calls as the reference's pixels_ does (cells no
+ * drawn partition covers keep their last colour; zero at first).
+ *   view ARD_VIEW_XY: plane z = coord; pixel (y - y_start) * size_x + (x - x_start)
+ *   view ARD_VIEW_YZ: plane x = coord; pixel (z - z_start) * size_y + (y - y_start)
+ * over the bounding box of all partitions, PML slabs included. coord < 0: the first source's z
+ * (xy) or x (yz), the reference's choice. Partitions are drawn in list order and read the plane
+ * with its global coordinate as their local index, as Partition::get_xy_plane / get_yz_plane
+ * do (ARD/partition.cpp:58-78); cells past a DCT partition's array (undefined there) read 0.
+ * Pixels are SDL_MapRGBA(RGBA8888, 255, r, g, b) values: 0xFF000000 | r << 16 | g << 8 | b.
+ * dims[2] receives (columns, rows): (size_x, size_y) or (size_y, size_z); pixels receives
+ * rows * columns values (pixels NULL: only dims, nothing drawn). Synchronises. */
+#define ARD_VIEW_XY 0
+#define ARD_VIEW_YZ 1
+int ard_render_view(ard_ctx* ctx, int view, int coord, unsigned int* pixels, int* dims);
+
 /* Partition::get_pressure_field (ARD/partition.h:52): the whole pressure volume, w*h*d floats.
  * Synchronises. The batched form evaluates DCT partitions from their modes on demand. */
 int ard_copy_pressure_field(ard_ctx* ctx, int part, float* out);

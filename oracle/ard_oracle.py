@@ -179,3 +179,70 @@ def from_scene(scene, double=False, pml_scale=None):
         o.add_recorder(*r)
     o.finalize()
     return o
+
+
+def colour(p):
+    """Pixel of one pressure value in Simulation::Update's visualisation
+    (ARD/simulation.cpp:346-364), FP32 operations in the reference's order:
+    norm = 0.5 * std::max(-1, std::min(1, 10 p)) + 0.5; norm >= 0.5 gives
+    r = g = 255 - round(255 * 2 * (norm - 0.5)), b = 255, else r = 255,
+    g = b = 255 - round(255 * (1 - 2 norm)); SDL_MapRGBA(RGBA8888, 255, r, g, b)
+    = 0xFF000000 | r << 16 | g << 8 | b. round() is half away from zero."""
+    f = np.float32
+    v = np.asarray(p, np.float32) * f(10)
+    v = np.where(v < f(1), v, f(1)).astype(np.float32)      # std::min(1.0f, v): (v < 1) ? v : 1
+    v = np.where(f(-1) < v, v, f(-1)).astype(np.float32)    # std::max(-1.0f, v): (-1 < v) ? v : -1
+    norm = (f(0.5) * v + f(0.5)).astype(np.float32)
+    hi = norm >= f(0.5)
+
+    def rnd(x):  # C round() of a non-negative float
+        return np.floor(np.asarray(x, np.float64) + 0.5)
+
+    a = 255 - rnd((f(255) * f(2)) * (norm - f(0.5)))
+    b = 255 - rnd(f(255) * (f(1) - f(2) * norm))
+    r = np.where(hi, a, 255).astype(np.uint32)
+    g = np.where(hi, a, b).astype(np.uint32)
+    bl = np.where(hi, 255, b).astype(np.uint32)
+    return (np.uint32(0xFF000000) | (r << 16) | (g << 8) | bl).astype(np.uint32)
+
+
+def render(parts, fields, view, coord, pixels=None):
+    """The visualisation step of Simulation::Update (ARD/simulation.cpp:320-420).
+
+    parts: partition info tuples (is_pml, type, parent, x0, y0, z0, w, h, d)
+    in list order; fields: each partition's pressure [d][h][w]; view 0 draws
+    the xy plane z = coord, view 1 the yz plane x = coord. A partition is drawn
+    when start <= coord <= end along the view axis, reading its plane with
+    coord as its local index (Partition::get_xy_plane / get_yz_plane,
+    ARD/partition.cpp:58-78): a DCT partition through its flat array (cells
+    past the array, undefined in the reference, read 0), a PML slab through
+    GetIndex (out of range reads the zero sentinel, ARD/pml_partition.cpp:6-15).
+    Pixels go to (y - y0) * size_x + (x - x0) (xy) or (z - z0) * size_y +
+    (y - y0) (yz) of the persistent array `pixels` (zeros if None), sized
+    max(size_x, size_z) * size_y. Returns (pixels, columns, rows)."""
+    x0 = min(p[3] for p in parts); y0 = min(p[4] for p in parts); z0 = min(p[5] for p in parts)
+    x1 = max(p[3] + p[6] for p in parts); y1 = max(p[4] + p[7] for p in parts)
+    z1 = max(p[5] + p[8] for p in parts)
+    sx, sy, sz = x1 - x0, y1 - y0, z1 - z0
+    if pixels is None:
+        pixels = np.zeros(max(sx, sz) * sy, np.uint32)
+    for p, fld in zip(parts, fields):
+        is_pml, _, _, px, py, pz, w, h, d = p
+        lo, hi = (pz, pz + d) if view == 0 else (px, px + w)
+        if lo > coord or hi < coord:
+            continue
+        flat = np.asarray(fld, np.float32).ravel()
+        rows, cols = (h, w) if view == 0 else (d, h)
+        i = np.arange(rows)[:, None]
+        j = np.arange(cols)[None, :]
+        idx = coord * h * w + i * w + j if view == 0 else i * h * w + j * w + coord
+        ok = (idx >= 0) & (idx < flat.size)
+        if is_pml:
+            ok &= (0 <= coord) & (coord < (d if view == 0 else w))
+        val = np.where(ok, flat[np.clip(idx, 0, flat.size - 1)], np.float32(0)).astype(np.float32)
+        if view == 0:
+            pos = (py - y0 + i) * sx + (px - x0 + j)
+        else:
+            pos = (pz - z0 + i) * sy + (py - y0 + j)
+        pixels[pos.ravel()] = colour(val).ravel()
+    return pixels, (sx, sy) if view == 0 else (sy, sz)

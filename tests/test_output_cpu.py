@@ -133,3 +133,66 @@ def test_cube_outside_the_array_is_an_error():
     sim = A.Simulation(sc, device=-1, finalize=False, cubes=True)
     with pytest.raises(A.ArdError, match="host-only"):
         A._check(sim.L.ard_finalize(sim.ctx))
+
+
+# --- visualisation plane (ARD/simulation.cpp:320-420): the oracle's restatement
+
+
+def test_colour_closed_forms():
+    """White at rest, saturated blue / red beyond |p| = 0.1, the half-way
+    shades at |p| = 0.05 (255 - round(127.5) = 127), NaN as std::min/max pass
+    it (to norm = 1)."""
+    got = O.colour(np.array([0.0, 0.1, 7.0, -0.1, -3.0, 0.05, -0.05, np.nan], np.float32))
+    want = [0xFFFFFFFF, 0xFF0000FF, 0xFF0000FF, 0xFFFF0000, 0xFFFF0000, 0xFF7F7FFF, 0xFFFF7F7F, 0xFF0000FF]
+    assert [int(v) for v in got] == want
+    # 10 p = 0.0098: norm - 0.5 = 0.0049 -> 510 * 0.0049 = 2.499 -> 2 -> 253
+    assert int(O.colour(np.float32(0.00098))) == 0xFFFDFDFF
+    # positive pressures fade from white to blue monotonically
+    p = np.linspace(0, 0.1, 2001, dtype=np.float32)
+    rg = (O.colour(p) >> 16) & 0xFF
+    assert np.all(np.diff(rg.astype(int)) <= 0) and rg[0] == 255 and rg[-1] == 0
+    assert np.all(O.colour(p) & 0xFF == 255)
+
+
+def _layout():
+    parts = [(0, -1, -1, 0, 0, 0, 4, 3, 2), (0, -1, -1, 4, 0, 0, 4, 3, 2), (1, 4, 0, 0, 0, -1, 4, 3, 1)]
+    z, y, x = np.meshgrid(np.arange(2), np.arange(3), np.arange(4), indexing="ij")
+    f0 = (0.001 * (100 * z + 10 * y + x)).astype(np.float32)
+    f1 = -f0
+    f2 = np.full((1, 3, 4), 0.05, np.float32)
+    return parts, [f0, f1, f2]
+
+
+def test_render_xy_plane_positions_and_order():
+    parts, fields = _layout()
+    c = lambda v: int(O.colour(np.float32(v)))
+    # z = 1: both DCT boxes (z in [0, 2]) at local z 1; the PML slab (z in [-1, 0]) is skipped
+    pix, (cols, rows) = O.render(parts, fields, 0, 1)
+    assert (cols, rows) == (8, 3) and pix.size == 8 * 3
+    assert pix[0 * 8 + 0] == c(0.100) and pix[2 * 8 + 3] == c(0.123)
+    assert pix[1 * 8 + 4] == c(-0.110) and pix[2 * 8 + 7] == c(-0.123)
+    # z = 0: the slab is drawn after the DCT boxes and covers box 0's pixels
+    pix, _ = O.render(parts, fields, 0, 0)
+    assert pix[1 * 8 + 2] == c(0.05) and pix[1 * 8 + 6] == c(-0.012)
+    # z = 2 = the boxes' end: drawn, reading past their arrays (0 here: white)
+    pix, _ = O.render(parts, fields, 0, 2)
+    assert np.all(pix == 0xFFFFFFFF)
+    # z = 3: nothing drawn; a persistent array keeps its previous colours
+    prev = np.full(8 * 3, 7, np.uint32)
+    pix, _ = O.render(parts, fields, 0, 3, pixels=prev.copy())
+    assert np.all(pix == 7)
+
+
+def test_render_yz_plane_reads_global_x_as_local_index():
+    parts, fields = _layout()
+    c = lambda v: int(O.colour(np.float32(v)))
+    # x = 5 lies in box 1 (x in [4, 8]), which reads local x 5 >= w = 4: flat
+    # index i*12 + j*4 + 5, i.e. cell (z, y + 1, x 1) while inside the array
+    pix, (cols, rows) = O.render(parts, fields, 1, 5)
+    assert (cols, rows) == (3, 3)
+    zoff = 1  # z starts at -1 (the slab)
+    assert pix[(zoff + 0) * 3 + 0] == c(-0.011)   # flat 5: z 0, y 1, x 1
+    assert pix[(zoff + 0) * 3 + 2] == c(-0.101)   # flat 13: z 1, y 0, x 1
+    assert pix[(zoff + 1) * 3 + 2] == 0xFFFFFFFF  # flat 25 >= 24: past the array
+    # the slab (x in [0, 4]) and box 0 are not drawn; their rows stay zero
+    assert np.all(pix[:3] == 0)
