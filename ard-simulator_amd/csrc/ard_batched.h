@@ -9,10 +9,15 @@
 namespace ard {
 
 // sum / difference of the face pair planes (x, n-1-x), x = 0..2
+// Fold and unfold walk their planes in items of `vec` consecutive cells (4 when
+// the plane length and every pointer allow 16-byte accesses, else 1); each
+// descriptor's items start on a 256-item boundary of the grid (block-uniform
+// descriptor lookup).
 struct FoldDesc {
   const float* src;  // [6][plane] raw face slab, plane q = x for q < 3, n-6+q for q >= 3
   float* dst;        // [6][plane]: q < 3 sum, q >= 3 difference
   long long plane;
+  int vec;
 };
 
 struct UnfoldDesc {
@@ -20,6 +25,7 @@ struct UnfoldDesc {
   float* dst;         // [6][plane] in face-pair order
   long long plane;
   int nchunk;
+  int vec;
 };
 
 // 1D factors of a point in mode space: fx[W], fy[H], fz[D]; for a source the

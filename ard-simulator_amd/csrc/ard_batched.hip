@@ -26,6 +26,14 @@ namespace {
 long long ceil_div(long long a, long long b) { return (a + b - 1) / b; }
 const double kPi = 3.14159265358979323846;
 
+// 4-cell items when the plane length and the pointers allow 16-byte accesses
+int vec_of(long long plane, std::initializer_list<const void*> ptrs) {
+  if (plane % 4) return 1;
+  for (const void* p : ptrs)
+    if (reinterpret_cast<uintptr_t>(p) % 16) return 1;
+  return 4;
+}
+
 }  // namespace
 
 int apply_tiling(Ctx& c, int rpw, int kzc);
@@ -389,11 +397,12 @@ int apply_tiling(Ctx& c, int rpw, int kzc) {
     P.kz_chunk = int(ceil_div(ceil_div(p.d, P.nbz), 2) * 2);  // even: each block starts on an even plane
     P.nbz = int(ceil_div(p.d, P.kz_chunk));
     for (int w = 0; w < 3; ++w)
-      unfolds.push_back(UnfoldDesc{P.T[w], P.S[w], P.plane[w], w == 0 ? P.nbx : (w == 1 ? P.nby : P.nbz)});
+      unfolds.push_back(UnfoldDesc{P.T[w], P.S[w], P.plane[w], w == 0 ? P.nbx : (w == 1 ? P.nby : P.nbz),
+                                   vec_of(P.plane[w], {P.T[w], P.S[w]})});
   }
   std::vector<long long> su;
   long long t = 0;
-  for (auto& u : unfolds) { su.push_back(t); t += 3 * u.plane; }
+  for (auto& u : unfolds) { su.push_back(t); t += ceil_div(3 * u.plane / u.vec, 256) * 256; }
   B.nunfold = int(unfolds.size());
   B.unfold_total = t;
   if ((r = upload_vec(c, (void**)&B.d_unfold, unfolds.data(), unfolds.size() * sizeof(UnfoldDesc)))) return r;
@@ -512,7 +521,7 @@ int finalize_batched(Ctx& c) {
       if ((r = upload_vec(c, (void**)&P.yz[ax - 1], t.data(), t.size() * 4))) return r;
     }
     if ((r = c.dalloc((void**)&P.rec_part, size_t(P.nbx) * P.nby * P.nbz * ARD_MAX_REC_PER_PART * 4))) return r;
-    for (int w = 0; w < 3; ++w) folds.push_back(FoldDesc{P.F[w], P.S[w], P.plane[w]});
+    for (int w = 0; w < 3; ++w) folds.push_back(FoldDesc{P.F[w], P.S[w], P.plane[w], vec_of(P.plane[w], {P.F[w], P.S[w]})});
     const float *fw, *iw, *fh, *ih, *fd, *id;
     if ((r = c.cos_tables(p.w, &fw, &iw)) || (r = c.cos_tables(p.h, &fh, &ih)) || (r = c.cos_tables(p.d, &fd, &id)))
       return r;
@@ -564,7 +573,7 @@ int finalize_batched(Ctx& c) {
   {
     std::vector<long long> st;
     long long t = 0;
-    for (auto& f : folds) { st.push_back(t); t += 3 * f.plane; }
+    for (auto& f : folds) { st.push_back(t); t += ceil_div(3 * f.plane / f.vec, 256) * 256; }
     B.nfold = int(folds.size());
     B.fold_total = t;
     if ((r = upload_vec(c, (void**)&B.d_fold, folds.data(), folds.size() * sizeof(FoldDesc)))) return r;
@@ -696,9 +705,11 @@ int build_graphs(Ctx& c) {
 }  // namespace
 
 
-// The autotuner: times the modal pass for each tiling candidate and the face
-// transforms for each kernel variant on this scene, keeps the fastest. Runs at
-// finalize, when the state is all zeros, so the timed launches change nothing.
+// The autotuner: times the modal pass with the unfold kernel that sums its
+// partial reductions (the tiling sets how many partials there are) for each
+// tiling candidate, and the face transforms for each kernel variant, on this
+// scene; keeps the fastest. Runs at finalize, when the state is all zeros, so
+// the timed launches change nothing.
 int autotune(Ctx& c) {
   BatchedState& B = *c.bs;
   hipEvent_t e0, e1;
@@ -717,11 +728,12 @@ int autotune(Ctx& c) {
   int best_rpw = 4, best_kzc = 48, tried = 0;
   float best = 1e30f;
   int r = ARD_OK;
-  for (int rpw : {4, 6}) {
+  for (int rpw : {4, 6, 8}) {
     for (int kzc : {16, 32, 48, 96}) {
       if ((r = apply_tiling(c, rpw, kzc))) break;
       const float ms = time_ms([&] {
         for (auto& g : B.modal) launch_modal(g.rpw, g.pts, g.d_desc[0], g.d_start, g.n, g.blocks, 0, c.stream);
+        launch_unfold(B.d_unfold, B.d_unfold_start, B.nunfold, B.unfold_total, c.stream);
       }, 3);
       tried++;
       if (ms < best) { best = ms; best_rpw = rpw; best_kzc = kzc; }

@@ -36,25 +36,35 @@ __device__ __forceinline__ int find_desc_ll(const long long* __restrict__ starts
 }
 }  // namespace
 
+typedef float v4f __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(1))) v4f gv4f;
+
 // ---------------------------------------------------------------------------
 // Fold: face pair (x, n-1-x) -> sum / difference planes, x = 0..2.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_fold(const FoldDesc* __restrict__ d, const long long* __restrict__ starts,
                                               int n) {
-  const long long e = (long long)blockIdx.x * 256 + threadIdx.x;
-  const int i = find_desc_ll(starts, n, e);
+  const long long b0 = (long long)blockIdx.x * 256;
+  const int i = find_desc_ll(starts, n, b0);  // block-uniform
   const FoldDesc& F = d[i];
-  const long long le = e - starts[i];
-  if (le >= 3 * F.plane) return;
-  const unsigned pl = unsigned(F.plane);  // planes hold < 2^31 cells
-  const int q = int(unsigned(le) / pl);
-  const long long o = unsigned(le) - unsigned(q) * pl;
+  const long long it = b0 - starts[i] + threadIdx.x;
+  const long long pl = F.plane;
+  if (it * F.vec >= 3 * pl) return;
+  const long long cell = it * F.vec;
+  const int q = int(cell / pl);
+  const long long o = cell - q * pl;
   const gfloat* src = ARD_CG(F.src);
   gfloat* dst = ARD_G(F.dst);
-  const float a = src[q * F.plane + o];
-  const float b = src[(5 - q) * F.plane + o];
-  dst[q * F.plane + o] = a + b;
-  dst[(3 + q) * F.plane + o] = a - b;
+  if (F.vec == 4) {
+    const v4f a = *reinterpret_cast<const gv4f*>(src + q * pl + o);
+    const v4f b = *reinterpret_cast<const gv4f*>(src + (5 - q) * pl + o);
+    *reinterpret_cast<gv4f*>(dst + q * pl + o) = v4f{a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w};
+    *reinterpret_cast<gv4f*>(dst + (3 + q) * pl + o) = v4f{a.x - b.x, a.y - b.y, a.z - b.z, a.w - b.w};
+  } else {
+    const float a = src[q * pl + o], b = src[(5 - q) * pl + o];
+    dst[q * pl + o] = a + b;
+    dst[(3 + q) * pl + o] = a - b;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -63,22 +73,50 @@ __global__ __launch_bounds__(256) void k_fold(const FoldDesc* __restrict__ d, co
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_unfold(const UnfoldDesc* __restrict__ d,
                                                 const long long* __restrict__ starts, int n) {
-  const long long e = (long long)blockIdx.x * 256 + threadIdx.x;
-  const int i = find_desc_ll(starts, n, e);
+  const long long b0 = (long long)blockIdx.x * 256;
+  const int i = find_desc_ll(starts, n, b0);  // block-uniform
   const UnfoldDesc& U = d[i];
-  const long long le = e - starts[i];
-  if (le >= 3 * U.plane) return;
-  const unsigned pl = unsigned(U.plane);  // planes hold < 2^31 cells
-  const int q = int(unsigned(le) / pl);
-  const long long o = unsigned(le) - unsigned(q) * pl;
-  float ev = 0.f, od = 0.f;
-  for (int c = 0; c < U.nchunk; ++c) {
-    const gfloat* p = ARD_CG(U.part) + (long long)c * 6 * U.plane;
-    ev += p[q * U.plane + o];
-    od += p[(3 + q) * U.plane + o];
+  const long long it = b0 - starts[i] + threadIdx.x;
+  const long long pl = U.plane;
+  if (it * U.vec >= 3 * pl) return;
+  const long long cell = it * U.vec;
+  const int q = int(cell / pl);
+  const long long o = cell - q * pl;
+  const gfloat* p = ARD_CG(U.part) + q * pl + o;
+  gfloat* dst = ARD_G(U.dst);
+  const long long cs = 6 * pl;  // chunk stride
+  const int nc = U.nchunk;
+  if (U.vec == 4) {
+    v4f ev = v4f{0.f, 0.f, 0.f, 0.f}, od = ev;
+    int c = 0;
+    for (; c + 1 < nc; c += 2) {  // two chunks' loads in flight per iteration
+      const v4f e0 = *reinterpret_cast<const gv4f*>(p + c * cs);
+      const v4f o0 = *reinterpret_cast<const gv4f*>(p + c * cs + 3 * pl);
+      const v4f e1 = *reinterpret_cast<const gv4f*>(p + (c + 1) * cs);
+      const v4f o1 = *reinterpret_cast<const gv4f*>(p + (c + 1) * cs + 3 * pl);
+      ev.x += e0.x; ev.y += e0.y; ev.z += e0.z; ev.w += e0.w;
+      od.x += o0.x; od.y += o0.y; od.z += o0.z; od.w += o0.w;
+      ev.x += e1.x; ev.y += e1.y; ev.z += e1.z; ev.w += e1.w;
+      od.x += o1.x; od.y += o1.y; od.z += o1.z; od.w += o1.w;
+    }
+    if (c < nc) {
+      const v4f e0 = *reinterpret_cast<const gv4f*>(p + c * cs);
+      const v4f o0 = *reinterpret_cast<const gv4f*>(p + c * cs + 3 * pl);
+      ev.x += e0.x; ev.y += e0.y; ev.z += e0.z; ev.w += e0.w;
+      od.x += o0.x; od.y += o0.y; od.z += o0.z; od.w += o0.w;
+    }
+    *reinterpret_cast<gv4f*>(dst + q * pl + o) = v4f{ev.x + od.x, ev.y + od.y, ev.z + od.z, ev.w + od.w};
+    *reinterpret_cast<gv4f*>(dst + (5 - q) * pl + o) =
+        v4f{ev.x - od.x, ev.y - od.y, ev.z - od.z, ev.w - od.w};
+  } else {
+    float ev = 0.f, od = 0.f;
+    for (int c = 0; c < nc; ++c) {
+      ev += p[c * cs];
+      od += p[c * cs + 3 * pl];
+    }
+    dst[q * pl + o] = ev + od;
+    dst[(5 - q) * pl + o] = ev - od;
   }
-  ARD_G(U.dst)[q * U.plane + o] = ev + od;
-  ARD_G(U.dst)[(5 - q) * U.plane + o] = ev - od;
 }
 
 // ---------------------------------------------------------------------------
@@ -131,15 +169,6 @@ __device__ __forceinline__ float sinc2_poly(float u) {
   p = fmaf(p, u, -3.3333334e-1f);
   return fmaf(p, u, 1.0f);
 }
-
-// Workgroups are dealt to the 8 XCDs in turn (block b to XCD b % 8). This
-// renumbering gives each XCD one contiguous range of logical blocks, so tiles
-// that share face spectra and cache lines share that XCD's L2.
-__device__ __forceinline__ int xcd_block(int bid, int n) {
-  const int x = bid & 7, i = bid >> 3, q = n >> 3, r = n & 7;
-  return x * q + min(x, r) + i;
-}
-
 typedef __amdgpu_buffer_rsrc_t brsrc;
 // raw buffer resource over n bytes (gfx9 word 3: 32-bit float data, no swizzle)
 __device__ __forceinline__ brsrc make_rsrc(const void* p, unsigned nbytes) {

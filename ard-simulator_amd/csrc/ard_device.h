@@ -7,6 +7,14 @@
 
 namespace ard {
 
+// Workgroups are dealt to the 8 XCDs in turn (block b to XCD b % 8). This
+// renumbering gives each XCD one contiguous range of logical blocks, so tiles
+// that share face spectra and cache lines share that XCD's L2.
+__device__ __forceinline__ int xcd_block(int bid, int n) {
+  const int x = bid & 7, i = bid >> 3, q = n >> 3, r = n & 7;
+  return x * q + min(x, r) + i;
+}
+
 // Pointers read from descriptor tables are generic pointers to the compiler,
 // which then emits flat memory instructions; these casts state that they
 // point to global memory, so it can use global and scalar loads.

@@ -202,8 +202,7 @@ void Recorder::RecordField(int time_step) {
   real_t v;
   std::vector<real_t> cube;
   const bool has_cube = sample(time_step, &v, output_.is_open() ? &cube : nullptr);
-  if (output_.is_open()) {
-    if (!has_cube) throw Error("recorder cubes are off in this Simulation (Simulation::m_record_cubes)");
+  if (output_.is_open() && has_cube) {  // no cube rows when the Simulation samples none (m_record_cubes 0)
     for (real_t c : cube) output_ << c << " ";
     output_ << std::endl;
   }

@@ -151,7 +151,8 @@ class Recorder {
   // ARD/recorder.cpp:40-57: for time_step < total_steps, appends the 10^3 cube
   // around the recorder to out_<id>.txt and the point sample to
   // response_<id>.txt, in the reference's text format (operator<< on float).
-  // Registered with a Simulation, both come from the device traces.
+  // Registered with a Simulation, both come from the device traces (no cube
+  // rows if the Simulation was built with m_record_cubes = 0).
   void RecordField(int time_step = 0);
   // ARD/recorder.cpp:59-65 (time_step <= total_steps, point sample only)
   void RecordResponse(int time_step = 0);

@@ -15,6 +15,8 @@
 //                 partition's array)
 //   --plain       per-partition form (default: batched)
 //   --autotune    batched form: time the tiling candidates before stepping
+//   --transform V transform kernels (ard_set_transform_variant: 0 vector FMA,
+//                 1 matrix cores, 2 BF16 split, 3 matrix cores even/odd folded)
 //   --timing      launch each phase directly and print per-phase kernel times
 //   --no-output   step without writing recorder files (no per-step device sync)
 //   --no-cubes    recorders write response_<id>.txt only (no 10^3 cube sampling)
@@ -54,6 +56,12 @@ int main(int argc, char** argv) {
     else if (!std::strcmp(argv[i], "--rec-local")) Simulation::m_recorder_mode = ARD_REC_LOCAL;
     else if (!std::strcmp(argv[i], "--plain")) Simulation::m_form = ARD_FORM_PLAIN;
     else if (!std::strcmp(argv[i], "--autotune")) Simulation::m_autotune = 1;
+    else if (!std::strcmp(argv[i], "--transform") && i + 1 < argc) {
+      if (ard_set_transform_variant(std::atoi(argv[++i])) != ARD_OK) {
+        std::fprintf(stderr, "%s\n", ard_last_error());
+        return 2;
+      }
+    }
     else if (!std::strcmp(argv[i], "--timing")) timing = true;
     else if (!std::strcmp(argv[i], "--no-output")) output = false;
     else if (!std::strcmp(argv[i], "--no-cubes")) Simulation::m_record_cubes = 0;
@@ -71,7 +79,7 @@ int main(int argc, char** argv) {
   }
   if (pos.size() < 2) {
     std::fprintf(stderr,
-                 "usage: %s [--cells] [--rec-local] [--plain] [--autotune] [--timing] [--no-output] [--no-cubes] "
+                 "usage: %s [--cells] [--rec-local] [--plain] [--autotune] [--transform V] [--timing] [--no-output] [--no-cubes] "
                  "[--c0 V] [--duration S] [--frame F] [--look-from V] [--copy-mib N] scene.txt sources.txt [recorders.txt] [steps] [h] [out_root]\n",
                  argv[0]);
     return 2;
