@@ -358,7 +358,8 @@ int build_modal(Ctx& c) {
 }
 
 // Last-writer masks of the grouped interface launch (final_writer_masks),
-// uploaded with each boundary's element offset.
+// uploaded with each boundary's element offset (a multiple of the 256-thread
+// block, which k_interface's per-block descriptor lookup relies on).
 int build_masks(Ctx& c, std::vector<BndDesc>& descs, const std::vector<long long>& elems) {
   BatchedState& B = *c.bs;
   std::vector<uint8_t> mask;
@@ -371,7 +372,7 @@ int build_masks(Ctx& c, std::vector<BndDesc>& descs, const std::vector<long long
   for (int b = 0; b < int(descs.size()); ++b) {
     descs[b].mask = dm + mstart[b];
     descs[b].elem0 = e0;
-    e0 += elems[b];
+    e0 += ceil_div(elems[b], 256) * 256;
   }
   B.bnd_total = e0;
   return ARD_OK;

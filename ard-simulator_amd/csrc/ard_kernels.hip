@@ -702,18 +702,20 @@ __device__ __forceinline__ void write_f(const PartAccess& A, int x, int y, int z
 __global__ __launch_bounds__(256) void k_interface(const BndDesc* __restrict__ descs, int ndesc,
                                                    const PartAccess* __restrict__ parts, int par,
                                                    long long total) {
-  const long long e = (long long)blockIdx.x * 256 + threadIdx.x;
-  if (e >= total) return;
+  // each boundary's elements start on a block boundary: one lookup per block
+  const long long b0 = (long long)blockIdx.x * 256;
   int lo = 0, hi = ndesc - 1;
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
-    if (descs[mid].elem0 <= e) lo = mid;
+    if (descs[mid].elem0 <= b0) lo = mid;
     else hi = mid - 1;
   }
   const BndDesc& B = descs[lo];
+  const long long e = b0 + threadIdx.x;
   // elements of one boundary write distinct cells, so their order is free:
   // the first tangential coordinate runs fastest (contiguous in the face slabs)
   const unsigned le = unsigned(e - B.elem0);
+  if (e >= total || le >= unsigned(B.t0n) * unsigned(B.t1n)) return;
   const unsigned lv = le / unsigned(B.t0n);
   const int u = B.t0s + int(le - lv * unsigned(B.t0n));
   const int v = B.t1s + int(lv);
