@@ -177,6 +177,10 @@ __device__ __forceinline__ brsrc make_rsrc(const void* p, unsigned nbytes) {
 __device__ __forceinline__ float bload(brsrc r, unsigned voff, unsigned soff) {
   return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, int(voff), int(soff), 0));
 }
+// streamed once per step (the mode state): non-temporal ("nt") loads
+__device__ __forceinline__ float bload_nt(brsrc r, unsigned voff, unsigned soff) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, int(voff), int(soff), 2));
+}
 
 // Modal pass of the batched form, one block per (64 kx) x (4 RPW ky) x (kz
 // chunk) tile of one partition. For every mode:
@@ -343,8 +347,8 @@ __global__ __launch_bounds__(256) void k_modal(const ModalDesc* __restrict__ des
     const unsigned po = unsigned(kz * HWp);
 #pragma unroll
     for (int r = 0; r < RPW; ++r) {
-      in.mc[r] = bload(rMc, vx, (po + unsigned(rom[r])) * 4u);
-      in.mp[r] = bload(rMp, vx, (po + unsigned(rom[r])) * 4u);
+      in.mc[r] = bload_nt(rMc, vx, (po + unsigned(rom[r])) * 4u);
+      in.mp[r] = bload_nt(rMp, vx, (po + unsigned(rom[r])) * 4u);
     }
 #pragma unroll
     for (int q = 0; q < 6; ++q) in.gy[q] = bload(rGy, vx, unsigned(q * DW + kz * W) * 4u);
@@ -393,7 +397,7 @@ __global__ __launch_bounds__(256) void k_modal(const ModalDesc* __restrict__ des
       }
       const float mv = fmaf(a, in.mc[r], fmaf(-0.999f, in.mp[r], g * fh));
       const bool ok = rok[r] && xok;
-      if (ok) Mp[po + unsigned(rom[r]) + unsigned(kx)] = mv;
+      if (ok) __builtin_nontemporal_store(mv, &Mp[po + unsigned(rom[r]) + unsigned(kx)]);  // streamed, not reused in this step
       const float mn = ok ? mv : 0.f;
       s_m[kl][row][lane] = mn;
       tz[r][PZ * 3 + 0] = fmaf(za.w, mn, tz[r][PZ * 3 + 0]);

@@ -24,7 +24,7 @@ template <bool B_K_CONTIG, bool C_M_CONTIG, int EP>
 __global__ __launch_bounds__(256) void k_gemm(const GemmDesc* __restrict__ descs, int ndesc) {
   __shared__ float As[2][16][68];
   __shared__ float Bs[2][16][68];
-  const int tile = blockIdx.x;
+  const int tile = xcd_block(int(blockIdx.x), int(gridDim.x));  // contiguous tile ranges per XCD
   int lo = 0, hi = ndesc - 1;
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
@@ -34,8 +34,10 @@ __global__ __launch_bounds__(256) void k_gemm(const GemmDesc* __restrict__ descs
   const GemmDesc& g = descs[lo];
   const int M = g.M, N = g.N, K = g.K, N1 = g.N1;
   const int t = tile - g.tile0;
-  const int m0 = (t / g.tiles_n) * 64;
-  const int n0 = (t % g.tiles_n) * 64;
+  // m tiles fastest: the tiles of one B panel run together and share it in L2
+  const int tiles_m = (M + 63) >> 6;
+  const int m0 = (t % tiles_m) * 64;
+  const int n0 = (t / tiles_m) * 64;
   const int tid = threadIdx.x;
 
   // B element offsets of the columns this thread loads (fixed over k)
@@ -156,7 +158,7 @@ template <bool B_K_CONTIG, int EP>
 __global__ __launch_bounds__(256) void k_gemm_mfma(const GemmDesc* __restrict__ descs, int ndesc) {
   __shared__ float As[2][16][68];
   __shared__ float Bs[2][16][68];
-  const int tile = blockIdx.x;
+  const int tile = xcd_block(int(blockIdx.x), int(gridDim.x));  // contiguous tile ranges per XCD
   int lo = 0, hi = ndesc - 1;
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
@@ -166,8 +168,10 @@ __global__ __launch_bounds__(256) void k_gemm_mfma(const GemmDesc* __restrict__ 
   const GemmDesc& g = descs[lo];
   const int M = g.M, N = g.N, K = g.K, N1 = g.N1;
   const int t = tile - g.tile0;
-  const int m0 = (t / g.tiles_n) * 64;
-  const int n0 = (t % g.tiles_n) * 64;
+  // m tiles fastest: the tiles of one B panel run together and share it in L2
+  const int tiles_m = (M + 63) >> 6;
+  const int m0 = (t % tiles_m) * 64;
+  const int n0 = (t / tiles_m) * 64;
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = tid >> 6;
@@ -233,17 +237,26 @@ __global__ __launch_bounds__(256) void k_gemm_mfma(const GemmDesc* __restrict__ 
     for (int kk = 0; kk < 16; kk += 2) {
       const float a = As[buf][kk + (lane >> 5)][wm + (lane & 31)];
       const float b = Bs[buf][kk + (lane >> 5)][wn + (lane & 31)];
-      acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, acc, 0, 0, 0);
+      // K-contiguous B: in the transform passes C is then contiguous along
+      // m, so the instruction's columns (lane % 32) take m: C^T = B^T A^T
+      acc = B_K_CONTIG ? __builtin_amdgcn_mfma_f32_32x32x2f32(b, a, acc, 0, 0, 0)
+                       : __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, acc, 0, 0, 0);
     }
   }
-  const int j = n0 + wn + (lane & 31);
-  if (j >= N) return;
-  const long long cn = (long long)(j % N1) * g.c1 + (long long)(j / N1) * g.c2;
+  // accumulator r of this lane: (m, n) = (wm + lane % 32, wn + row(r)) when
+  // B is K-contiguous, else (wm + row(r), wn + lane % 32), with
+  // row(r) = 8 (r / 4) + 4 (lane / 32) + r % 4
+  const int jl = (B_K_CONTIG ? m0 + wm : n0 + wn) + (lane & 31);
+  if (jl >= (B_K_CONTIG ? M : N)) return;
+  long long cn = 0;
+  if (!B_K_CONTIG) cn = (long long)(jl % N1) * g.c1 + (long long)(jl / N1) * g.c2;
   gfloat* __restrict__ C = ARD_G(g.C);
 #pragma unroll
   for (int r = 0; r < 16; ++r) {
-    const int i = m0 + wm + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-    if (i >= M) continue;
+    const int il = (B_K_CONTIG ? n0 + wn : m0 + wm) + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+    if (il >= (B_K_CONTIG ? N : M)) continue;
+    const int i = B_K_CONTIG ? jl : il;
+    if (B_K_CONTIG) cn = (long long)(il % N1) * g.c1 + (long long)(il / N1) * g.c2;
     const long long o = (long long)i * g.cM + cn;
     const float v = g.alpha * acc[r];
     if (EP == 0) C[o] = v;
@@ -273,7 +286,7 @@ template <bool B_K_CONTIG, int EP>
 __global__ __launch_bounds__(256) void k_gemm_bf16x3(const GemmDesc* __restrict__ descs, int ndesc) {
   __shared__ float As[2][16][68];
   __shared__ float Bs[2][16][68];
-  const int tile = blockIdx.x;
+  const int tile = xcd_block(int(blockIdx.x), int(gridDim.x));  // contiguous tile ranges per XCD
   int lo = 0, hi = ndesc - 1;
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
@@ -283,8 +296,10 @@ __global__ __launch_bounds__(256) void k_gemm_bf16x3(const GemmDesc* __restrict_
   const GemmDesc& g = descs[lo];
   const int M = g.M, N = g.N, K = g.K, N1 = g.N1;
   const int t = tile - g.tile0;
-  const int m0 = (t / g.tiles_n) * 64;
-  const int n0 = (t % g.tiles_n) * 64;
+  // m tiles fastest: the tiles of one B panel run together and share it in L2
+  const int tiles_m = (M + 63) >> 6;
+  const int m0 = (t % tiles_m) * 64;
+  const int n0 = (t / tiles_m) * 64;
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = tid >> 6;
@@ -397,7 +412,7 @@ __global__ __launch_bounds__(256) void k_gemm_mfma_fold(const GemmDesc* __restri
   constexpr int NS = FOLD == 2 ? 2 : 1;  // operand sets (E, O)
   __shared__ float As[2][NS][16][68];
   __shared__ float Bs[2][NS][16][68];
-  const int tile = blockIdx.x;
+  const int tile = xcd_block(int(blockIdx.x), int(gridDim.x));  // contiguous tile ranges per XCD
   int lo = 0, hi = ndesc - 1;
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
@@ -407,8 +422,10 @@ __global__ __launch_bounds__(256) void k_gemm_mfma_fold(const GemmDesc* __restri
   const GemmDesc& g = descs[lo];
   const int M = g.M, N = g.N, K = g.K, N1 = g.N1, L = g.L;
   const int t = tile - g.tile0;
-  const int m0 = (t / g.tiles_n) * 64;
-  const int n0 = (t % g.tiles_n) * 64;
+  // m tiles fastest: the tiles of one B panel run together and share it in L2
+  const int tiles_m = (M + 63) >> 6;
+  const int m0 = (t % tiles_m) * 64;
+  const int n0 = (t / tiles_m) * 64;
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = tid >> 6;
@@ -488,22 +505,32 @@ __global__ __launch_bounds__(256) void k_gemm_mfma_fold(const GemmDesc* __restri
     if (k0 + 16 < K) gload(k0 + 16);
 #pragma unroll
     for (int kk = 0; kk < 16; kk += 2) {
+      if (k0 + kk >= K) break;  // block-uniform: the zero-filled tail of the last chunk
 #pragma unroll
       for (int q = 0; q < NS; ++q) {
         const float a = As[buf][q][kk + (lane >> 5)][wm + (lane & 31)];
         const float b = Bs[buf][q][kk + (lane >> 5)][wn + (lane & 31)];
-        acc[q] = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, acc[q], 0, 0, 0);
+        // K-contiguous B: the output is contiguous along m, so the
+        // instruction's columns (lane % 32) take m: C^T = B^T A^T
+        acc[q] = B_K_CONTIG ? __builtin_amdgcn_mfma_f32_32x32x2f32(b, a, acc[q], 0, 0, 0)
+                            : __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, acc[q], 0, 0, 0);
       }
     }
   }
-  const int j = n0 + wn + (lane & 31);
-  if (j >= N) return;
-  const long long cn = (long long)(j % N1) * g.c1 + (long long)(j / N1) * g.c2;
   gfloat* __restrict__ C = ARD_G(g.C);
+  // accumulator r of this lane: (m, n) = (wm + lane % 32, wn + row(r)) when
+  // B is K-contiguous, else (wm + row(r), wn + lane % 32), with
+  // row(r) = 8 (r / 4) + 4 (lane / 32) + r % 4
+  const int jl = (B_K_CONTIG ? m0 + wm : n0 + wn) + (lane & 31);
+  if (jl >= (B_K_CONTIG ? M : N)) return;
+  long long cn = 0;
+  if (!B_K_CONTIG) cn = (long long)(jl % N1) * g.c1 + (long long)(jl / N1) * g.c2;
 #pragma unroll
   for (int r = 0; r < 16; ++r) {
-    const int i = m0 + wm + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-    if (i >= M) continue;
+    const int il = (B_K_CONTIG ? n0 + wn : m0 + wm) + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+    if (il >= (B_K_CONTIG ? N : M)) continue;
+    const int i = B_K_CONTIG ? jl : il;
+    if (B_K_CONTIG) cn = (long long)(il % N1) * g.c1 + (long long)(il / N1) * g.c2;
     if (FOLD == 1) {
       const int k = i < g.Mh ? 2 * i : 2 * (i - g.Mh) + 1;
       if (k < L) C[(long long)k * g.cM + cn] = g.alpha * acc[0][r];
@@ -571,7 +598,9 @@ int launch_gemm(const GemmDesc* d_descs, int ndesc, int total_tiles, bool b_k_co
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_pml(const PmlDesc* __restrict__ descs, int ndesc, int par, float dh,
                                              float dt, float c0) {
-  const long long b = blockIdx.x;
+  // contiguous block ranges per XCD: the stencil's y and z neighbours, read
+  // by nearby blocks, stay in one L2
+  const long long b = xcd_block(int(blockIdx.x), int(gridDim.x));
   int lo = 0, hi = ndesc - 1;
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
