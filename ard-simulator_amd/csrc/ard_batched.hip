@@ -149,6 +149,7 @@ void point_table(int n, int x, bool forward, float* out) {
 
 GemmDesc gemm_k_contig(const float* A, int L, const float* B, float* C, int rows, long long ld) {
   GemmDesc g{};
+  g.b_elems = unsigned(rows * ld);
   g.A = A; g.B = B; g.C = C;
   g.M = L; g.K = L; g.N = rows; g.N1 = rows;
   g.bK = 1; g.b1 = ld; g.b2 = 0;
@@ -161,6 +162,7 @@ GemmDesc gemm_k_contig(const float* A, int L, const float* B, float* C, int rows
 // transform along the middle axis of a [6][L][inner] stack
 GemmDesc gemm_n_contig(const float* A, int L, const float* B, float* C, int inner) {
   GemmDesc g{};
+  g.b_elems = unsigned(6 * L * inner);
   g.A = A; g.B = B; g.C = C;
   g.M = L; g.K = L; g.N = 6 * inner; g.N1 = inner;
   g.bK = inner; g.b1 = 1; g.b2 = (long long)L * inner;
@@ -170,25 +172,34 @@ GemmDesc gemm_n_contig(const float* A, int L, const float* B, float* C, int inne
   return g;
 }
 
-// Even/odd folded cosine matrices of an axis of length L (k_gemm_mfma_fold):
-// forward [Mh + floor(L/2)][ceil(L/2)]: rows m < ceil(L/2) = REDFT10 row 2m,
-// zero rows up to Mh (a multiple of 64), then the odd rows 2m+1; inverse
-// [2][ceil(L/2)][ceil(L/2)]: REDFT01 columns 2k' (E) and 2k'+1 (O) of rows j.
+// Even/odd folded cosine matrices of an axis of length L (k_gemm_mfma_fold),
+// zero-padded so that the kernel's 64-row x 16-column operand tiles never
+// leave them: columns to Kp = 16 ceil(ceil(L/2) / 16);
+// forward [fold_rows_fwd(L)][Kp]: rows m < ceil(L/2) = REDFT10 row 2m, zero
+// rows up to Mh (a multiple of 64), then the odd rows 2m+1, zero rows to a
+// multiple of 64; inverse [2][fold_rows_inv(L)][Kp]: REDFT01 columns 2k' (E)
+// and 2k'+1 (O) of rows j. For odd L the forward operand's middle element is
+// x[j] + x[L-1-j] = 2 x[j], so the even rows' middle column is halved.
 // FP64, stored FP32; cached per length in the context.
+int fold_kp(int L) { return int(ceil_div((L + 1) / 2, 16) * 16); }
+int fold_rows_fwd(int L) { return int(ceil_div((L + 1) / 2, 64) * 64 + ceil_div(L / 2, 64) * 64); }
+int fold_rows_inv(int L) { return int(ceil_div((L + 1) / 2, 64) * 64); }
+
 int fold_tables(Ctx& c, int L, const float** fwd, const float** inv) {
   auto it = c.fold_tables.find(L);
   if (it == c.fold_tables.end()) {
-    const int Kh = (L + 1) / 2, Me = (L + 1) / 2, Mo = L / 2;
-    const int Mh = int(ceil_div(Me, 64) * 64);
-    std::vector<float> f(size_t(Mh + Mo) * Kh, 0.f), v(size_t(2) * Kh * Kh, 0.f);
+    const int Kh = (L + 1) / 2, Me = (L + 1) / 2, Mo = L / 2, Kp = fold_kp(L);
+    const int Mh = int(ceil_div(Me, 64) * 64), Rf = fold_rows_fwd(L), Ri = fold_rows_inv(L);
+    std::vector<float> f(size_t(Rf) * Kp, 0.f), v(size_t(2) * Ri * Kp, 0.f);
     auto c2 = [&](int k, int j) { return 2.0 * std::cos(kPi * k * (j + 0.5) / L); };
     auto c3 = [&](int j, int k) { return k == 0 ? 1.0 : 2.0 * std::cos(kPi * k * (j + 0.5) / L); };
     for (int j = 0; j < Kh; ++j) {
-      for (int m = 0; m < Me; ++m) f[size_t(m) * Kh + j] = float(c2(2 * m, j));
-      for (int m = 0; m < Mo; ++m) f[size_t(Mh + m) * Kh + j] = float(c2(2 * m + 1, j));
+      const double mid = (L % 2 == 1 && j == L / 2) ? 0.5 : 1.0;
+      for (int m = 0; m < Me; ++m) f[size_t(m) * Kp + j] = float(mid * c2(2 * m, j));
+      for (int m = 0; m < Mo; ++m) f[size_t(Mh + m) * Kp + j] = float(c2(2 * m + 1, j));
       for (int k = 0; k < Kh; ++k) {
-        if (2 * k < L) v[size_t(j) * Kh + k] = float(c3(j, 2 * k));
-        if (2 * k + 1 < L) v[size_t(Kh + j) * Kh + k] = float(c3(j, 2 * k + 1));
+        if (2 * k < L) v[size_t(j) * Kp + k] = float(c3(j, 2 * k));
+        if (2 * k + 1 < L) v[size_t(Ri + j) * Kp + k] = float(c3(j, 2 * k + 1));
       }
     }
     float *df, *di;
@@ -558,6 +569,8 @@ int finalize_batched(Ctx& c) {
       if ((r = fold_tables(c, L, &ff, &fi))) return r;
       g.L = L;
       g.K = (L + 1) / 2;
+      g.Kp = fold_kp(L);
+      g.Ap = k < 2 ? fold_rows_fwd(L) : fold_rows_inv(L);
       if (k < 2) {
         g.A = ff;
         g.Mh = int(ceil_div((L + 1) / 2, 64) * 64);

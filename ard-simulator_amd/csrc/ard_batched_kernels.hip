@@ -169,14 +169,6 @@ __device__ __forceinline__ float sinc2_poly(float u) {
   p = fmaf(p, u, -3.3333334e-1f);
   return fmaf(p, u, 1.0f);
 }
-typedef __amdgpu_buffer_rsrc_t brsrc;
-// raw buffer resource over n bytes (gfx9 word 3: 32-bit float data, no swizzle)
-__device__ __forceinline__ brsrc make_rsrc(const void* p, unsigned nbytes) {
-  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), 0, int(nbytes), 0x00020000);
-}
-__device__ __forceinline__ float bload(brsrc r, unsigned voff, unsigned soff) {
-  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, int(voff), int(soff), 0));
-}
 // streamed once per step (the mode state): non-temporal ("nt") loads
 __device__ __forceinline__ float bload_nt(brsrc r, unsigned voff, unsigned soff) {
   return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, int(voff), int(soff), 2));

@@ -25,6 +25,16 @@ typedef __attribute__((address_space(1))) float gfloat;
 typedef __attribute__((address_space(4))) const float cfloat;
 #define ARD_CC(p) ((cfloat*)(p))
 
+// Raw buffer resource over n bytes (gfx9 word 3: 32-bit float data, no
+// swizzle): loads at byte offsets at or past n return 0.
+typedef __amdgpu_buffer_rsrc_t brsrc;
+__device__ __forceinline__ brsrc make_rsrc(const void* p, unsigned nbytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), 0, int(nbytes), 0x00020000);
+}
+__device__ __forceinline__ float bload(brsrc r, unsigned voff, unsigned soff) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, int(voff), int(soff), 0));
+}
+
 // One problem of a grouped, strided GEMM  C = alpha * A * B  (A: M x K row-major).
 // B(k, n) = B[k*bK + (n % N1)*b1 + (n / N1)*b2], C(m, n) likewise with cM, c1, c2.
 // Every per-axis DCT pass is one such problem: A is the cosine matrix of the
@@ -52,6 +62,11 @@ struct GemmDesc {
   // E = A_E X_even and O = A_O X_odd and stores x[j] = E + O, x[L-1-j] = E - O.
   int L;
   int Mh;
+  // Extents for buffer-resource loads (k_gemm_mfma_fold): A holds NS sets of
+  // [Ap][Kp] (rows padded to a multiple of 64, columns to 16, with zeros),
+  // B is b_elems floats from B.
+  int Ap, Kp;
+  unsigned b_elems;
 };
 
 // A PML slab (ARD/pml_partition.h). Buffers are double-buffered by step parity:

@@ -420,7 +420,7 @@ __global__ __launch_bounds__(256) void k_gemm_mfma_fold(const GemmDesc* __restri
     else hi = mid - 1;
   }
   const GemmDesc& g = descs[lo];
-  const int M = g.M, N = g.N, K = g.K, N1 = g.N1, L = g.L;
+  const int M = g.M, N = g.N, K = g.K, N1 = g.N1, L = g.L, Kp = g.Kp;
   const int t = tile - g.tile0;
   // m tiles fastest: the tiles of one B panel run together and share it in L2
   const int tiles_m = (M + 63) >> 6;
@@ -432,80 +432,79 @@ __global__ __launch_bounds__(256) void k_gemm_mfma_fold(const GemmDesc* __restri
   const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
   const bool odd_tile = FOLD == 1 && m0 >= g.Mh;  // forward: uniform per tile
 
-  long long boff[4];
-  int bn[4];
+  // Operands through buffer resources: A is zero-padded to whole 64 x 16
+  // tiles, so its loads need no clamps; B loads past the operand array read
+  // 0 (range-checked), and B elements past K or N meet zero A columns or
+  // are never stored, so no selects either.
+  const brsrc rA = make_rsrc(g.A, unsigned(NS * g.Ap * Kp) * 4u);
+  const brsrc rB = make_rsrc(g.B, g.b_elems * 4u);
+  const int bK = int(g.bK);
+  // this thread's element offsets at k0 = 0
+  unsigned aoff[NS][4];
+#pragma unroll
+  for (int q = 0; q < NS; ++q)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) aoff[q][r] = unsigned((q * g.Ap + m0 + (tid >> 4) + 16 * r) * Kp + (tid & 15)) * 4u;
+  // B(k, n) = B[brow(k) * bK + col(n)]: brow(k) = k, L-1-k (forward), 2k + w (inverse)
+  int bcol[4];
   if (B_K_CONTIG) {
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      const int n = n0 + tid / 16 + 16 * r;
-      bn[r] = n;
-      boff[r] = n < N ? (long long)(n % N1) * g.b1 + (long long)(n / N1) * g.b2 : 0;
+      const int n = min(n0 + (tid >> 4) + 16 * r, N - 1);
+      bcol[r] = int((n % N1) * g.b1 + (n / N1) * g.b2);
     }
   } else {
-    const int n = n0 + (tid & 63);
-    bn[0] = n;
-    boff[0] = n < N ? (long long)(n % N1) * g.b1 + (long long)(n / N1) * g.b2 : 0;
+    const int n = min(n0 + lane, N - 1);
+    bcol[0] = int((n % N1) * g.b1 + (n / N1) * g.b2);
   }
+  const int kt = B_K_CONTIG ? (tid & 15) : (tid >> 6);  // this thread's k within a chunk (+ 4 r if !B_K_CONTIG)
   f32x16 acc[NS];
 #pragma unroll
   for (int q = 0; q < NS; ++q)
 #pragma unroll
     for (int r = 0; r < 16; ++r) acc[q][r] = 0.f;
 
-  const gfloat* __restrict__ A = ARD_CG(g.A);
-  const gfloat* __restrict__ B = ARD_CG(g.B);
   float ra[NS][4], rb[2][4];
-  // B rows of K index k: forward x[k] and x[L-1-k]; inverse X[2k] and X[2k+1]
-  auto brow = [&](int k, int w) { return FOLD == 1 ? (w == 0 ? k : L - 1 - k) : 2 * k + w; };
   auto gload = [&](const int k0) {
 #pragma unroll
     for (int q = 0; q < NS; ++q)
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int k = tid & 15, m = (tid >> 4) + 16 * r;
-        ra[q][r] = A[(long long)q * M * K + (long long)min(m0 + m, M - 1) * K + min(k0 + k, K - 1)];
-      }
+      for (int r = 0; r < 4; ++r) ra[q][r] = bload(rA, aoff[q][r], unsigned(k0) * 4u);
 #pragma unroll
     for (int w = 0; w < 2; ++w)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int gk = min(B_K_CONTIG ? k0 + (tid & 15) : k0 + (tid >> 6) + 4 * r, K - 1);
-        rb[w][r] = B[(long long)min(brow(gk, w), L - 1) * g.bK + boff[B_K_CONTIG ? r : 0]];
+        const int k = k0 + kt + (B_K_CONTIG ? 0 : 4 * r);
+        const int row = FOLD == 1 ? (w == 0 ? k : L - 1 - k) : 2 * k + w;  // negative: offset past the end, reads 0
+        rb[w][r] = bload(rB, unsigned(row * bK + bcol[B_K_CONTIG ? r : 0]) * 4u, 0u);
       }
   };
-  auto sstore = [&](const int k0, const int buf) {
+  auto sstore = [&](const int buf) {
 #pragma unroll
     for (int q = 0; q < NS; ++q)
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int k = tid & 15, m = (tid >> 4) + 16 * r;
-        As[buf][q][k][m] = (m0 + m < M && k0 + k < K) ? ra[q][r] : 0.f;
-      }
+      for (int r = 0; r < 4; ++r) As[buf][q][tid & 15][(tid >> 4) + 16 * r] = ra[q][r];
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       const int k = B_K_CONTIG ? (tid & 15) : (tid >> 6) + 4 * r;
       const int n = B_K_CONTIG ? (tid >> 4) + 16 * r : (tid & 63);
-      const int gk = k0 + k;
-      const bool ok = (B_K_CONTIG ? bn[r] : bn[0]) < N && gk < K;
       if (FOLD == 1) {
-        const bool mid = gk == L - 1 - gk;  // odd L: the middle element, counted once
-        const float v = odd_tile ? (mid ? 0.f : rb[0][r] - rb[1][r]) : (mid ? rb[0][r] : rb[0][r] + rb[1][r]);
-        Bs[buf][0][k][n] = ok ? v : 0.f;
+        Bs[buf][0][k][n] = odd_tile ? rb[0][r] - rb[1][r] : rb[0][r] + rb[1][r];
       } else {
-        Bs[buf][0][k][n] = ok && 2 * gk < L ? rb[0][r] : 0.f;
-        Bs[buf][1][k][n] = ok && 2 * gk + 1 < L ? rb[1][r] : 0.f;
+        Bs[buf][0][k][n] = rb[0][r];
+        Bs[buf][1][k][n] = rb[1][r];
       }
     }
   };
   gload(0);
   int buf = 0;
   for (int k0 = 0; k0 < K; k0 += 16, buf ^= 1) {
-    sstore(k0, buf);
+    sstore(buf);
     __syncthreads();
     if (k0 + 16 < K) gload(k0 + 16);
 #pragma unroll
     for (int kk = 0; kk < 16; kk += 2) {
-      if (k0 + kk >= K) break;  // block-uniform: the zero-filled tail of the last chunk
+      if (k0 + kk >= K) break;  // block-uniform: the zero-padded tail of the last chunk
 #pragma unroll
       for (int q = 0; q < NS; ++q) {
         const float a = As[buf][q][kk + (lane >> 5)][wm + (lane & 31)];
