@@ -661,8 +661,9 @@ int enqueue_step(Ctx& c, int par, PhaseTimer* T) {
     if (T) T->end(st, ev);
   };
   phase(ARD_PH_SOURCES, [&] { return launch_bsources(B.d_src, B.nsrc, c.d_step, c.courant, st); });
-  // captured: PML on the side stream, joined before the interface, which reads
-  // the new PML pressure and writes PML forces; timed: all in order on one stream
+  // captured: PML on the side stream, joined before the modal pass (the
+  // interface reads the new PML pressure and writes PML forces); timed: all in
+  // order on one stream
   const bool fork = !T && B.side;
   if (fork) {
     r |= hipEventRecord(B.ev_fork, st) != hipSuccess;
@@ -679,12 +680,15 @@ int enqueue_step(Ctx& c, int par, PhaseTimer* T) {
   };
   phase(ARD_PH_FACE_FWD1, [&] { return face(0); });
   phase(ARD_PH_FACE_FWD2, [&] { return face(1); });
+  // the PML slabs (memory-bound) overlap the face transforms (matrix-core
+  // bound) and are joined before the modal pass, which streams the whole
+  // mode state and gains nothing from sharing HBM
+  if (fork) r |= hipStreamWaitEvent(st, B.ev_join, 0) != hipSuccess;
   for (auto& g : B.modal)
     phase(ARD_PH_MODAL, [&] { return launch_modal(g.rpw, g.pts, g.d_desc[par], g.d_start, g.n, g.blocks, par, st); });
   phase(ARD_PH_UNFOLD, [&] { return launch_unfold(B.d_unfold, B.d_unfold_start, B.nunfold, B.unfold_total, st); });
   phase(ARD_PH_FACE_INV1, [&] { return face(2); });
   phase(ARD_PH_FACE_INV2, [&] { return face(3); });
-  if (fork) r |= hipStreamWaitEvent(st, B.ev_join, 0) != hipSuccess;
   // the interface reads the PML pressure just written: buffer par ^ 1
   phase(ARD_PH_INTERFACE, [&] { return launch_interface(B.d_bnd, B.nbnd, B.d_parts, par ^ 1, B.bnd_total, st); });
   phase(ARD_PH_RECORD, [&] {
