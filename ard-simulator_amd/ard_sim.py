@@ -27,6 +27,7 @@ EXPORTED = [
     "ard_boundary_info", "ard_recorder_partition", "ard_step", "ard_time_step", "ard_fetch_recorders",
     "ard_copy_pressure_field", "ard_get_pressure", "ard_set_force", "ard_synchronize", "ard_set_kernel_timing",
     "ard_kernel_times", "ard_selftest_transform", "ard_set_transform_variant", "ard_batched_config",
+    "ard_set_batched_tiling",
     "ard_adjoint", "ard_batch_create", "ard_batch_step", "ard_batch_fetch_recorders", "ard_batch_context",
     "ard_batch_room_partition", "ard_batch_destroy", "ard_set_recorder_cubes", "ard_fetch_recorder_cubes",
     "ard_render_view",
@@ -86,6 +87,7 @@ def lib():
         L.ard_selftest_transform.argtypes = [i, i, i, i, i, i, fp, fp, fp]
         L.ard_set_transform_variant.argtypes = [i]
         L.ard_batched_config.argtypes = [vp, ip]
+        L.ard_set_batched_tiling.argtypes = [vp, ctypes.c_int, ctypes.c_int]
         L.ard_adjoint.argtypes = [vp, i, fp, fp, fp, i]
         L.ard_batch_create.argtypes = [ctypes.POINTER(vp), i, ctypes.POINTER(vp)]
         L.ard_batch_step.argtypes = [vp, i]
@@ -271,6 +273,10 @@ class Simulation:
         ds = np.zeros(6, dtype=np.float32)
         _check(self.L.ard_adjoint(self.ctx, nsteps, _fp(g), _fp(dw), _fp(ds), int(checkpoint_stride)))
         return dw, ds
+
+    def set_batched_tiling(self, rows_per_wave, kz_per_block):
+        """Pins the modal-pass tiling of a finalized batched context."""
+        _check(self.L.ard_set_batched_tiling(self.ctx, int(rows_per_wave), int(kz_per_block)))
 
     def batched_config(self):
         """(rows per wave, kz planes per block, transform variant, candidates timed)."""

@@ -463,10 +463,10 @@ int finalize_batched(Ctx& c) {
       return set_error(ARD_ERR_UNSUPPORTED, "a DCT partition of 2^30 or more cells (split it)");
     P.plane[0] = D * H; P.plane[1] = D * W; P.plane[2] = H * W;
     // partial-reduction buffers are sized for the finest tiling the
-    // autotuner may pick: 4 rows per wave, 16 kz planes per block
-    P.rpw = 4;
+    // autotuner may pick: 2 rows per wave, 16 kz planes per block
+    P.rpw = 2;
     P.nbx = int(ceil_div(W, 64));
-    P.nby = int(ceil_div(H, 16));
+    P.nby = int(ceil_div(H, 8));
     P.kz_chunk = 16;
     P.nbz = int(ceil_div(D, 16));
     const long long shell = 6 * (P.plane[0] + P.plane[1] + P.plane[2]);
@@ -746,7 +746,7 @@ int autotune(Ctx& c) {
   int best_rpw = 4, best_kzc = 48, tried = 0;
   float best = 1e30f;
   int r = ARD_OK;
-  for (int rpw : {4, 6, 8}) {
+  for (int rpw : {2, 4, 6, 8}) {
     for (int kzc : {16, 32, 48, 96}) {
       if ((r = apply_tiling(c, rpw, kzc))) break;
       const float ms = time_ms([&] {
@@ -780,6 +780,14 @@ int autotune(Ctx& c) {
   if ((r = apply_tiling(c, best_rpw, best_kzc))) return r;
   B.tuned = tried;
   return ARD_OK;
+}
+
+int batched_set_tiling(Ctx& c, int rpw, int kzc) {
+  if (!c.bs) return set_error(ARD_ERR_STATE, "not a finalized batched context");
+  if (rpw != 2 && rpw != 4 && rpw != 6 && rpw != 8) return set_error(ARD_ERR_INVALID, "rows per wave must be 2, 4, 6 or 8");
+  // the partial-reduction buffers hold the finest tiling: 16 kz planes per block
+  if (kzc < 16 || kzc > 4096) return set_error(ARD_ERR_INVALID, "kz planes per block must be in [16, 4096]");
+  return apply_tiling(c, rpw, kzc);
 }
 
 int batched_config(Ctx& c, int* out4) {

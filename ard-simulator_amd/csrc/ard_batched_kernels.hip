@@ -204,7 +204,7 @@ __global__ __launch_bounds__(256) void k_modal(const ModalDesc* __restrict__ des
   __shared__ __attribute__((aligned(16))) float s_m[KB][TY][LD];     // new modes of a batch
   __shared__ __attribute__((aligned(16))) float s_ty[KB][6][4][64];  // y sums: [plane][ky parity * 3 + j][wave][lane]
   __shared__ __attribute__((aligned(16))) float s_rx[64 * 3];        // x reduction weights of the tile's kx
-  __shared__ __attribute__((aligned(16))) float s_gx[KB][TY][8];     // x-face spectra of the batch's rows (6 used)
+  __shared__ __attribute__((aligned(16))) float s_gx[KB][TY][8];     // x-face spectra of the batch's rows: even 0..2, odd 4..6
   __shared__ __attribute__((aligned(16))) float s_zt[KB][8];         // z factors of the batch's planes
   __shared__ __attribute__((aligned(16))) float s_pc[PTS ? KB : 1][TY][NPT];  // point-force y*z factors
   __shared__ __attribute__((aligned(16))) float s_rfx[PTS ? NRC : 1][64];     // recorder x factors
@@ -241,14 +241,11 @@ __global__ __launch_bounds__(256) void k_modal(const ModalDesc* __restrict__ des
   const int nsrc = PTS ? min(P.nsrc, NPT) : 0;
   const int nrec = PTS ? min(P.nrec, NRC) : 0;
 
-  // x factors, split by parity so that the folded spectra need no select
-  float exs[3], exd[3];
+  // x factors; the kx parity picks the folded spectra they multiply (even
+  // modes the sums, odd modes the differences)
+  float ex[3];
 #pragma unroll
-  for (int i = 0; i < 3; ++i) {
-    const float e = ARD_CG(P.Ex)[kxc * 3 + i];
-    exs[i] = px ? 0.f : e;
-    exd[i] = px ? e : 0.f;
-  }
+  for (int i = 0; i < 3; ++i) ex[i] = ARD_CG(P.Ex)[kxc * 3 + i];
   if (wave == 0) {
 #pragma unroll
     for (int i = 0; i < 3; ++i) s_rx[lane * 3 + i] = xok ? ARD_CG(P.Rx)[kxc * 3 + i] : 0.f;
@@ -306,9 +303,10 @@ __global__ __launch_bounds__(256) void k_modal(const ModalDesc* __restrict__ des
       const int e = tid + 256 * t;
       float v = 0.f;
       if (e < NGX) {
-        const int q = e & 7, row = (e >> 3) % TY, kz = min(kb + e / (8 * TY), kz1 - 1);
+        const int sl = e & 7, row = (e >> 3) % TY, kz = min(kb + e / (8 * TY), kz1 - 1);
         const int ky = min(kyb + row, H - 1);
-        v = q < 6 ? ARD_CG(P.Gx)[(kz * H + ky) * 6 + q] : 0.f;
+        const int q = sl < 3 ? sl : sl - 1;  // slots 0..2 even, 4..6 odd
+        v = (sl != 3 && sl != 7) ? ARD_CG(P.Gx)[(kz * H + ky) * 6 + q] : 0.f;
       } else if (e < NGX + NZT) {
         const int f = e - NGX, kz = min(kb + (f >> 3), kz1 - 1);
         v = ARD_CG(P.zt)[kz * 8 + (f & 7)];
@@ -358,8 +356,7 @@ __global__ __launch_bounds__(256) void k_modal(const ModalDesc* __restrict__ des
     for (int r = 0; r < RPW; ++r) {
       const int PY = r & 1;
       const int row = wave * RPW + r;
-      const float4 ga = *reinterpret_cast<const float4*>(&s_gx[kl][row][0]);  // folded x spectra q 0..3
-      const float4 gb = *reinterpret_cast<const float4*>(&s_gx[kl][row][4]);  // q 4..5
+      const float4 ga = *reinterpret_cast<const float4*>(&s_gx[kl][row][4 * px]);  // this kx parity's x spectra
       // modal coefficients (ARD/dct_partition.cpp:24-33,68) from u = (w dt / 2)^2,
       // separable in the 1-based wavenumbers:
       //   s^2 = sin^2(w dt / 2) = u S(u),  a = .999 * 2 cos(w dt) = 1.998 - 3.996 s^2,
@@ -368,12 +365,9 @@ __global__ __launch_bounds__(256) void k_modal(const ModalDesc* __restrict__ des
       const float S = sinc2_poly(u);
       const float a = fmaf(-3.996f, u * S, 1.998f);
       const float g = gsc * S;
-      float fh = exs[0] * ga.x;
-      fh = fmaf(exs[1], ga.y, fh);
-      fh = fmaf(exs[2], ga.z, fh);
-      fh = fmaf(exd[0], ga.w, fh);
-      fh = fmaf(exd[1], gb.x, fh);
-      fh = fmaf(exd[2], gb.y, fh);
+      float fh = ex[0] * ga.x;
+      fh = fmaf(ex[1], ga.y, fh);
+      fh = fmaf(ex[2], ga.z, fh);
       fh = fmaf(ey[r][0], in.gy[PY * 3 + 0], fh);
       fh = fmaf(ey[r][1], in.gy[PY * 3 + 1], fh);
       fh = fmaf(ey[r][2], in.gy[PY * 3 + 2], fh);
@@ -584,6 +578,7 @@ int launch_modal(int rpw, bool pts, const ModalDesc* d, const long long* starts,
     else hipLaunchKernelGGL((k_modal<R, false>), g, b, 0, s, d, starts, n, par);               \
     break;
   switch (rpw) {
+    ARD_MODAL_CASE(2)
     ARD_MODAL_CASE(4)
     ARD_MODAL_CASE(6)
     ARD_MODAL_CASE(8)

@@ -506,6 +506,12 @@ int ard_batched_config(ard_ctx* ctx, int* out4) {
   return batched_config(*c, out4);
 }
 
+int ard_set_batched_tiling(ard_ctx* ctx, int rows_per_wave, int kz_per_block) {
+  Ctx* c = reinterpret_cast<Ctx*>(ctx);
+  if (!c) return fail(ARD_ERR_INVALID, "null context");
+  return batched_set_tiling(*c, rows_per_wave, kz_per_block);
+}
+
 int ard_set_transform_variant(int variant) {
   if (variant < 0 || variant > 3) return fail(ARD_ERR_INVALID, "unknown transform variant");
   g_gemm_variant = variant;

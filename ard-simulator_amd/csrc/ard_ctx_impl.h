@@ -139,6 +139,7 @@ int batched_pressure_field(Ctx& c, int part, const float** out);
 int batched_set_force(Ctx& c, int part, int x, int y, int z, float f);
 void destroy_batched(Ctx& c);
 int batched_config(Ctx& c, int* out4);
+int batched_set_tiling(Ctx& c, int rpw, int kzc);
 int batched_modes(Ctx& c, int part, const float* M[2], int* wp);
 
 // recorder cubes (ard_output.hip)

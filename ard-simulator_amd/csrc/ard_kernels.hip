@@ -503,8 +503,7 @@ __global__ __launch_bounds__(256) void k_gemm_mfma_fold(const GemmDesc* __restri
     __syncthreads();
     if (k0 + 16 < K) gload(k0 + 16);
 #pragma unroll
-    for (int kk = 0; kk < 16; kk += 2) {
-      if (k0 + kk >= K) break;  // block-uniform: the zero-padded tail of the last chunk
+    for (int kk = 0; kk < 16; kk += 2) {  // past K: zero-padded A columns
 #pragma unroll
       for (int q = 0; q < NS; ++q) {
         const float a = As[buf][q][kk + (lane >> 5)][wm + (lane & 31)];

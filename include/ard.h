@@ -199,6 +199,13 @@ int ard_set_transform_variant(int variant);
  * ard_params.autotune is set): out[4] = rows per wave, kz planes per block,
  * transform variant, number of candidates timed. No counterpart in the reference. */
 int ard_batched_config(ard_ctx* ctx, int* out4);
+/* Sets the batched form's modal-pass tiling after ard_finalize: rows per wave
+ * (2, 4, 6 or 8) and kz planes per block (16 to 4096, rounded to an even
+ * chunk per partition), as the autotuner would. ARD_ERR_STATE for a context
+ * that is not a finalized batched one, ARD_ERR_INVALID for other values. No
+ * counterpart in the reference: it pins the tiling, for tests and callers that
+ * skip the autotuner. */
+int ard_set_batched_tiling(ard_ctx* ctx, int rows_per_wave, int kz_per_block);
 
 /* Discrete adjoint of the plain form (no counterpart in the reference, which
  * has no gradients; it transposes the step of ARD/simulation.cpp:300-317).

@@ -74,6 +74,24 @@ def test_scene3_batched_transform_variants(variant):
     assert e64 <= TOL
 
 
+@pytest.mark.parametrize("rpw,kzc", [(2, 16), (2, 48), (4, 16), (6, 32), (8, 96)])
+def test_scene3_batched_tilings(rpw, kzc):
+    """Every modal-pass tiling the autotuner may choose (rows per wave, kz
+    planes per block) against the FP64 golden trace, set after stepping has
+    begun (the tiling changes no state, only the launch shape)."""
+    sc = S.scene3()
+    print("case scene3 batched tiling", rpw, kzc, flush=True)
+    sim = A.Simulation(sc, form=A.FORM_BATCHED)
+    sim.update(100)
+    sim.set_batched_tiling(rpw, kzc)
+    assert sim.batched_config()[:2] == (rpw, kzc)
+    sim.update(sc.steps - 100)
+    tr = sim.recorders()
+    e64 = rel_l2(tr, np.load(os.path.join(GOLD, "scene3_fp64.npy")))
+    print("scene3 tiling %d/%d rel-L2 vs fp64 %.3e" % (rpw, kzc, e64))
+    assert e64 <= TOL
+
+
 def small_scene():
     """Three boxes with partial faces (PML runs, edge cells written twice)."""
     sc = S.two_rooms()
