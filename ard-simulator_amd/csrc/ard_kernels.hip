@@ -594,8 +594,9 @@ int launch_gemm(const GemmDesc* d_descs, int ndesc, int total_tiles, bool b_k_co
 // PML slab update, one thread per cell (ARD/pml_partition.cpp:113-200).
 // Out-of-slab neighbours read 0 (the reference's sentinel element, :6-15).
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_pml(const PmlDesc* __restrict__ descs, int ndesc, int par, float dh,
-                                             float dt, float c0) {
+// r180 = 1 / (180 h^2), r12 = 1 / (12 h), rdt = 1 / dt, formed on the host in FP32
+__global__ __launch_bounds__(256) void k_pml(const PmlDesc* __restrict__ descs, int ndesc, int par, float r180,
+                                             float r12, float dt, float rdt, float c0) {
   // contiguous block ranges per XCD: the stencil's y and z neighbours, read
   // by nearby blocks, stay in one L2
   const long long b = xcd_block(int(blockIdx.x), int(gridDim.x));
@@ -607,7 +608,7 @@ __global__ __launch_bounds__(256) void k_pml(const PmlDesc* __restrict__ descs, 
   }
   const PmlDesc& P = descs[lo];
   const int W = P.W, H = P.H, D = P.D;
-  const unsigned cell = unsigned(b - P.block0) * 256u + threadIdx.x;  // slabs hold < 2^31 cells
+  const unsigned cell = unsigned(b - P.block0) * 256u + threadIdx.x;  // slabs hold < 2^29 cells (checked at setup)
   if (cell >= unsigned(W * H * D)) return;
   const unsigned row = cell / unsigned(W);
   const int i = int(cell - row * unsigned(W));
@@ -618,28 +619,46 @@ __global__ __launch_bounds__(256) void k_pml(const PmlDesc* __restrict__ descs, 
   const gfloat* __restrict__ fx = ARD_CG(P.phi[par][0]);
   const gfloat* __restrict__ fy = ARD_CG(P.phi[par][1]);
   const gfloat* __restrict__ fz = ARD_CG(P.phi[par][2]);
-  const long long sx = 1, sy = W, sz = (long long)W * H;
-  // neighbour read with the zero sentinel outside the slab; the load itself
-  // goes to a clamped (valid) address so that it is never conditional
-  auto at = [&](const gfloat* a, int x, int y, int z) -> float {
-    const bool in = !(x < 0 || x >= W || y < 0 || y >= H || z < 0 || z >= D);
-    const int xc = min(max(x, 0), W - 1), yc = min(max(y, 0), H - 1), zc = min(max(z, 0), D - 1);
-    const float v = a[zc * sz + yc * sy + xc * sx];
-    return in ? v : 0.f;
-  };
+  // Neighbour reads through buffer resources over the slab: a neighbour
+  // outside the slab gets an offset past the end, which reads 0 (the zero
+  // sentinel of ARD/pml_partition.cpp:6-15). One offset per axis and distance
+  // serves every array read there.
+  const unsigned nbytes = unsigned(W * H * D) * 4u;  // slabs hold < 2^29 cells
+  const brsrc rp = make_rsrc(P.p[par], nbytes);
+  const brsrc rfx = make_rsrc(P.phi[par][0], nbytes);
+  const brsrc rfy = make_rsrc(P.phi[par][1], nbytes);
+  const brsrc rfz = make_rsrc(P.phi[par][2], nbytes);
+  const unsigned c4b = cell * 4u;
+  constexpr unsigned kOut = 0x80000000u;
+  unsigned ox[7], oy[7], oz[7];
+#pragma unroll
+  for (int m = 0; m < 7; ++m) {
+    const int d = m - 3;
+    ox[m] = unsigned(i + d) < unsigned(W) ? c4b + unsigned(d * 4) : kOut;
+    oy[m] = unsigned(j + d) < unsigned(H) ? c4b + unsigned(d * 4 * W) : kOut;
+    oz[m] = unsigned(k + d) < unsigned(D) ? c4b + unsigned(d * 4 * W * H) : kOut;
+  }
   const float c6[7] = {2.f, -27.f, 270.f, -490.f, 270.f, -27.f, 2.f};
   const float c4[5] = {1.f, -8.f, 0.f, 8.f, -1.f};
+  float px[7], py[7], pz[7];
+#pragma unroll
+  for (int m = 0; m < 7; ++m) {
+    px[m] = bload(rp, ox[m], 0u);
+    py[m] = bload(rp, oy[m], 0u);
+    pz[m] = bload(rp, oz[m], 0u);
+  }
   float dxx = 0.f, dyy = 0.f, dzz = 0.f;
 #pragma unroll
   for (int m = 0; m < 7; ++m) {
-    dxx += c6[m] * at(p, i + m - 3, j, k);
-    dyy += c6[m] * at(p, i, j + m - 3, k);
-    dzz += c6[m] * at(p, i, j, k + m - 3);
+    dxx += c6[m] * px[m];
+    dyy += c6[m] * py[m];
+    dzz += c6[m] * pz[m];
   }
-  const float inv180 = 180.f * dh * dh;
-  dxx /= inv180;
-  dyy /= inv180;
-  dzz /= inv180;
+  // divisions by the step's constants as products with their reciprocals
+  // (within 1.5 ulp of the quotients)
+  dxx *= r180;
+  dyy *= r180;
+  dzz *= r180;
   const float z = ARD_CG(P.zprof)[P.axis == 0 ? i : (P.axis == 1 ? j : k)];
   const float zx = P.axis == 0 ? z : 0.f, zy = P.axis == 1 ? z : 0.f, zz = P.axis == 2 ? z : 0.f;
   const float pc = p[cell];
@@ -647,22 +666,21 @@ __global__ __launch_bounds__(256) void k_pml(const PmlDesc* __restrict__ descs, 
   const float term1 = 2.f * pc;
   const float term2 = -po;
   const float term3 = c0 * c0 * (dxx + dyy + dzz);
-  const float term4 = -(zx + zy + zz) * (pc - po) / dt;
+  const float term4 = -(zx + zy + zz) * (pc - po) * rdt;
   const float term5 = -(zx * zy + zy * zz + zx * zz) * pc;
   float dpx = 0.f, dpy = 0.f, dpz = 0.f, dux = 0.f, duy = 0.f, duz = 0.f;
 #pragma unroll
   for (int m = 0; m < 5; ++m) {
     if (m == 2) continue;  // zero coefficient
-    dpx += c4[m] * at(fx, i + m - 2, j, k);
-    dpy += c4[m] * at(fy, i, j + m - 2, k);
-    dpz += c4[m] * at(fz, i, j, k + m - 2);
-    dux += c4[m] * at(p, i + m - 2, j, k);
-    duy += c4[m] * at(p, i, j + m - 2, k);
-    duz += c4[m] * at(p, i, j, k + m - 2);
+    dpx += c4[m] * bload(rfx, ox[m + 1], 0u);
+    dpy += c4[m] * bload(rfy, oy[m + 1], 0u);
+    dpz += c4[m] * bload(rfz, oz[m + 1], 0u);
+    dux += c4[m] * px[m + 1];
+    duy += c4[m] * py[m + 1];
+    duz += c4[m] * pz[m + 1];
   }
-  const float d12 = 12.f * dh;
-  dpx /= d12; dpy /= d12; dpz /= d12;
-  dux /= d12; duy /= d12; duz /= d12;
+  dpx *= r12; dpy *= r12; dpz *= r12;
+  dux *= r12; duy *= r12; duz *= r12;
   const float term6 = dpx + dpy + dpz;
   gfloat* __restrict__ f = ARD_G(P.f);
   const float fc = f[cell];
@@ -677,7 +695,8 @@ __global__ __launch_bounds__(256) void k_pml(const PmlDesc* __restrict__ descs, 
 int launch_pml(const PmlDesc* d_descs, int ndesc, long long total_blocks, int par, float dh, float dt, float c0,
                hipStream_t s) {
   if (total_blocks <= 0) return 0;
-  hipLaunchKernelGGL(k_pml, dim3(unsigned(total_blocks)), dim3(256), 0, s, d_descs, ndesc, par, dh, dt, c0);
+  const float r180 = 1.f / (180.f * dh * dh), r12 = 1.f / (12.f * dh), rdt = 1.f / dt;
+  hipLaunchKernelGGL(k_pml, dim3(unsigned(total_blocks)), dim3(256), 0, s, d_descs, ndesc, par, r180, r12, dt, rdt, c0);
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 
