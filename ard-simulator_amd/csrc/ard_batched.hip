@@ -108,7 +108,7 @@ struct BatchedState {
   // tiling of the modal pass and transform kernel, chosen at finalize
   int rpw = 4, kz_chunk = 48, gemm_variant = 1, tuned = 0;
   // second stream and events: in the step graph the PML slabs run beside the
-  // DCT chain (fold .. inverse transforms), which they share no data with
+  // fold and forward face transforms, which they share no data with
   hipStream_t side = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   ~BatchedState() {
@@ -680,9 +680,8 @@ int enqueue_step(Ctx& c, int par, PhaseTimer* T) {
   };
   phase(ARD_PH_FACE_FWD1, [&] { return face(0); });
   phase(ARD_PH_FACE_FWD2, [&] { return face(1); });
-  // the PML slabs (memory-bound) overlap the face transforms (matrix-core
-  // bound) and are joined before the modal pass, which streams the whole
-  // mode state and gains nothing from sharing HBM
+  // the PML stencil overlaps the face transforms (matrix cores) and is joined
+  // before the modal pass, which streams the whole mode state
   if (fork) r |= hipStreamWaitEvent(st, B.ev_join, 0) != hipSuccess;
   for (auto& g : B.modal)
     phase(ARD_PH_MODAL, [&] { return launch_modal(g.rpw, g.pts, g.d_desc[par], g.d_start, g.n, g.blocks, par, st); });
