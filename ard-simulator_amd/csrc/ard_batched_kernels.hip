@@ -398,15 +398,17 @@ __global__ __launch_bounds__(256) void k_modal(const ModalDesc* __restrict__ des
   };
 
   // x and y sums of a finished batch (planes kb .. kb+n-1):
-  // x: KB * TY (plane, row) slots, four threads per slot; thread qtr of a slot
-  // takes kx = 16 j + 4 qtr + t (j, t < 4) and the four meet through quad
-  // permutes. The threads of a slot share its trip count, so every quad is
+  // x (xreduce, before the batch barrier): each wave sums its own KB * RPW
+  // (plane, row) slots of the mode tile, four lanes per slot; lane qtr of a
+  // slot takes kx = 16 j + 4 qtr + t (j, t < 4) and the four meet through
+  // quad permutes. The lanes of a slot share its trip count, so every quad is
   // whole. The recorders' x sums ride along, weighted by their y*z factors.
-  // y: the four waves' sums of each (plane, j, kx), four kx per thread.
-  auto reduce = [&](const int kb, const int n) {
-    const int qtr = tid & 3;
-    for (int c = tid >> 2; c < KB * TY; c += 64) {
-      const int kl = c / TY, row = c % TY;
+  // y (ysum, after it): the four waves' sums of each (plane, j, kx), four kx
+  // per thread.
+  auto xreduce = [&](const int kb, const int n) {
+    const int qtr = lane & 3;
+    for (int cc = lane >> 2; cc < KB * RPW; cc += 16) {
+      const int kl = cc / RPW, row = wave * RPW + cc % RPW;
       float ev0 = 0.f, ev1 = 0.f, ev2 = 0.f, od0 = 0.f, od1 = 0.f, od2 = 0.f;
       float rs[NRC] = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll 1  // one group of 4 kx at a time: few registers live beside the prefetched plane
@@ -459,6 +461,8 @@ __global__ __launch_bounds__(256) void k_modal(const ModalDesc* __restrict__ des
         }
       }
     }
+  };
+  auto ysum = [&](const int kb, const int n) {
     for (int e = tid; e < KB * 6 * 16; e += 256) {
       const int x4 = e & 15, q = (e >> 4) % 6, pl = e / 96;
       if (pl >= n) continue;
@@ -496,8 +500,11 @@ __global__ __launch_bounds__(256) void k_modal(const ModalDesc* __restrict__ des
         plane(nxt, kz + 1, kl + 1, std::integral_constant<int, 1>());
       }
     }
-    __syncthreads();  // batch in LDS; its factors no longer read
-    reduce(kb, n);
+    // the x sums read only this wave's rows of the mode tile: no block barrier
+    __builtin_amdgcn_wave_barrier();
+    xreduce(kb, n);
+    __syncthreads();  // y sums in LDS; the batch's factors no longer read
+    ysum(kb, n);
     stage_store();
     __syncthreads();  // LDS free for the next batch, its factors in place
   }
