@@ -62,9 +62,9 @@ struct GemmDesc {
   // E = A_E X_even and O = A_O X_odd and stores x[j] = E + O, x[L-1-j] = E - O.
   int L;
   int Mh;
-  // Extents for buffer-resource loads (k_gemm_mfma_fold): A holds NS sets of
-  // [Ap][Kp] (rows padded to a multiple of 64, columns to 16, with zeros),
-  // B is b_elems floats from B.
+  // Extents for buffer-resource loads (k_gemm_mfma_fold): A holds one
+  // [Ap][Kp] matrix (forward) or two (inverse: E then O), rows padded to a
+  // multiple of 64 and columns to 16 with zeros; B is b_elems floats from B.
   int Ap, Kp;
   unsigned b_elems;
 };
