@@ -8,7 +8,10 @@ form (hipGraph per step) gives `value`; the plain per-partition form is timed
 beside it on the same scene, and their recorder traces are compared.
 
     python bench.py [--steps K] [--warmup W] [--workload hall|scene3|two_rooms]
-                    [--no-plain] [--no-cpu-baseline]
+                    [--no-plain] [--no-cpu-baseline] [--no-rooms64]
+
+Beside the hall, the 64-room batch (batched plus precision) is timed with its
+face transforms in FP32 and in BF16 split precision (`rooms64_precision`).
 """
 import argparse
 import json
@@ -90,6 +93,42 @@ def traffic_from_profile(kernel):
     return d.get("kernels", {}).get(kernel)
 
 
+def rooms64_precision(steps=1500, timed=300):
+    """The batched-plus-precision workload: 64 independent rooms (sides U[2,6] m,
+    1-4 boxes each, h = 0.05 m) in one batched context, with the per-axis face
+    transforms in FP32 (folded, matrix cores) and in BF16 split precision (three
+    BF16 products, FP32 accumulate). Throughput of each over `timed` steps and
+    the recorder rel-L2 of BF16 split against FP32 over `steps` steps (the
+    errors against the FP64 restatement are the GPU precision tests')."""
+    rooms = S.split_rooms(S.rooms64(steps=steps))
+    out, traces = {}, {}
+    for v, name in ((3, "fp32_folded"), (2, "bf16x3")):
+        A.set_transform_variant(v)
+        try:
+            b = A.Batch(rooms)
+        finally:
+            A.set_transform_variant(3)
+        parts = b.partitions()
+        cells = sum(p[6] * p[7] * p[8] for p in parts)
+        b.update(20)
+        b.synchronize()
+        t0 = time.perf_counter()
+        b.update(timed)
+        b.synchronize()
+        secs = time.perf_counter() - t0
+        b.update(steps - 20 - timed)
+        traces[name] = np.concatenate([b.recorders(r, 0, steps).astype(np.float64).ravel()
+                                       for r in range(len(rooms))])
+        b.close()
+        out[name] = {"value": round(cells * timed / secs / 1e9, 4), "unit": "Gcell-steps/s",
+                     "ms_per_step": round(secs * 1e3 / timed, 4)}
+    ref = traces["fp32_folded"]
+    out["bf16x3"]["recorder_rel_l2_vs_fp32"] = float(np.linalg.norm(traces["bf16x3"] - ref) / np.linalg.norm(ref))
+    out["rooms"] = len(rooms)
+    out["cells"] = int(cells)
+    return out
+
+
 def cpu_baseline(sc):
     """The CPU oracle (FP32 restatement of the reference's step, dense DCTs,
     OpenMP) timed on this host for one step of the same scene."""
@@ -120,6 +159,8 @@ def main():
     ap.add_argument("--workload", default="hall")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-rooms64", action="store_true",
+                    help="skip the 64-room batch in FP32 and BF16 split precision")
     ap.add_argument("--transform-variant", type=int, default=1,
                     help="0 vector FMA, 1 matrix cores FP32, 2 matrix cores BF16 split, 3 matrix cores FP32 folded")
     ap.add_argument("--no-autotune", action="store_true")
@@ -207,6 +248,9 @@ def main():
         out["plain"] = {"value": round(plain_value, 4), "ms_per_step": round(secs_p * 1e3 / n_p, 3), "steps": n_p}
         out["batched_over_plain"] = round(value / plain_value, 2)
         out["recorder_rel_l2_batched_vs_plain"] = {"value": rel, "steps": nt}
+    if not args.no_rooms64:
+        out["rooms64_precision"] = rooms64_precision()
+        A.set_transform_variant(args.transform_variant)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(sc)
     print(json.dumps(out))
