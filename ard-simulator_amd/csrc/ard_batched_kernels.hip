@@ -36,7 +36,6 @@ __device__ __forceinline__ int find_desc_ll(const long long* __restrict__ starts
 }
 }  // namespace
 
-typedef float v4f __attribute__((ext_vector_type(4)));
 typedef __attribute__((address_space(1))) v4f gv4f;
 
 // ---------------------------------------------------------------------------

@@ -27,6 +27,7 @@ typedef __attribute__((address_space(4))) const float cfloat;
 
 // Raw buffer resource over n bytes (gfx9 word 3: 32-bit float data, no
 // swizzle): loads at byte offsets at or past n return 0.
+typedef float v4f __attribute__((ext_vector_type(4)));
 typedef __amdgpu_buffer_rsrc_t brsrc;
 __device__ __forceinline__ brsrc make_rsrc(const void* p, unsigned nbytes) {
   return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), 0, int(nbytes), 0x00020000);

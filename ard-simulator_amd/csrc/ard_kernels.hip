@@ -437,22 +437,22 @@ __global__ __launch_bounds__(256) void k_gemm_mfma_fold(const GemmDesc* __restri
   // 0 (range-checked), and B elements past K or N meet zero A columns or
   // are never stored, so no selects either.
   const brsrc rA = make_rsrc(g.A, unsigned(NS * g.Ap * Kp) * 4u);
-  const brsrc rB = make_rsrc(g.B, g.b_elems * 4u);
+  // B's 16-byte loads that hold needed elements end at most 3 floats past a
+  // row, so past the array's last row too: B arrays are followed by at least
+  // 4 floats of the same allocation (the face buffers are carved from one block)
+  const brsrc rB = make_rsrc(g.B, (g.b_elems + 4u) * 4u);
   const int bK = int(g.bK);
-  // this thread's element offsets at k0 = 0
-  unsigned aoff[NS][4];
+  // A: thread (m = tid / 4, k = 4 (tid % 4) .. +3), one 16-byte load per set
+  // and chunk (byte offsets at k0 = 0; 16-byte loads need only 4-byte alignment)
+  unsigned aoff[NS];
 #pragma unroll
-  for (int q = 0; q < NS; ++q)
-#pragma unroll
-    for (int r = 0; r < 4; ++r) aoff[q][r] = unsigned((q * g.Ap + m0 + (tid >> 4) + 16 * r) * Kp + (tid & 15)) * 4u;
+  for (int q = 0; q < NS; ++q) aoff[q] = unsigned((q * g.Ap + m0 + (tid >> 2)) * Kp + 4 * (tid & 3)) * 4u;
   // B(k, n) = B[brow(k) * bK + col(n)]: brow(k) = k, L-1-k (forward), 2k + w (inverse)
+  // K-contiguous B: thread (n = tid / 4, k = 4 (tid % 4) .. +3), 16-byte loads
   int bcol[4];
   if (B_K_CONTIG) {
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int n = min(n0 + (tid >> 4) + 16 * r, N - 1);
-      bcol[r] = int((n % N1) * g.b1 + (n / N1) * g.b2);
-    }
+    const int n = min(n0 + (tid >> 2), N - 1);
+    bcol[0] = int((n % N1) * g.b1 + (n / N1) * g.b2);
   } else {
     const int n = min(n0 + lane, N - 1);
     bcol[0] = int((n % N1) * g.b1 + (n / N1) * g.b2);
@@ -464,30 +464,51 @@ __global__ __launch_bounds__(256) void k_gemm_mfma_fold(const GemmDesc* __restri
 #pragma unroll
     for (int r = 0; r < 16; ++r) acc[q][r] = 0.f;
 
-  float ra[NS][4], rb[2][4];
+  v4f ra[NS];
+  float rb[2][4];  // K-contiguous B: [w][i] for k = 4 (tid % 4) + i; else [w][r] for k = tid / 64 + 4 r
   auto gload = [&](const int k0) {
 #pragma unroll
     for (int q = 0; q < NS; ++q)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) ra[q][r] = bload(rA, aoff[q][r], unsigned(k0) * 4u);
-#pragma unroll
-    for (int w = 0; w < 2; ++w)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int k = k0 + kt + (B_K_CONTIG ? 0 : 4 * r);
-        const int row = FOLD == 1 ? (w == 0 ? k : L - 1 - k) : 2 * k + w;  // negative: offset past the end, reads 0
-        rb[w][r] = bload(rB, unsigned(row * bK + bcol[B_K_CONTIG ? r : 0]) * 4u, 0u);
+      ra[q] = __builtin_bit_cast(v4f, __builtin_amdgcn_raw_buffer_load_b128(rA, int(aoff[q]), int(k0 * 4), 0));
+    if (B_K_CONTIG) {
+      const int k = k0 + 4 * (tid & 3);
+      if (FOLD == 1) {
+        // x[k .. k+3] and x[L-4-k .. L-1-k] (reversed); negative offsets read 0
+        const v4f d = __builtin_bit_cast(v4f, __builtin_amdgcn_raw_buffer_load_b128(rB, int(unsigned(bcol[0] + k) * 4u), 0, 0));
+        const v4f m = __builtin_bit_cast(v4f, __builtin_amdgcn_raw_buffer_load_b128(rB, int(unsigned(bcol[0] + L - 4 - k) * 4u), 0, 0));
+        rb[0][0] = d.x; rb[0][1] = d.y; rb[0][2] = d.z; rb[0][3] = d.w;
+        rb[1][0] = m.w; rb[1][1] = m.z; rb[1][2] = m.y; rb[1][3] = m.x;
+      } else {
+        // rows 2k .. 2k+7: (k, E), (k, O), (k+1, E), ...
+        const v4f f0 = __builtin_bit_cast(v4f, __builtin_amdgcn_raw_buffer_load_b128(rB, int(unsigned(bcol[0] + 2 * k) * 4u), 0, 0));
+        const v4f f1 = __builtin_bit_cast(v4f, __builtin_amdgcn_raw_buffer_load_b128(rB, int(unsigned(bcol[0] + 2 * k + 4) * 4u), 0, 0));
+        rb[0][0] = f0.x; rb[1][0] = f0.y; rb[0][1] = f0.z; rb[1][1] = f0.w;
+        rb[0][2] = f1.x; rb[1][2] = f1.y; rb[0][3] = f1.z; rb[1][3] = f1.w;
       }
+    } else {
+#pragma unroll
+      for (int w = 0; w < 2; ++w)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int k = k0 + kt + 4 * r;
+          const int row = FOLD == 1 ? (w == 0 ? k : L - 1 - k) : 2 * k + w;  // negative: offset past the end, reads 0
+          rb[w][r] = bload(rB, unsigned(row * bK + bcol[0]) * 4u, 0u);
+        }
+    }
   };
   auto sstore = [&](const int buf) {
+    const int ka = 4 * (tid & 3), ma = tid >> 2;
 #pragma unroll
-    for (int q = 0; q < NS; ++q)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) As[buf][q][tid & 15][(tid >> 4) + 16 * r] = ra[q][r];
+    for (int q = 0; q < NS; ++q) {
+      As[buf][q][ka + 0][ma] = ra[q].x;
+      As[buf][q][ka + 1][ma] = ra[q].y;
+      As[buf][q][ka + 2][ma] = ra[q].z;
+      As[buf][q][ka + 3][ma] = ra[q].w;
+    }
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      const int k = B_K_CONTIG ? (tid & 15) : (tid >> 6) + 4 * r;
-      const int n = B_K_CONTIG ? (tid >> 4) + 16 * r : (tid & 63);
+      const int k = B_K_CONTIG ? ka + r : (tid >> 6) + 4 * r;
+      const int n = B_K_CONTIG ? ma : (tid & 63);
       if (FOLD == 1) {
         Bs[buf][0][k][n] = odd_tile ? rb[0][r] - rb[1][r] : rb[0][r] + rb[1][r];
       } else {
