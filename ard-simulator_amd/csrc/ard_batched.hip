@@ -82,6 +82,7 @@ struct BatchedState {
   GemmDesc* d_gemm[4] = {};   // FT1, FT2, IT1, IT2
   int ngemm[4] = {}, gemm_tiles[4] = {};
   GemmDesc* d_gemm_fold[4] = {};  // the same passes even/odd folded (transform variant 3)
+  int2* d_fold_tiles[4] = {};      // their tile tables (fold_tile_table)
   int gemm_fold_tiles[4] = {};
   struct ModalGroup {
     int rpw;
@@ -211,6 +212,18 @@ int fold_tables(Ctx& c, int L, const float** fwd, const float** inv) {
   *fwd = it->second.first;
   *inv = it->second.second;
   return ARD_OK;
+}
+
+// Tile table of a folded transform launch: every (descriptor, tile) pair,
+// longest K first (a stable sort keeps each descriptor's m-fastest order).
+int fold_tile_table(Ctx& c, const std::vector<GemmDesc>& v, int2** d) {
+  std::vector<int2> t;
+  for (int i = 0; i < int(v.size()); ++i) {
+    const int n = int(ceil_div(v[i].M, 64)) * v[i].tiles_n;
+    for (int k = 0; k < n; ++k) t.push_back(int2{i, k});
+  }
+  std::stable_sort(t.begin(), t.end(), [&](const int2& a, const int2& b) { return v[a.x].K > v[b.x].K; });
+  return upload_vec(c, (void**)d, t.data(), t.size() * sizeof(int2));
 }
 
 int upload_gemms(Ctx& c, std::vector<GemmDesc>& v, GemmDesc** d, int* tiles) {
@@ -583,6 +596,7 @@ int finalize_batched(Ctx& c) {
     }
     if ((r = upload_gemms(c, gm[k], &B.d_gemm[k], &B.gemm_tiles[k]))) return r;
     if ((r = upload_gemms(c, gf, &B.d_gemm_fold[k], &B.gemm_fold_tiles[k]))) return r;
+    if ((r = fold_tile_table(c, gf, &B.d_fold_tiles[k]))) return r;
   }
   {
     std::vector<long long> st;
@@ -675,7 +689,7 @@ int enqueue_step(Ctx& c, int par, PhaseTimer* T) {
   phase(ARD_PH_FOLD, [&] { return launch_fold(B.d_fold, B.d_fold_start, B.nfold, B.fold_total, st); });
   auto face = [&](int k) {
     if (B.gemm_variant == 3)
-      return launch_gemm_fold(B.d_gemm_fold[k], B.ngemm[k], B.gemm_fold_tiles[k], k % 2 == 0, k < 2 ? 1 : 2, st);
+      return launch_gemm_fold(B.d_gemm_fold[k], B.d_fold_tiles[k], B.gemm_fold_tiles[k], k % 2 == 0, k < 2 ? 1 : 2, st);
     return launch_gemm(B.d_gemm[k], B.ngemm[k], B.gemm_tiles[k], k % 2 == 0, k % 2 == 0, 0, st, B.gemm_variant);
   };
   phase(ARD_PH_FACE_FWD1, [&] { return face(0); });
@@ -765,7 +779,8 @@ int autotune(Ctx& c) {
     const float ms = time_ms([&] {
       for (int k = 0; k < 4; ++k) {
         if (v == 3)
-          launch_gemm_fold(B.d_gemm_fold[k], B.ngemm[k], B.gemm_fold_tiles[k], k % 2 == 0, k < 2 ? 1 : 2, c.stream);
+          launch_gemm_fold(B.d_gemm_fold[k], B.d_fold_tiles[k], B.gemm_fold_tiles[k], k % 2 == 0, k < 2 ? 1 : 2,
+                           c.stream);
         else
           launch_gemm(B.d_gemm[k], B.ngemm[k], B.gemm_tiles[k], k % 2 == 0, k % 2 == 0, 0, c.stream, v);
       }

@@ -11,7 +11,9 @@ extern int g_gemm_variant;
 int launch_gemm(const GemmDesc* d_descs, int ndesc, int total_tiles, bool b_k_contig, bool c_m_contig, int ep,
                 hipStream_t s, int variant = -1);  // variant -1: g_gemm_variant
 // Even/odd folded transform on the matrix cores (FP32): fold 1 forward, 2 inverse.
-int launch_gemm_fold(const GemmDesc* d_descs, int ndesc, int total_tiles, bool b_k_contig, int fold, hipStream_t s);
+// d_tiles[b] = (descriptor, tile within it) of block b (fold_tile_table).
+int launch_gemm_fold(const GemmDesc* d_descs, const int2* d_tiles, int total_tiles, bool b_k_contig, int fold,
+                     hipStream_t s);
 int launch_pml(const PmlDesc* d_descs, int ndesc, long long total_blocks, int par, float dh, float dt, float c0,
                hipStream_t s);
 int launch_interface(const BndDesc* d_descs, int ndesc, const PartAccess* d_parts, int par, long long total,

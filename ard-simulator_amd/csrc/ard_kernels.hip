@@ -408,20 +408,17 @@ __global__ __launch_bounds__(256) void k_gemm_bf16x3(const GemmDesc* __restrict_
 // each); every tile accumulates both and writes the pair.
 // ---------------------------------------------------------------------------
 template <bool B_K_CONTIG, int FOLD>
-__global__ __launch_bounds__(256) void k_gemm_mfma_fold(const GemmDesc* __restrict__ descs, int ndesc) {
+__global__ __launch_bounds__(256) void k_gemm_mfma_fold(const GemmDesc* __restrict__ descs,
+                                                        const int2* __restrict__ tiles) {
   constexpr int NS = FOLD == 2 ? 2 : 1;  // operand sets (E, O)
   __shared__ float As[2][NS][16][68];
   __shared__ float Bs[2][NS][16][68];
-  const int tile = xcd_block(int(blockIdx.x), int(gridDim.x));  // contiguous tile ranges per XCD
-  int lo = 0, hi = ndesc - 1;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (descs[mid].tile0 <= tile) lo = mid;
-    else hi = mid - 1;
-  }
-  const GemmDesc& g = descs[lo];
+  // the launch's tile table: (descriptor, tile within it), longest K first,
+  // so the long tiles start early and the short ones fill in at the end
+  const int2 tm = tiles[blockIdx.x];
+  const GemmDesc& g = descs[tm.x];
   const int M = g.M, N = g.N, K = g.K, N1 = g.N1, L = g.L, Kp = g.Kp;
-  const int t = tile - g.tile0;
+  const int t = tm.y;
   // m tiles fastest: the tiles of one B panel run together and share it in L2
   const int tiles_m = (M + 63) >> 6;
   const int m0 = (t % tiles_m) * 64;
@@ -561,13 +558,14 @@ __global__ __launch_bounds__(256) void k_gemm_mfma_fold(const GemmDesc* __restri
   }
 }
 
-int launch_gemm_fold(const GemmDesc* d_descs, int ndesc, int total_tiles, bool b_k_contig, int fold, hipStream_t s) {
+int launch_gemm_fold(const GemmDesc* d_descs, const int2* d_tiles, int total_tiles, bool b_k_contig, int fold,
+                     hipStream_t s) {
   if (total_tiles <= 0) return 0;
   dim3 grid(total_tiles), block(256);
-  if (fold == 1 && b_k_contig) hipLaunchKernelGGL((k_gemm_mfma_fold<true, 1>), grid, block, 0, s, d_descs, ndesc);
-  else if (fold == 1) hipLaunchKernelGGL((k_gemm_mfma_fold<false, 1>), grid, block, 0, s, d_descs, ndesc);
-  else if (fold == 2 && b_k_contig) hipLaunchKernelGGL((k_gemm_mfma_fold<true, 2>), grid, block, 0, s, d_descs, ndesc);
-  else if (fold == 2) hipLaunchKernelGGL((k_gemm_mfma_fold<false, 2>), grid, block, 0, s, d_descs, ndesc);
+  if (fold == 1 && b_k_contig) hipLaunchKernelGGL((k_gemm_mfma_fold<true, 1>), grid, block, 0, s, d_descs, d_tiles);
+  else if (fold == 1) hipLaunchKernelGGL((k_gemm_mfma_fold<false, 1>), grid, block, 0, s, d_descs, d_tiles);
+  else if (fold == 2 && b_k_contig) hipLaunchKernelGGL((k_gemm_mfma_fold<true, 2>), grid, block, 0, s, d_descs, d_tiles);
+  else if (fold == 2) hipLaunchKernelGGL((k_gemm_mfma_fold<false, 2>), grid, block, 0, s, d_descs, d_tiles);
   else return -1;
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
