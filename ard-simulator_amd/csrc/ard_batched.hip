@@ -214,15 +214,41 @@ int fold_tables(Ctx& c, int L, const float** fwd, const float** inv) {
   return ARD_OK;
 }
 
-// Tile table of a folded transform launch: every (descriptor, tile) pair,
-// longest K first (a stable sort keeps each descriptor's m-fastest order).
+// Tile table of a folded transform launch: every (descriptor, tile) pair.
+// A panel (one n tile of a descriptor, all its m tiles) shares its B operand,
+// so a panel goes to one XCD (block b runs on XCD b % 8); panels are dealt
+// longest K first to the XCD with the least work so far, and each XCD's
+// blocks run longest first.
 int fold_tile_table(Ctx& c, const std::vector<GemmDesc>& v, int2** d) {
-  std::vector<int2> t;
+  struct Panel { int desc, nt, tm; };
+  std::vector<Panel> panels;
   for (int i = 0; i < int(v.size()); ++i) {
-    const int n = int(ceil_div(v[i].M, 64)) * v[i].tiles_n;
-    for (int k = 0; k < n; ++k) t.push_back(int2{i, k});
+    const int tm = int(ceil_div(v[i].M, 64));
+    for (int nt = 0; nt < v[i].tiles_n; ++nt) panels.push_back(Panel{i, nt, tm});
   }
-  std::stable_sort(t.begin(), t.end(), [&](const int2& a, const int2& b) { return v[a.x].K > v[b.x].K; });
+  std::stable_sort(panels.begin(), panels.end(), [&](const Panel& a, const Panel& b) { return v[a.desc].K > v[b.desc].K; });
+  constexpr int kXcd = 8;
+  std::vector<int2> lists[kXcd];
+  long long work[kXcd] = {};
+  for (const Panel& p : panels) {
+    const int x = int(std::min_element(work, work + kXcd) - work);
+    for (int m = 0; m < p.tm; ++m) lists[x].push_back(int2{p.desc, p.nt * p.tm + m});
+    work[x] += (long long)p.tm * v[p.desc].K;
+  }
+  std::vector<int2> t;
+  size_t pos[kXcd] = {};
+  size_t left = 0;
+  for (auto& l : lists) left += l.size();
+  while (left) {  // block b from XCD b % 8's list; an exhausted list yields to the longest remaining one
+    int x = int(t.size() % kXcd);
+    if (pos[x] == lists[x].size()) {
+      size_t best = 0;
+      for (int y = 0; y < kXcd; ++y)
+        if (lists[y].size() - pos[y] > best) best = lists[y].size() - pos[y], x = y;
+    }
+    t.push_back(lists[x][pos[x]++]);
+    --left;
+  }
   return upload_vec(c, (void**)d, t.data(), t.size() * sizeof(int2));
 }
 
