@@ -710,7 +710,9 @@ int enqueue_step(Ctx& c, int par, PhaseTimer* T) {
     r |= hipStreamWaitEvent(B.side, B.ev_fork, 0) != hipSuccess;
   }
   hipStream_t sp = fork ? B.side : st;
-  phase(ARD_PH_PML, [&] { return launch_pml(B.d_pml, B.npml, B.pml_blocks, par, c.prm.h, c.prm.dt, c.prm.c0, sp); });
+  phase(ARD_PH_PML, [&] {
+    return launch_pml(B.d_pml, B.npml, B.pml_blocks, par, c.prm.pml_layers, c.prm.h, c.prm.dt, c.prm.c0, sp);
+  });
   if (fork) r |= hipEventRecord(B.ev_join, B.side) != hipSuccess;
   phase(ARD_PH_FOLD, [&] { return launch_fold(B.d_fold, B.d_fold_start, B.nfold, B.fold_total, st); });
   auto face = [&](int k) {

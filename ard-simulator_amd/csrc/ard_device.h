@@ -35,6 +35,10 @@ __device__ __forceinline__ brsrc make_rsrc(const void* p, unsigned nbytes) {
 __device__ __forceinline__ float bload(brsrc r, unsigned voff, unsigned soff) {
   return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, int(voff), int(soff), 0));
 }
+// stores at byte offsets at or past n are dropped
+__device__ __forceinline__ void bstore(float v, brsrc r, unsigned voff) {
+  __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, int(voff), 0, 0);
+}
 
 // One problem of a grouped, strided GEMM  C = alpha * A * B  (A: M x K row-major).
 // B(k, n) = B[k*bK + (n % N1)*b1 + (n / N1)*b2], C(m, n) likewise with cM, c1, c2.
@@ -81,6 +85,7 @@ struct PmlDesc {
   float* f;
   const float* zprof;  // zeta along that axis (length = W, H or D)
   long long block0;    // first block of this slab in a grouped launch
+  int nb;              // tiled update: tiles along the first axis across the normal (0: one thread per cell)
 };
 
 // Pressure / force access to one partition as seen by the interface kernel.

@@ -14,8 +14,13 @@ int launch_gemm(const GemmDesc* d_descs, int ndesc, int total_tiles, bool b_k_co
 // d_tiles[b] = (descriptor, tile within it) of block b (fold_tile_table).
 int launch_gemm_fold(const GemmDesc* d_descs, const int2* d_tiles, int total_tiles, bool b_k_contig, int fold,
                      hipStream_t s);
-int launch_pml(const PmlDesc* d_descs, int ndesc, long long total_blocks, int par, float dh, float dt, float c0,
-               hipStream_t s);
+// Blocks of one slab's update for PML thickness L (the slab's extent along
+// its normal axis); sets d.nb. Slabs 3 to 8 cells thick take the LDS-tiled
+// update, thicker ones one thread per cell. Every slab of one launch must
+// share L; -1 if the slab is not L cells thick.
+long long pml_slab_blocks(PmlDesc& d, int L);
+int launch_pml(const PmlDesc* d_descs, int ndesc, long long total_blocks, int par, int L, float dh, float dt,
+               float c0, hipStream_t s);
 int launch_interface(const BndDesc* d_descs, int ndesc, const PartAccess* d_parts, int par, long long total,
                      hipStream_t s);
 int launch_sources(const SourceDesc* d_src, int n, const int* d_step, float courant, hipStream_t s);

@@ -711,11 +711,252 @@ __global__ __launch_bounds__(256) void k_pml(const PmlDesc* __restrict__ descs, 
   if (fc != 0.f) f[cell] = 0.f;  // force cleared after the update (:199)
 }
 
-int launch_pml(const PmlDesc* d_descs, int ndesc, long long total_blocks, int par, float dh, float dt, float c0,
-               hipStream_t s) {
+// ---------------------------------------------------------------------------
+// LDS-tiled PML slab update, same arithmetic as k_pml. A block of 256 threads
+// owns a tile that spans the slab's whole thickness L along its normal axis
+// and 32 x 8 (normal z or y) or 16 x 16 (normal x) cells across it, 256 L
+// cells, L per thread in memory order. p is staged with a 3-cell halo across
+// the normal, each phi_a with a 2-cell halo along a when a lies across the
+// normal, all from buffer loads that read 0 outside the slab; along the
+// normal the tile covers the slab, so neighbours there outside [0, L) are the
+// zero sentinel and are skipped. Each p and phi value is loaded from global
+// memory about twice per tile instead of 21 and 5 times per cell.
+// ---------------------------------------------------------------------------
+template <int L, int AX>
+struct PmlTile {
+  static constexpr int TX = AX == 0 ? L : 32;
+  static constexpr int TY = AX == 1 ? L : (AX == 0 ? 16 : 8);
+  static constexpr int TZ = AX == 2 ? L : (AX == 0 ? 16 : 8);
+  // p halo per axis (none along the normal) and its staged box
+  static constexpr int HX = AX == 0 ? 0 : 3, HY = AX == 1 ? 0 : 3, HZ = AX == 2 ? 0 : 3;
+  static constexpr int PX = TX + 2 * HX, PY = TY + 2 * HY, PZ = TZ + 2 * HZ;
+  // phi_a halo along a (none along the normal)
+  static constexpr int GX = AX == 0 ? 0 : 2, GY = AX == 1 ? 0 : 2, GZ = AX == 2 ? 0 : 2;
+  static constexpr int NP = PX * PY * PZ;
+  static constexpr int N0 = (TX + 2 * GX) * TY * TZ, N1 = TX * (TY + 2 * GY) * TZ, N2 = TX * TY * (TZ + 2 * GZ);
+  static constexpr int total = NP + N0 + N1 + N2;
+};
+
+template <int L>
+constexpr int pml_tile_lds() {
+  constexpr int a = PmlTile<L, 0>::total, b = PmlTile<L, 1>::total, c = PmlTile<L, 2>::total;
+  return a > b ? (a > c ? a : c) : (b > c ? b : c);
+}
+
+// Staging of the box [x0, x0 + BX) x [y0, y0 + BY) x [z0, z0 + BZ) of one
+// slab array (x fastest), cells outside the slab reading 0: every load of
+// every box is issued first (pml_fetch), then written to LDS (pml_put).
+template <int BX, int BY, int BZ>
+struct PmlBox {
+  static constexpr int N = BX * BY * BZ, Q = (N + 255) / 256;  // elements, loads per thread
+};
+
+template <int BX, int BY, int BZ, int Q>
+__device__ __forceinline__ void pml_fetch(float (&v)[Q], brsrc r, int x0, int y0, int z0, int W, int H, int D) {
+  constexpr int N = PmlBox<BX, BY, BZ>::N;
+  static_assert(Q == PmlBox<BX, BY, BZ>::Q, "one register per 256 box elements");
+  constexpr unsigned kOut = 0x80000000u;
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    const int e = int(threadIdx.x) + 256 * q;
+    const int x = e % BX, y = (e / BX) % BY, z = e / (BX * BY);
+    const unsigned gx = unsigned(x0 + x), gy = unsigned(y0 + y), gz = unsigned(z0 + z);
+    const bool in = (e < N) & (gx < unsigned(W)) & (gy < unsigned(H)) & (gz < unsigned(D));
+    const unsigned off = ((gz * unsigned(H) + gy) * unsigned(W) + gx) * 4u;
+    v[q] = bload(r, in ? off : kOut, 0u);
+  }
+}
+
+template <int BX, int BY, int BZ, int Q>
+__device__ __forceinline__ void pml_put(float* __restrict__ s, const float (&v)[Q]) {
+  static_assert(Q == PmlBox<BX, BY, BZ>::Q, "one register per 256 box elements");
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    const int e = int(threadIdx.x) + 256 * q;
+    if (e < PmlBox<BX, BY, BZ>::N) s[e] = v[q];
+  }
+}
+
+// s[c + d * stride] for a neighbour at d cells along the normal from a cell
+// at normal position ln, 0 (the zero sentinel) outside [0, L); the address is
+// clamped into the tile so every lane reads inside the staged box.
+template <int L>
+__device__ __forceinline__ float pml_normal(const float* __restrict__ s, int c, int ln, int d, int stride) {
+  const int nd = min(max(ln + d, 0), L - 1);
+  const float v = s[c + (nd - ln) * stride];
+  return unsigned(ln + d) < unsigned(L) ? v : 0.f;
+}
+
+template <int L, int AX>
+__device__ __forceinline__ void pml_tile(const PmlDesc& P, int t, float* __restrict__ sm, int par, float r180,
+                                         float r12, float dt, float rdt, float c0) {
+  using T = PmlTile<L, AX>;
+  using BP = PmlBox<T::PX, T::PY, T::PZ>;
+  using B0 = PmlBox<T::TX + 2 * T::GX, T::TY, T::TZ>;
+  using B1 = PmlBox<T::TX, T::TY + 2 * T::GY, T::TZ>;
+  using B2 = PmlBox<T::TX, T::TY, T::TZ + 2 * T::GZ>;
+  const int W = P.W, H = P.H, D = P.D;
+  const int ta = t % P.nb, tb = t / P.nb;
+  const int x0 = AX == 0 ? 0 : ta * T::TX;
+  const int y0 = AX == 0 ? ta * T::TY : (AX == 1 ? 0 : tb * T::TY);
+  const int z0 = AX == 2 ? 0 : tb * T::TZ;
+  const unsigned nbytes = unsigned(W * H * D) * 4u;  // slabs hold < 2^29 cells (checked at setup)
+  constexpr unsigned kOut = 0x80000000u;
+  // the thread's L cells in memory order: tile coordinates, byte offsets
+  // (kOut beyond the slab, where loads read 0 and stores are dropped), and
+  // their p_old, force and zeta
+  int lx[L], ly[L], lz[L];
+  unsigned coff[L];
+  float po_r[L], f_r[L], z_r[L];
+  const brsrc rpn = make_rsrc(P.p[par ^ 1], nbytes), rf = make_rsrc(P.f, nbytes);
+  {
+    const brsrc rz = make_rsrc(P.zprof, unsigned(L) * 4u);
+#pragma unroll
+    for (int r = 0; r < L; ++r) {
+      const int e = int(threadIdx.x) + 256 * r;
+      lx[r] = e % T::TX;
+      ly[r] = (e / T::TX) % T::TY;
+      lz[r] = AX == 2 ? r : e / (T::TX * T::TY);  // a z-normal tile is 32 x 8 = 256 cells per plane
+      const int i = x0 + lx[r], j = y0 + ly[r], k = z0 + lz[r];
+      coff[r] = ((i < W) & (j < H) & (k < D)) ? unsigned((k * H + j) * W + i) * 4u : kOut;
+      po_r[r] = bload(rpn, coff[r], 0u);
+      f_r[r] = bload(rf, coff[r], 0u);
+      z_r[r] = bload(rz, unsigned(AX == 0 ? lx[r] : (AX == 1 ? ly[r] : lz[r])) * 4u, 0u);
+    }
+  }
+  float* __restrict__ sp = sm;
+  float* __restrict__ s0 = sp + T::NP;
+  float* __restrict__ s1 = s0 + T::N0;
+  float* __restrict__ s2 = s1 + T::N1;
+  {
+    float vp[BP::Q], v0[B0::Q], v1[B1::Q], v2[B2::Q];
+    pml_fetch<T::PX, T::PY, T::PZ>(vp, make_rsrc(P.p[par], nbytes), x0 - T::HX, y0 - T::HY, z0 - T::HZ, W, H, D);
+    pml_fetch<T::TX + 2 * T::GX, T::TY, T::TZ>(v0, make_rsrc(P.phi[par][0], nbytes), x0 - T::GX, y0, z0, W, H, D);
+    pml_fetch<T::TX, T::TY + 2 * T::GY, T::TZ>(v1, make_rsrc(P.phi[par][1], nbytes), x0, y0 - T::GY, z0, W, H, D);
+    pml_fetch<T::TX, T::TY, T::TZ + 2 * T::GZ>(v2, make_rsrc(P.phi[par][2], nbytes), x0, y0, z0 - T::GZ, W, H, D);
+    pml_put<T::PX, T::PY, T::PZ>(sp, vp);
+    pml_put<T::TX + 2 * T::GX, T::TY, T::TZ>(s0, v0);
+    pml_put<T::TX, T::TY + 2 * T::GY, T::TZ>(s1, v1);
+    pml_put<T::TX, T::TY, T::TZ + 2 * T::GZ>(s2, v2);
+  }
+  __syncthreads();
+  const brsrc rq0 = make_rsrc(P.phi[par ^ 1][0], nbytes), rq1 = make_rsrc(P.phi[par ^ 1][1], nbytes),
+              rq2 = make_rsrc(P.phi[par ^ 1][2], nbytes);
+  const float c6[7] = {2.f, -27.f, 270.f, -490.f, 270.f, -27.f, 2.f};
+  const float c4[5] = {1.f, -8.f, 0.f, 8.f, -1.f};
+  constexpr int SX1 = T::TX + 2 * T::GX;  // row length of the phi_x box
+  constexpr int SY1 = T::TY + 2 * T::GY;  // column height of the phi_y box
+  constexpr int SPZ = T::PX * T::PY;      // plane of the p box
+#pragma unroll
+  for (int r = 0; r < L; ++r) {
+    // cells beyond the slab in a partial tile compute on zeros; their stores are dropped
+    const int ln = AX == 0 ? lx[r] : (AX == 1 ? ly[r] : lz[r]);  // position along the normal
+    const int cp = ((lz[r] + T::HZ) * T::PY + ly[r] + T::HY) * T::PX + lx[r] + T::HX;
+    float px[7], py[7], pz[7];
+#pragma unroll
+    for (int m = 0; m < 7; ++m) {
+      const int d = m - 3;
+      px[m] = AX == 0 ? pml_normal<L>(sp, cp, ln, d, 1) : sp[cp + d];
+      py[m] = AX == 1 ? pml_normal<L>(sp, cp, ln, d, T::PX) : sp[cp + d * T::PX];
+      pz[m] = AX == 2 ? pml_normal<L>(sp, cp, ln, d, SPZ) : sp[cp + d * SPZ];
+    }
+    float dxx = 0.f, dyy = 0.f, dzz = 0.f;
+#pragma unroll
+    for (int m = 0; m < 7; ++m) {
+      dxx += c6[m] * px[m];
+      dyy += c6[m] * py[m];
+      dzz += c6[m] * pz[m];
+    }
+    dxx *= r180;
+    dyy *= r180;
+    dzz *= r180;
+    const float z = z_r[r];
+    const float zx = AX == 0 ? z : 0.f, zy = AX == 1 ? z : 0.f, zz = AX == 2 ? z : 0.f;
+    const float pc = px[3];
+    const float po = po_r[r];
+    const float term1 = 2.f * pc;
+    const float term2 = -po;
+    const float term3 = c0 * c0 * (dxx + dyy + dzz);
+    const float term4 = -(zx + zy + zz) * (pc - po) * rdt;
+    const float term5 = -(zx * zy + zy * zz + zx * zz) * pc;
+    const int c0i = (lz[r] * T::TY + ly[r]) * SX1 + lx[r] + T::GX;
+    const int c1i = (lz[r] * SY1 + ly[r] + T::GY) * T::TX + lx[r];
+    const int c2i = ((lz[r] + T::GZ) * T::TY + ly[r]) * T::TX + lx[r];
+    float dpx = 0.f, dpy = 0.f, dpz = 0.f, dux = 0.f, duy = 0.f, duz = 0.f;
+#pragma unroll
+    for (int m = 0; m < 5; ++m) {
+      if (m == 2) continue;  // zero coefficient
+      const int d = m - 2;
+      const float gx = AX == 0 ? pml_normal<L>(s0, c0i, ln, d, 1) : s0[c0i + d];
+      const float gy = AX == 1 ? pml_normal<L>(s1, c1i, ln, d, T::TX) : s1[c1i + d * T::TX];
+      const float gz = AX == 2 ? pml_normal<L>(s2, c2i, ln, d, T::TX * T::TY) : s2[c2i + d * T::TX * T::TY];
+      dpx += c4[m] * gx;
+      dpy += c4[m] * gy;
+      dpz += c4[m] * gz;
+      dux += c4[m] * px[m + 1];
+      duy += c4[m] * py[m + 1];
+      duz += c4[m] * pz[m + 1];
+    }
+    dpx *= r12; dpy *= r12; dpz *= r12;
+    dux *= r12; duy *= r12; duz *= r12;
+    const float term6 = dpx + dpy + dpz;
+    const float fc = f_r[r];
+    bstore(term1 + term2 + dt * dt * (term3 + term4 + term5 + term6 + fc), rpn, coff[r]);
+    const float phx = s0[c0i], phy = s1[c1i], phz = s2[c2i];
+    bstore(phx - dt * zx * phx + dt * (zy + zz - zx) * dux, rq0, coff[r]);
+    bstore(phy - dt * zy * phy + dt * (zx + zz - zy) * duy, rq1, coff[r]);
+    bstore(phz - dt * zz * phz + dt * (zx + zy - zz) * duz, rq2, coff[r]);
+    bstore(0.f, rf, fc != 0.f ? coff[r] : kOut);  // force cleared after the update (ARD/pml_partition.cpp:199)
+  }
+}
+
+template <int L>
+__global__ __launch_bounds__(256) void k_pml_tiled(const PmlDesc* __restrict__ descs, int ndesc, int par,
+                                                   float r180, float r12, float dt, float rdt, float c0) {
+  __shared__ float sm[pml_tile_lds<L>()];
+  const long long b = xcd_block(int(blockIdx.x), int(gridDim.x));
+  int lo = 0, hi = ndesc - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (descs[mid].block0 <= b) lo = mid;
+    else hi = mid - 1;
+  }
+  const PmlDesc& P = descs[lo];
+  const int t = int(b - P.block0);
+  if (P.axis == 0) pml_tile<L, 0>(P, t, sm, par, r180, r12, dt, rdt, c0);
+  else if (P.axis == 1) pml_tile<L, 1>(P, t, sm, par, r180, r12, dt, rdt, c0);
+  else pml_tile<L, 2>(P, t, sm, par, r180, r12, dt, rdt, c0);
+}
+
+long long pml_slab_blocks(PmlDesc& d, int L) {
+  const long long n = (long long)d.W * d.H * d.D;
+  const int ext[3] = {d.W, d.H, d.D};
+  if (L < 3 || L > 8) {
+    d.nb = 0;
+    return (n + 255) / 256;
+  }
+  if (ext[d.axis] != L) return -1;  // the tiled update needs the slab L cells thick
+  // tile extents across the normal: first and second axis other than it
+  const int a0 = d.axis == 0 ? 1 : 0, a1 = d.axis == 2 ? 1 : 2;
+  const int t0 = d.axis == 0 ? 16 : 32, t1 = d.axis == 0 ? 16 : 8;
+  d.nb = (ext[a0] + t0 - 1) / t0;
+  return (long long)d.nb * ((ext[a1] + t1 - 1) / t1);
+}
+
+int launch_pml(const PmlDesc* d_descs, int ndesc, long long total_blocks, int par, int L, float dh, float dt,
+               float c0, hipStream_t s) {
   if (total_blocks <= 0) return 0;
   const float r180 = 1.f / (180.f * dh * dh), r12 = 1.f / (12.f * dh), rdt = 1.f / dt;
-  hipLaunchKernelGGL(k_pml, dim3(unsigned(total_blocks)), dim3(256), 0, s, d_descs, ndesc, par, r180, r12, dt, rdt, c0);
+  const dim3 grid(static_cast<unsigned>(total_blocks)), block(256);
+  switch (L) {
+    case 3: hipLaunchKernelGGL(k_pml_tiled<3>, grid, block, 0, s, d_descs, ndesc, par, r180, r12, dt, rdt, c0); break;
+    case 4: hipLaunchKernelGGL(k_pml_tiled<4>, grid, block, 0, s, d_descs, ndesc, par, r180, r12, dt, rdt, c0); break;
+    case 5: hipLaunchKernelGGL(k_pml_tiled<5>, grid, block, 0, s, d_descs, ndesc, par, r180, r12, dt, rdt, c0); break;
+    case 6: hipLaunchKernelGGL(k_pml_tiled<6>, grid, block, 0, s, d_descs, ndesc, par, r180, r12, dt, rdt, c0); break;
+    case 7: hipLaunchKernelGGL(k_pml_tiled<7>, grid, block, 0, s, d_descs, ndesc, par, r180, r12, dt, rdt, c0); break;
+    case 8: hipLaunchKernelGGL(k_pml_tiled<8>, grid, block, 0, s, d_descs, ndesc, par, r180, r12, dt, rdt, c0); break;
+    default: hipLaunchKernelGGL(k_pml, grid, block, 0, s, d_descs, ndesc, par, r180, r12, dt, rdt, c0); break;
+  }
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 

@@ -240,9 +240,11 @@ int setup_common(Ctx& c) {
     d.f = B.f;
     d.zprof = B.zprof;
     d.block0 = 0;
+    const long long nblk = pml_slab_blocks(d, c.prm.pml_layers);
+    if (nblk < 0) return set_error(ARD_ERR_INVALID, "a PML slab thinner or thicker than pml_layers");
     B.desc = int(c.h_pml.size());
     c.h_pml.push_back(d);
-    c.pml_blocks.push_back(ceil_div(n, 256));
+    c.pml_blocks.push_back(nblk);
   }
   if (!c.h_pml.empty()) {
     if ((r = c.dalloc((void**)&c.d_pml, c.h_pml.size() * sizeof(PmlDesc)))) return r;
@@ -424,7 +426,8 @@ int step_plain(Ctx& c, int n) {
     }
     for (size_t i = 0; i < c.h_pml.size(); ++i) {
       ev = T.begin(c.stream, ARD_PH_PML);
-      if (launch_pml(c.d_pml + i, 1, c.pml_blocks[i], par, c.prm.h, c.prm.dt, c.prm.c0, c.stream))
+      if (launch_pml(c.d_pml + i, 1, c.pml_blocks[i], par, c.prm.pml_layers, c.prm.h, c.prm.dt, c.prm.c0,
+                     c.stream))
         return set_error(ARD_ERR_HIP, "PML launch");
       T.end(c.stream, ev);
     }
