@@ -964,45 +964,45 @@ int launch_pml(const PmlDesc* d_descs, int ndesc, long long total_blocks, int pa
 // Interface forcing (ARD/boundary.cpp:21-173). One thread per face element
 // computes the 6 outputs m = -3..2 from the 3 cells on each side.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ long long shell_index(const PartAccess& A, int x, int y, int z, int* which) {
-  // face-shell storage of the batched form: a cell belongs to the x-shell if
-  // it is within 3 of an x face, else to the y-shell, else to the z-shell.
+// Address of cell (x, y, z) of a partition's pressure (current step) or
+// force, with selects only (no branch on the lane's cell), so that a thread's
+// six reads issue together. *in = false outside a dense field: the address is
+// then a clamped cell and the caller reads 0 (the zero sentinel).
+// Face-shell storage of the batched form: a cell belongs to the x-shell
+// [6][D][H] if it is within 3 of an x face, else to the y-shell [6][D][W],
+// else to the z-shell [6][H][W]; plane q < 3 is cell q from the low face,
+// q >= 3 cell q - 6 from the high one.
+__device__ __forceinline__ const float* shell_addr(const PartAccess& A, float* const* sh, int x, int y, int z) {
   const int W = A.W, H = A.H, D = A.D;
-  if (x < 3 || x >= W - 3) {
-    const int q = x < 3 ? x : x - W + 6;
-    *which = 0;
-    return ((long long)q * D + z) * H + y;
-  }
-  if (y < 3 || y >= H - 3) {
-    const int q = y < 3 ? y : y - H + 6;
-    *which = 1;
-    return ((long long)q * D + z) * W + x;
-  }
-  const int q = z < 3 ? z : z - D + 6;
-  *which = 2;
-  return ((long long)q * H + y) * W + x;
+  // the three shell bases are block-uniform: held in scalar registers, so the
+  // compiler selects between addresses instead of loading a lane's base
+  float* b0 = sh[0];
+  float* b1 = sh[1];
+  float* b2 = sh[2];
+  asm volatile("" : "+s"(b0), "+s"(b1), "+s"(b2));
+  const bool sx = x < 3 || x >= W - 3;
+  const bool sy = y < 3 || y >= H - 3;
+  const int qx = x < 3 ? x : x - W + 6, qy = y < 3 ? y : y - H + 6, qz = z < 3 ? z : z - D + 6;
+  // shells hold 6 D H, 6 D W, 6 H W < 2^30 cells (partitions are at least 6 cells wide)
+  const unsigned ox = (unsigned(qx * D + z) * unsigned(H) + unsigned(y));
+  const unsigned oy = (unsigned(qy * D + z) * unsigned(W) + unsigned(x));
+  const unsigned oz = (unsigned(qz * H + y) * unsigned(W) + unsigned(x));
+  return sx ? b0 + ox : (sy ? b1 + oy : b2 + oz);
 }
 
-__device__ __forceinline__ float read_p(const PartAccess& A, int par, int x, int y, int z) {
-  if (A.kind == 2) {
-    int w;
-    const long long o = shell_index(A, x, y, z, &w);
-    return ARD_CG(A.ps[w])[o];
+__device__ __forceinline__ const float* p_addr(const PartAccess& A, int par, int x, int y, int z, bool* in) {
+  if (A.kind == 2) {  // uniform across the block
+    *in = true;
+    return shell_addr(A, A.ps, x, y, z);
   }
-  const bool in = !(x < 0 || x >= A.W || y < 0 || y >= A.H || z < 0 || z >= A.D);
+  *in = !(x < 0 || x >= A.W || y < 0 || y >= A.H || z < 0 || z >= A.D);
   const int xc = min(max(x, 0), A.W - 1), yc = min(max(y, 0), A.H - 1), zc = min(max(z, 0), A.D - 1);
-  const float v = ARD_CG(A.p[par])[((long long)zc * A.H + yc) * A.W + xc];
-  return in ? v : 0.f;
+  return A.p[par] + (((long long)zc * A.H + yc) * A.W + xc);
 }
 
-__device__ __forceinline__ void write_f(const PartAccess& A, int x, int y, int z, float v) {
-  if (A.kind == 2) {
-    int w;
-    const long long o = shell_index(A, x, y, z, &w);
-    ARD_G(A.fs[w])[o] = v;
-    return;
-  }
-  ARD_G(A.f)[((long long)z * A.H + y) * A.W + x] = v;
+__device__ __forceinline__ float* f_addr(const PartAccess& A, int x, int y, int z) {
+  if (A.kind == 2) return const_cast<float*>(shell_addr(A, A.fs, x, y, z));
+  return A.f + (((long long)z * A.H + y) * A.W + x);
 }
 
 __global__ __launch_bounds__(256) void k_interface(const BndDesc* __restrict__ descs, int ndesc,
@@ -1030,19 +1030,30 @@ __global__ __launch_bounds__(256) void k_interface(const BndDesc* __restrict__ d
   const int u0 = u - B.org[0][0], v0 = v - B.org[0][1];
   const int u1 = u - B.org[1][0], v1 = v - B.org[1][1];
   const int n0 = B.type == 0 ? S0.W : (B.type == 1 ? S0.H : S0.D);
-  float pl[3], pr[3];
+  // the element's cells: side 0 at normal n0-3..n0-1, side 1 at 0..2
+  const float* ad[6];
+  bool in[6];
 #pragma unroll
   for (int n = 0; n < 3; ++n) {
     if (B.type == 0) {
-      pl[n] = read_p(S0, par, n0 - 3 + n, u0, v0);
-      pr[n] = read_p(S1, par, n, u1, v1);
+      ad[n] = p_addr(S0, par, n0 - 3 + n, u0, v0, &in[n]);
+      ad[3 + n] = p_addr(S1, par, n, u1, v1, &in[3 + n]);
     } else if (B.type == 1) {
-      pl[n] = read_p(S0, par, u0, n0 - 3 + n, v0);
-      pr[n] = read_p(S1, par, u1, n, v1);
+      ad[n] = p_addr(S0, par, u0, n0 - 3 + n, v0, &in[n]);
+      ad[3 + n] = p_addr(S1, par, u1, n, v1, &in[3 + n]);
     } else {
-      pl[n] = read_p(S0, par, u0, v0, n0 - 3 + n);
-      pr[n] = read_p(S1, par, u1, v1, n);
+      ad[n] = p_addr(S0, par, u0, v0, n0 - 3 + n, &in[n]);
+      ad[3 + n] = p_addr(S1, par, u1, v1, n, &in[3 + n]);
     }
+  }
+  float pv[6];
+#pragma unroll
+  for (int q = 0; q < 6; ++q) pv[q] = *ARD_CG(ad[q]);
+  float pl[3], pr[3];
+#pragma unroll
+  for (int n = 0; n < 3; ++n) {
+    pl[n] = in[n] ? pv[n] : 0.f;
+    pr[n] = in[3 + n] ? pv[3 + n] : 0.f;
   }
   const float C[6][6] = {{0, 0, -2, 2, 0, 0},       {0, -2, 27, -27, 2, 0},  {-2, 27, -270, 270, -27, 2},
                          {2, -27, 270, -270, 27, -2}, {0, 2, -27, 27, -2, 0}, {0, 0, 2, -2, 0, 0}};
@@ -1059,15 +1070,14 @@ __global__ __launch_bounds__(256) void k_interface(const BndDesc* __restrict__ d
     if (m < 0) {
       const float sip = B.self[0] ? (s1 + s2) : s2;
       const float val = sip * B.k[0];
-      if (B.type == 0) write_f(S0, n0 + m, u0, v0, val);
-      else if (B.type == 1) write_f(S0, u0, n0 + m, v0, val);
-      else write_f(S0, u0, v0, n0 + m, val);
+      float* fa = B.type == 0 ? f_addr(S0, n0 + m, u0, v0)
+                              : (B.type == 1 ? f_addr(S0, u0, n0 + m, v0) : f_addr(S0, u0, v0, n0 + m));
+      *ARD_G(fa) = val;
     } else {
       const float sip = B.self[1] ? (s1 + s2) : s1;
       const float val = sip * B.k[1];
-      if (B.type == 0) write_f(S1, m, u1, v1, val);
-      else if (B.type == 1) write_f(S1, u1, m, v1, val);
-      else write_f(S1, u1, v1, m, val);
+      float* fa = B.type == 0 ? f_addr(S1, m, u1, v1) : (B.type == 1 ? f_addr(S1, u1, m, v1) : f_addr(S1, u1, v1, m));
+      *ARD_G(fa) = val;
     }
   }
 }

@@ -122,7 +122,7 @@ int build_boundary_descs(Ctx& c, std::vector<BndDesc>& out, std::vector<long lon
 }
 
 // Which face slab stores cell (x,y,z) of a W x H x D partition, and where
-// (host twin of shell_index, ard_kernels.hip); -1 for an interior cell.
+// (host twin of shell_addr, ard_kernels.hip); -1 for an interior cell.
 int shell_of(int W, int H, int D, int x, int y, int z, long long* off) {
   if (x < 3 || x >= W - 3) {
     const int q = x < 3 ? x : x - W + 6;
