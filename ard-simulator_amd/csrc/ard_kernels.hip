@@ -541,12 +541,33 @@ __global__ __launch_bounds__(256) void k_gemm_mfma_fold(const GemmDesc* __restri
   if (jl >= (B_K_CONTIG ? M : N)) return;
   long long cn = 0;
   if (!B_K_CONTIG) cn = (long long)(jl % N1) * g.c1 + (long long)(jl / N1) * g.c2;
+  // K-contiguous B: the lane's 16 columns are nb + 8 (r / 4) + r % 4; one
+  // quotient and remainder by N1 for nb (a reciprocal estimate, corrected),
+  // then each column's by stepping the remainder past N1
+  const int nb = n0 + wn + 4 * (lane >> 5);
+  int nq = 0, nr = 0;
+  if (B_K_CONTIG) {
+    nq = int(float(nb) * (1.f / float(N1)));
+    nr = nb - nq * N1;
+    while (nr < 0) { nr += N1; --nq; }
+    while (nr >= N1) { nr -= N1; ++nq; }
+  }
 #pragma unroll
   for (int r = 0; r < 16; ++r) {
     const int il = (B_K_CONTIG ? n0 + wn : m0 + wm) + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
     if (il >= (B_K_CONTIG ? N : M)) continue;
     const int i = B_K_CONTIG ? jl : il;
-    if (B_K_CONTIG) cn = (long long)(il % N1) * g.c1 + (long long)(il / N1) * g.c2;
+    if (B_K_CONTIG) {
+      int q = nq, rr = nr + 8 * (r >> 2) + (r & 3);  // rr < N1 + 28
+      if (N1 >= 28) {  // uniform: one step at most
+        const bool w = rr >= N1;
+        rr -= w ? N1 : 0;
+        q += w ? 1 : 0;
+      } else {
+        while (rr >= N1) { rr -= N1; ++q; }
+      }
+      cn = (long long)rr * g.c1 + (long long)q * g.c2;
+    }
     if (FOLD == 1) {
       const int k = i < g.Mh ? 2 * i : 2 * (i - g.Mh) + 1;
       if (k < L) C[(long long)k * g.cM + cn] = g.alpha * acc[0][r];
