@@ -535,14 +535,26 @@ __global__ __launch_bounds__(256) void k_modal(const ModalDesc* __restrict__ des
 // ---------------------------------------------------------------------------
 // Recorder samples from the block partials, then the step counter.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_brecord(const BRecDesc* __restrict__ r, int n, float* __restrict__ trace,
-                                                 int cap, int* __restrict__ d_step) {
-  // one wave per recorder: its lanes sum the block partials in FP64
+__global__ __launch_bounds__(1024) void k_brecord(const BRecDesc* __restrict__ r, int n, float* __restrict__ trace,
+                                                  int cap, int* __restrict__ d_step) {
+  // one wave per recorder: its lanes sum the block partials in FP64, eight
+  // loads in flight per lane
   const int t = *d_step;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  for (int i = wave; i < n; i += 4) {
+  for (int i = wave; i < n; i += 16) {
+    const int nb = r[i].nblocks;
+    const float* __restrict__ part = r[i].part;
     double s = 0.0;
-    for (int b = lane; b < r[i].nblocks; b += 64) s += double(r[i].part[(long long)b * ARD_MAX_REC_PER_PART]);
+    for (int b0 = 0; b0 < nb; b0 += 512) {
+      float v[8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const int b = b0 + 64 * j + lane;
+        v[j] = b < nb ? part[(long long)b * ARD_MAX_REC_PER_PART] : 0.f;
+      }
+#pragma unroll
+      for (int j = 0; j < 8; ++j) s += double(v[j]);
+    }
 #pragma unroll
     for (int m = 1; m < 64; m <<= 1) s += __shfl_xor(s, m);
     if (lane == 0 && t < cap) trace[(long long)t * n + i] = float(s);
@@ -596,7 +608,7 @@ int launch_modal(int rpw, bool pts, const ModalDesc* d, const long long* starts,
 }
 
 int launch_brecord(const BRecDesc* d, int n, float* trace, int cap, int* d_step, hipStream_t s) {
-  hipLaunchKernelGGL(k_brecord, dim3(1), dim3(256), 0, s, d, n, trace, cap, d_step);
+  hipLaunchKernelGGL(k_brecord, dim3(1), dim3(1024), 0, s, d, n, trace, cap, d_step);
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 
