@@ -722,15 +722,15 @@ int enqueue_step(Ctx& c, int par, PhaseTimer* T) {
   };
   phase(ARD_PH_FACE_FWD1, [&] { return face(0); });
   phase(ARD_PH_FACE_FWD2, [&] { return face(1); });
-  // the PML stencil overlaps the face transforms (matrix cores) and is joined
-  // before the modal pass, which streams the whole mode state
-  if (fork) r |= hipStreamWaitEvent(st, B.ev_join, 0) != hipSuccess;
   for (auto& g : B.modal)
     phase(ARD_PH_MODAL, [&] { return launch_modal(g.rpw, g.pts, g.d_desc[par], g.d_start, g.n, g.blocks, par, st); });
   phase(ARD_PH_UNFOLD, [&] { return launch_unfold(B.d_unfold, B.d_unfold_start, B.nunfold, B.unfold_total, st); });
   phase(ARD_PH_FACE_INV1, [&] { return face(2); });
   phase(ARD_PH_FACE_INV2, [&] { return face(3); });
-  // the interface reads the PML pressure just written: buffer par ^ 1
+  // the PML stencil overlaps the fold and the face transforms and is joined
+  // here, at its first consumer (long finished by then, so the join costs the
+  // step no wait): the interface reads the PML pressure just written, buffer par ^ 1
+  if (fork) r |= hipStreamWaitEvent(st, B.ev_join, 0) != hipSuccess;
   phase(ARD_PH_INTERFACE, [&] { return launch_interface(B.d_bnd, B.nbnd, B.d_parts, par ^ 1, B.bnd_total, st); });
   phase(ARD_PH_RECORD, [&] {
     if (enqueue_cubes(c, par, st)) return 1;
