@@ -701,9 +701,11 @@ int enqueue_step(Ctx& c, int par, PhaseTimer* T) {
     if (T) T->end(st, ev);
   };
   phase(ARD_PH_SOURCES, [&] { return launch_bsources(B.d_src, B.nsrc, c.d_step, c.courant, st); });
-  // captured: PML on the side stream, joined before the modal pass (the
-  // interface reads the new PML pressure and writes PML forces); timed: all in
-  // order on one stream
+  // captured: PML on the side stream, joined before the interface (which
+  // reads the new PML pressure and writes PML forces); timed: all in order on
+  // one stream. Forking after the fold or after the first forward transform
+  // was slower (the fold is stretched less, but the PML then meets the modal
+  // pass or loses its overlap).
   const bool fork = !T && B.side;
   if (fork) {
     r |= hipEventRecord(B.ev_fork, st) != hipSuccess;
