@@ -177,20 +177,22 @@ GemmDesc gemm_n_contig(const float* A, int L, const float* B, float* C, int inne
 // zero-padded so that the kernel's 64-row x 16-column operand tiles never
 // leave them: columns to Kp = 16 ceil(ceil(L/2) / 16);
 // forward [fold_rows_fwd(L)][Kp]: rows m < ceil(L/2) = REDFT10 row 2m, zero
-// rows up to Mh (a multiple of 64), then the odd rows 2m+1, zero rows to a
+// rows up to Mh = fold_even_rows(L) (a multiple of 32: each wave's 32 rows of a
+// tile are all even or all odd), then the odd rows 2m+1, zero rows to a
 // multiple of 64; inverse [2][fold_rows_inv(L)][Kp]: REDFT01 columns 2k' (E)
 // and 2k'+1 (O) of rows j. For odd L the forward operand's middle element is
 // x[j] + x[L-1-j] = 2 x[j], so the even rows' middle column is halved.
 // FP64, stored FP32; cached per length in the context.
 int fold_kp(int L) { return int(ceil_div((L + 1) / 2, 16) * 16); }
-int fold_rows_fwd(int L) { return int(ceil_div((L + 1) / 2, 64) * 64 + ceil_div(L / 2, 64) * 64); }
+int fold_even_rows(int L) { return int(ceil_div((L + 1) / 2, 32) * 32); }
+int fold_rows_fwd(int L) { return int(ceil_div(fold_even_rows(L) + L / 2, 64) * 64); }
 int fold_rows_inv(int L) { return int(ceil_div((L + 1) / 2, 64) * 64); }
 
 int fold_tables(Ctx& c, int L, const float** fwd, const float** inv) {
   auto it = c.fold_tables.find(L);
   if (it == c.fold_tables.end()) {
     const int Kh = (L + 1) / 2, Me = (L + 1) / 2, Mo = L / 2, Kp = fold_kp(L);
-    const int Mh = int(ceil_div(Me, 64) * 64), Rf = fold_rows_fwd(L), Ri = fold_rows_inv(L);
+    const int Mh = fold_even_rows(L), Rf = fold_rows_fwd(L), Ri = fold_rows_inv(L);
     std::vector<float> f(size_t(Rf) * Kp, 0.f), v(size_t(2) * Ri * Kp, 0.f);
     auto c2 = [&](int k, int j) { return 2.0 * std::cos(kPi * k * (j + 0.5) / L); };
     auto c3 = [&](int j, int k) { return k == 0 ? 1.0 : 2.0 * std::cos(kPi * k * (j + 0.5) / L); };
@@ -612,7 +614,7 @@ int finalize_batched(Ctx& c) {
       g.Ap = k < 2 ? fold_rows_fwd(L) : fold_rows_inv(L);
       if (k < 2) {
         g.A = ff;
-        g.Mh = int(ceil_div((L + 1) / 2, 64) * 64);
+        g.Mh = fold_even_rows(L);
         g.M = g.Mh + L / 2;
       } else {
         g.A = fi;

@@ -403,8 +403,9 @@ __global__ __launch_bounds__(256) void k_gemm_bf16x3(const GemmDesc* __restrict_
 //                      x[j] = E + O,  x[L-1-j] = E - O
 // (the middle element of an odd L counts once). Half the multiply-adds of the
 // dense pass. Forward: rows [0, Mh) of the folded matrix are the even outputs
-// (Mh a multiple of 64, so a 64-row tile is all even or all odd), the sum or
-// difference is formed as B is staged. Inverse: A holds [A_E; A_O] (M rows
+// (Mh a multiple of 32, so each wave's 32 rows are all even or all odd); B is
+// staged twice, for the tile's first and second 32 rows, each as the sum (even
+// rows) or the difference (odd rows). Inverse: A holds [A_E; A_O] (M rows
 // each); every tile accumulates both and writes the pair.
 // ---------------------------------------------------------------------------
 template <bool B_K_CONTIG, int FOLD>
@@ -412,7 +413,7 @@ __global__ __launch_bounds__(256) void k_gemm_mfma_fold(const GemmDesc* __restri
                                                         const int2* __restrict__ tiles) {
   constexpr int NS = FOLD == 2 ? 2 : 1;  // operand sets (E, O)
   __shared__ float As[2][NS][16][68];
-  __shared__ float Bs[2][NS][16][68];
+  __shared__ float Bs[2][2][16][68];  // inverse: even, odd rows of X; forward: sums, differences
   // the launch's tile table: (descriptor, tile within it), longest K first,
   // so the long tiles start early and the short ones fill in at the end
   const int2 tm = tiles[blockIdx.x];
@@ -427,7 +428,11 @@ __global__ __launch_bounds__(256) void k_gemm_mfma_fold(const GemmDesc* __restri
   const int lane = tid & 63;
   const int wave = tid >> 6;
   const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
-  const bool odd_tile = FOLD == 1 && m0 >= g.Mh;  // forward: uniform per tile
+  // forward: B set h feeds the tile's rows m0 + 32 h .. +31 (waves wm = 32 h):
+  // x[j] + sg_h x[L-1-j], sg_h = -1 for odd rows (uniform per block)
+  const float sg0 = FOLD == 1 && m0 >= g.Mh ? -1.f : 1.f;
+  const float sg1 = FOLD == 1 && m0 + 32 >= g.Mh ? -1.f : 1.f;
+  const int bsel = wm >> 5;
 
   // Operands through buffer resources: A is zero-padded to whole 64 x 16
   // tiles, so its loads need no clamps; B loads past the operand array read
@@ -506,8 +511,9 @@ __global__ __launch_bounds__(256) void k_gemm_mfma_fold(const GemmDesc* __restri
     for (int r = 0; r < 4; ++r) {
       const int k = B_K_CONTIG ? ka + r : (tid >> 6) + 4 * r;
       const int n = B_K_CONTIG ? ma : (tid & 63);
-      if (FOLD == 1) {
-        Bs[buf][0][k][n] = odd_tile ? rb[0][r] - rb[1][r] : rb[0][r] + rb[1][r];
+      if (FOLD == 1) {  // x[j] +- x[L-1-j], rounded once
+        Bs[buf][0][k][n] = rb[0][r] + sg0 * rb[1][r];
+        Bs[buf][1][k][n] = rb[0][r] + sg1 * rb[1][r];
       } else {
         Bs[buf][0][k][n] = rb[0][r];
         Bs[buf][1][k][n] = rb[1][r];
@@ -525,7 +531,7 @@ __global__ __launch_bounds__(256) void k_gemm_mfma_fold(const GemmDesc* __restri
 #pragma unroll
       for (int q = 0; q < NS; ++q) {
         const float a = As[buf][q][kk + (lane >> 5)][wm + (lane & 31)];
-        const float b = Bs[buf][q][kk + (lane >> 5)][wn + (lane & 31)];
+        const float b = Bs[buf][FOLD == 1 ? bsel : q][kk + (lane >> 5)][wn + (lane & 31)];
         // K-contiguous B: the output is contiguous along m, so the
         // instruction's columns (lane % 32) take m: C^T = B^T A^T
         acc[q] = B_K_CONTIG ? __builtin_amdgcn_mfma_f32_32x32x2f32(b, a, acc[q], 0, 0, 0)
