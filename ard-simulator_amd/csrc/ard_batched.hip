@@ -174,19 +174,20 @@ GemmDesc gemm_n_contig(const float* A, int L, const float* B, float* C, int inne
 }
 
 // Even/odd folded cosine matrices of an axis of length L (k_gemm_mfma_fold),
-// zero-padded so that the kernel's 64-row x 16-column operand tiles never
-// leave them: columns to Kp = 16 ceil(ceil(L/2) / 16);
+// zero-padded so that the kernel's operand tiles (64 rows forward, 32 rows of
+// each matrix inverse, 16 columns) never leave them: columns to
+// Kp = 16 ceil(ceil(L/2) / 16);
 // forward [fold_rows_fwd(L)][Kp]: rows m < ceil(L/2) = REDFT10 row 2m, zero
 // rows up to Mh = fold_even_rows(L) (a multiple of 32: each wave's 32 rows of a
 // tile are all even or all odd), then the odd rows 2m+1, zero rows to a
 // multiple of 64; inverse [2][fold_rows_inv(L)][Kp]: REDFT01 columns 2k' (E)
-// and 2k'+1 (O) of rows j. For odd L the forward operand's middle element is
+// and 2k'+1 (O) of rows j, zero rows to a multiple of 32. For odd L the forward operand's middle element is
 // x[j] + x[L-1-j] = 2 x[j], so the even rows' middle column is halved.
 // FP64, stored FP32; cached per length in the context.
 int fold_kp(int L) { return int(ceil_div((L + 1) / 2, 16) * 16); }
 int fold_even_rows(int L) { return int(ceil_div((L + 1) / 2, 32) * 32); }
 int fold_rows_fwd(int L) { return int(ceil_div(fold_even_rows(L) + L / 2, 64) * 64); }
-int fold_rows_inv(int L) { return int(ceil_div((L + 1) / 2, 64) * 64); }
+int fold_rows_inv(int L) { return int(ceil_div((L + 1) / 2, 32) * 32); }
 
 int fold_tables(Ctx& c, int L, const float** fwd, const float** inv) {
   auto it = c.fold_tables.find(L);
@@ -616,9 +617,9 @@ int finalize_batched(Ctx& c) {
         g.A = ff;
         g.Mh = fold_even_rows(L);
         g.M = g.Mh + L / 2;
-      } else {
+      } else {  // 64 tile rows (32 of A_E, 32 of A_O) per 32 outputs j
         g.A = fi;
-        g.M = (L + 1) / 2;
+        g.M = int(64 * ceil_div((L + 1) / 2, 32));
       }
       g.tiles_n = int(ceil_div(g.N, 64));
     }

@@ -69,7 +69,9 @@ struct GemmDesc {
   int Mh;
   // Extents for buffer-resource loads (k_gemm_mfma_fold): A holds one
   // [Ap][Kp] matrix (forward) or two (inverse: E then O), rows padded to a
-  // multiple of 64 and columns to 16 with zeros; B is b_elems floats from B.
+  // multiple of 64 (forward) or 32 (inverse) and columns to 16 with zeros; B
+  // is b_elems floats from B. Inverse: M = 64 ceil(ceil(L/2) / 32), 64 tile
+  // rows (32 of E, 32 of O) per 32 outputs.
   int Ap, Kp;
   unsigned b_elems;
 };

@@ -405,14 +405,15 @@ __global__ __launch_bounds__(256) void k_gemm_bf16x3(const GemmDesc* __restrict_
 // dense pass. Forward: rows [0, Mh) of the folded matrix are the even outputs
 // (Mh a multiple of 32, so each wave's 32 rows are all even or all odd); B is
 // staged twice, for the tile's first and second 32 rows, each as the sum (even
-// rows) or the difference (odd rows). Inverse: A holds [A_E; A_O] (M rows
-// each); every tile accumulates both and writes the pair.
+// rows) or the difference (odd rows). Inverse: a tile covers 32 outputs j; its
+// first 32 A rows are A_E and the next 32 A_O rows of those j (waves wm = 0
+// accumulate E, wm = 32 O), so M counts 64 rows per 32 j; E and O meet
+// through LDS and the E waves write x[j], the O waves x[L-1-j].
 // ---------------------------------------------------------------------------
 template <bool B_K_CONTIG, int FOLD>
 __global__ __launch_bounds__(256) void k_gemm_mfma_fold(const GemmDesc* __restrict__ descs,
                                                         const int2* __restrict__ tiles) {
-  constexpr int NS = FOLD == 2 ? 2 : 1;  // operand sets (E, O)
-  __shared__ float As[2][NS][16][68];
+  __shared__ float As[2][16][68];
   __shared__ float Bs[2][2][16][68];  // inverse: even, odd rows of X; forward: sums, differences
   // the launch's tile table: (descriptor, tile within it), longest K first,
   // so the long tiles start early and the short ones fill in at the end
@@ -428,8 +429,9 @@ __global__ __launch_bounds__(256) void k_gemm_mfma_fold(const GemmDesc* __restri
   const int lane = tid & 63;
   const int wave = tid >> 6;
   const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
-  // forward: B set h feeds the tile's rows m0 + 32 h .. +31 (waves wm = 32 h):
-  // x[j] + sg_h x[L-1-j], sg_h = -1 for odd rows (uniform per block)
+  // B set h feeds the tile's rows m0 + 32 h .. +31 (waves wm = 32 h). Forward:
+  // x[j] + sg_h x[L-1-j], sg_h = -1 for odd rows (uniform per block); inverse:
+  // set 0 the even rows of X (for E), set 1 the odd ones (for O)
   const float sg0 = FOLD == 1 && m0 >= g.Mh ? -1.f : 1.f;
   const float sg1 = FOLD == 1 && m0 + 32 >= g.Mh ? -1.f : 1.f;
   const int bsel = wm >> 5;
@@ -438,17 +440,19 @@ __global__ __launch_bounds__(256) void k_gemm_mfma_fold(const GemmDesc* __restri
   // tiles, so its loads need no clamps; B loads past the operand array read
   // 0 (range-checked), and B elements past K or N meet zero A columns or
   // are never stored, so no selects either.
-  const brsrc rA = make_rsrc(g.A, unsigned(NS * g.Ap * Kp) * 4u);
+  const brsrc rA = make_rsrc(g.A, unsigned((FOLD == 2 ? 2 : 1) * g.Ap * Kp) * 4u);
   // B's 16-byte loads that hold needed elements end at most 3 floats past a
   // row, so past the array's last row too: B arrays are followed by at least
   // 4 floats of the same allocation (the face buffers are carved from one block)
   const brsrc rB = make_rsrc(g.B, (g.b_elems + 4u) * 4u);
   const int bK = int(g.bK);
-  // A: thread (m = tid / 4, k = 4 (tid % 4) .. +3), one 16-byte load per set
-  // and chunk (byte offsets at k0 = 0; 16-byte loads need only 4-byte alignment)
-  unsigned aoff[NS];
-#pragma unroll
-  for (int q = 0; q < NS; ++q) aoff[q] = unsigned((q * g.Ap + m0 + (tid >> 2)) * Kp + 4 * (tid & 3)) * 4u;
+  // A: thread (tile row am = tid / 4, k = 4 (tid % 4) .. +3), one 16-byte load
+  // per chunk (byte offsets at k0 = 0; 16-byte loads need only 4-byte
+  // alignment). Inverse: tile row am is row j0 + am % 32 of A_E (am < 32) or
+  // A_O, the tables [2][Ap][Kp]
+  const int am = tid >> 2;
+  const int arow = FOLD == 2 ? (am >> 5) * g.Ap + (m0 >> 1) + (am & 31) : m0 + am;
+  const unsigned aoff = unsigned(arow * Kp + 4 * (tid & 3)) * 4u;
   // B(k, n) = B[brow(k) * bK + col(n)]: brow(k) = k, L-1-k (forward), 2k + w (inverse)
   // K-contiguous B: thread (n = tid / 4, k = 4 (tid % 4) .. +3), 16-byte loads
   int bcol[4];
@@ -460,18 +464,14 @@ __global__ __launch_bounds__(256) void k_gemm_mfma_fold(const GemmDesc* __restri
     bcol[0] = int((n % N1) * g.b1 + (n / N1) * g.b2);
   }
   const int kt = B_K_CONTIG ? (tid & 15) : (tid >> 6);  // this thread's k within a chunk (+ 4 r if !B_K_CONTIG)
-  f32x16 acc[NS];
+  f32x16 acc;
 #pragma unroll
-  for (int q = 0; q < NS; ++q)
-#pragma unroll
-    for (int r = 0; r < 16; ++r) acc[q][r] = 0.f;
+  for (int r = 0; r < 16; ++r) acc[r] = 0.f;
 
-  v4f ra[NS];
+  v4f ra;
   float rb[2][4];  // K-contiguous B: [w][i] for k = 4 (tid % 4) + i; else [w][r] for k = tid / 64 + 4 r
   auto gload = [&](const int k0) {
-#pragma unroll
-    for (int q = 0; q < NS; ++q)
-      ra[q] = __builtin_bit_cast(v4f, __builtin_amdgcn_raw_buffer_load_b128(rA, int(aoff[q]), int(k0 * 4), 0));
+    ra = __builtin_bit_cast(v4f, __builtin_amdgcn_raw_buffer_load_b128(rA, int(aoff), int(k0 * 4), 0));
     if (B_K_CONTIG) {
       const int k = k0 + 4 * (tid & 3);
       if (FOLD == 1) {
@@ -500,13 +500,10 @@ __global__ __launch_bounds__(256) void k_gemm_mfma_fold(const GemmDesc* __restri
   };
   auto sstore = [&](const int buf) {
     const int ka = 4 * (tid & 3), ma = tid >> 2;
-#pragma unroll
-    for (int q = 0; q < NS; ++q) {
-      As[buf][q][ka + 0][ma] = ra[q].x;
-      As[buf][q][ka + 1][ma] = ra[q].y;
-      As[buf][q][ka + 2][ma] = ra[q].z;
-      As[buf][q][ka + 3][ma] = ra[q].w;
-    }
+    As[buf][ka + 0][ma] = ra.x;
+    As[buf][ka + 1][ma] = ra.y;
+    As[buf][ka + 2][ma] = ra.z;
+    As[buf][ka + 3][ma] = ra.w;
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       const int k = B_K_CONTIG ? ka + r : (tid >> 6) + 4 * r;
@@ -528,23 +525,35 @@ __global__ __launch_bounds__(256) void k_gemm_mfma_fold(const GemmDesc* __restri
     if (k0 + 16 < K) gload(k0 + 16);
 #pragma unroll
     for (int kk = 0; kk < 16; kk += 2) {  // past K: zero-padded A columns
-#pragma unroll
-      for (int q = 0; q < NS; ++q) {
-        const float a = As[buf][q][kk + (lane >> 5)][wm + (lane & 31)];
-        const float b = Bs[buf][FOLD == 1 ? bsel : q][kk + (lane >> 5)][wn + (lane & 31)];
-        // K-contiguous B: the output is contiguous along m, so the
-        // instruction's columns (lane % 32) take m: C^T = B^T A^T
-        acc[q] = B_K_CONTIG ? __builtin_amdgcn_mfma_f32_32x32x2f32(b, a, acc[q], 0, 0, 0)
-                            : __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, acc[q], 0, 0, 0);
-      }
+      const float a = As[buf][kk + (lane >> 5)][wm + (lane & 31)];
+      const float b = Bs[buf][bsel][kk + (lane >> 5)][wn + (lane & 31)];
+      // K-contiguous B: the output is contiguous along m, so the
+      // instruction's columns (lane % 32) take m: C^T = B^T A^T
+      acc = B_K_CONTIG ? __builtin_amdgcn_mfma_f32_32x32x2f32(b, a, acc, 0, 0, 0)
+                       : __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, acc, 0, 0, 0);
     }
   }
+  // inverse: the E and O waves of the same outputs exchange accumulators
+  // ([set][wn / 32][r][lane] in the B tiles, free once every wave is done)
+  f32x16 oth;
+  if (FOLD == 2) {
+    __syncthreads();
+    float* __restrict__ xs = &Bs[0][0][0][0];  // 4096 of its 4352 floats
+#pragma unroll
+    for (int r = 0; r < 16; ++r) xs[((bsel * 2 + (wn >> 5)) * 16 + r) * 64 + lane] = acc[r];
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < 16; ++r) oth[r] = xs[(((bsel ^ 1) * 2 + (wn >> 5)) * 16 + r) * 64 + lane];
+  }
   gfloat* __restrict__ C = ARD_G(g.C);
-  // accumulator r of this lane: (m, n) = (wm + lane % 32, wn + row(r)) when
-  // B is K-contiguous, else (wm + row(r), wn + lane % 32), with
-  // row(r) = 8 (r / 4) + 4 (lane / 32) + r % 4
-  const int jl = (B_K_CONTIG ? m0 + wm : n0 + wn) + (lane & 31);
-  if (jl >= (B_K_CONTIG ? M : N)) return;
+  // accumulator r of this lane: (m, n) = (mb + lane % 32, wn + row(r)) when
+  // B is K-contiguous, else (mb + row(r), wn + lane % 32), with
+  // row(r) = 8 (r / 4) + 4 (lane / 32) + r % 4 and the wave's first output
+  // row mb = m0 + wm (forward) or j0 = m0 / 2 (inverse, both waves)
+  const int mb = FOLD == 2 ? (m0 >> 1) : m0 + wm;
+  const int mlim = FOLD == 2 ? (L + 1) / 2 : M;
+  const int jl = (B_K_CONTIG ? mb : n0 + wn) + (lane & 31);
+  if (jl >= (B_K_CONTIG ? mlim : N)) return;
   long long cn = 0;
   if (!B_K_CONTIG) cn = (long long)(jl % N1) * g.c1 + (long long)(jl / N1) * g.c2;
   // K-contiguous B: the lane's 16 columns are nb + 8 (r / 4) + r % 4; one
@@ -560,8 +569,8 @@ __global__ __launch_bounds__(256) void k_gemm_mfma_fold(const GemmDesc* __restri
   }
 #pragma unroll
   for (int r = 0; r < 16; ++r) {
-    const int il = (B_K_CONTIG ? n0 + wn : m0 + wm) + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-    if (il >= (B_K_CONTIG ? N : M)) continue;
+    const int il = (B_K_CONTIG ? n0 + wn : mb) + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+    if (il >= (B_K_CONTIG ? N : mlim)) continue;
     const int i = B_K_CONTIG ? jl : il;
     if (B_K_CONTIG) {
       int q = nq, rr = nr + 8 * (r >> 2) + (r & 3);  // rr < N1 + 28
@@ -576,11 +585,11 @@ __global__ __launch_bounds__(256) void k_gemm_mfma_fold(const GemmDesc* __restri
     }
     if (FOLD == 1) {
       const int k = i < g.Mh ? 2 * i : 2 * (i - g.Mh) + 1;
-      if (k < L) C[(long long)k * g.cM + cn] = g.alpha * acc[0][r];
+      if (k < L) C[(long long)k * g.cM + cn] = g.alpha * acc[r];
     } else {
-      const float e = acc[0][r], o = acc[NS - 1][r];
-      C[(long long)i * g.cM + cn] = g.alpha * (e + o);
-      if (L - 1 - i != i) C[(long long)(L - 1 - i) * g.cM + cn] = g.alpha * (e - o);
+      const float e = bsel == 0 ? acc[r] : oth[r], o = bsel == 0 ? oth[r] : acc[r];
+      if (bsel == 0) C[(long long)i * g.cM + cn] = g.alpha * (e + o);
+      else if (L - 1 - i != i) C[(long long)(L - 1 - i) * g.cM + cn] = g.alpha * (e - o);
     }
   }
 }
