@@ -11,6 +11,8 @@
 //   k_record       recorder point samples (ARD/recorder.cpp:40-57) into a device trace.
 // All kernels accept a table of problems so that one launch can also cover
 // every partition of a phase (the batched form).
+#include <type_traits>
+
 #include "ard_kernels.h"
 
 namespace ard {
@@ -517,14 +519,9 @@ __global__ __launch_bounds__(256) void k_gemm_mfma_fold(const GemmDesc* __restri
       }
     }
   };
-  gload(0);
-  int buf = 0;
-  for (int k0 = 0; k0 < K; k0 += 16, buf ^= 1) {
-    sstore(buf);
-    __syncthreads();
-    if (k0 + 16 < K) gload(k0 + 16);
+  auto mma = [&](const int buf, auto steps) {
 #pragma unroll
-    for (int kk = 0; kk < 16; kk += 2) {  // past K: zero-padded A columns
+    for (int kk = 0; kk < 2 * decltype(steps)::value; kk += 2) {  // past K: zero-padded A columns
       const float a = As[buf][kk + (lane >> 5)][wm + (lane & 31)];
       const float b = Bs[buf][bsel][kk + (lane >> 5)][wn + (lane & 31)];
       // K-contiguous B: the output is contiguous along m, so the
@@ -532,6 +529,19 @@ __global__ __launch_bounds__(256) void k_gemm_mfma_fold(const GemmDesc* __restri
       acc = B_K_CONTIG ? __builtin_amdgcn_mfma_f32_32x32x2f32(b, a, acc, 0, 0, 0)
                        : __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, acc, 0, 0, 0);
     }
+  };
+  gload(0);
+  int buf = 0, k0 = 0;
+  for (; k0 + 8 < K; k0 += 16, buf ^= 1) {  // chunks with more than 8 columns of K left
+    sstore(buf);
+    __syncthreads();
+    if (k0 + 16 < K) gload(k0 + 16);
+    mma(buf, std::integral_constant<int, 8>());
+  }
+  if (k0 < K) {  // a last chunk of at most 8 columns: its second half meets zero A columns only
+    sstore(buf);
+    __syncthreads();
+    mma(buf, std::integral_constant<int, 4>());
   }
   // inverse: the E and O waves of the same outputs exchange accumulators
   // ([set][wn / 32][r][lane] in the B tiles, free once every wave is done)
