@@ -710,16 +710,16 @@ int enqueue_step(Ctx& c, int par, PhaseTimer* T) {
   // was slower (the fold is stretched less, but the PML then meets the modal
   // pass or loses its overlap).
   const bool fork = !T && B.side;
-  if (fork) {
-    r |= hipEventRecord(B.ev_fork, st) != hipSuccess;
-    r |= hipStreamWaitEvent(B.side, B.ev_fork, 0) != hipSuccess;
-  }
+  if (fork) r |= hipEventRecord(B.ev_fork, st) != hipSuccess;
+  // the fold is enqueued first, so that its blocks are dispatched before the
+  // PML slabs' fill the CUs; the PML still depends on the sources only
+  phase(ARD_PH_FOLD, [&] { return launch_fold(B.d_fold, B.d_fold_start, B.nfold, B.fold_total, st); });
+  if (fork) r |= hipStreamWaitEvent(B.side, B.ev_fork, 0) != hipSuccess;
   hipStream_t sp = fork ? B.side : st;
   phase(ARD_PH_PML, [&] {
     return launch_pml(B.d_pml, B.npml, B.pml_blocks, par, c.prm.pml_layers, c.prm.h, c.prm.dt, c.prm.c0, sp);
   });
   if (fork) r |= hipEventRecord(B.ev_join, B.side) != hipSuccess;
-  phase(ARD_PH_FOLD, [&] { return launch_fold(B.d_fold, B.d_fold_start, B.nfold, B.fold_total, st); });
   auto face = [&](int k) {
     if (B.gemm_variant == 3)
       return launch_gemm_fold(B.d_gemm_fold[k], B.d_fold_tiles[k], B.gemm_fold_tiles[k], k % 2 == 0, k < 2 ? 1 : 2, st);
