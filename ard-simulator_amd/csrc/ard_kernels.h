@@ -14,10 +14,10 @@ int launch_gemm(const GemmDesc* d_descs, int ndesc, int total_tiles, bool b_k_co
 // d_tiles[b] = (descriptor, tile within it) of block b (fold_tile_table).
 int launch_gemm_fold(const GemmDesc* d_descs, const int2* d_tiles, int total_tiles, bool b_k_contig, int fold,
                      hipStream_t s);
-// Blocks of one slab's update for PML thickness L (the slab's extent along
-// its normal axis); sets d.nb. Slabs 3 to 8 cells thick take the LDS-tiled
-// update, thicker ones one thread per cell. Every slab of one launch must
-// share L; -1 if the slab is not L cells thick.
+// Blocks of one slab's update for PML thickness L (pml_layers); sets d.nb.
+// Slabs L cells thick along their normal, for L from 3 to 8, take the
+// LDS-tiled update; others (thicker layers, or slabs of another thickness
+// added through ard_add_pml_partition) one thread per cell, in the same launch.
 long long pml_slab_blocks(PmlDesc& d, int L);
 int launch_pml(const PmlDesc* d_descs, int ndesc, long long total_blocks, int par, int L, float dh, float dt,
                float c0, hipStream_t s);

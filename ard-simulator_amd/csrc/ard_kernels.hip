@@ -5,7 +5,9 @@
 //                  cosine matrix of the axis, LDS-tiled, 64x64 tiles per 256 threads;
 //                  the last forward pass fuses the modal update
 //                  (ARD/dct_volume.cpp:38-68, ARD/dct_partition.cpp:48-73).
-//   k_pml          Grote-Sim FDTD update of PML slabs (ARD/pml_partition.cpp:113-200).
+//   k_pml_tiled    Grote-Sim FDTD update of PML slabs (ARD/pml_partition.cpp:113-200),
+//                  LDS-tiled through the slab's thickness (3 to 8 cells);
+//   k_pml          the same update one thread per cell, for thicker slabs.
 //   k_interface    6th-order interface forcing (ARD/boundary.cpp:21-173).
 //   k_sources      source forcing (ARD/partition.cpp:187-197, ARD/gaussian_source.cpp:16-20).
 //   k_record       recorder point samples (ARD/recorder.cpp:40-57) into a device trace.
@@ -660,20 +662,11 @@ int launch_gemm(const GemmDesc* d_descs, int ndesc, int total_tiles, bool b_k_co
 // Out-of-slab neighbours read 0 (the reference's sentinel element, :6-15).
 // ---------------------------------------------------------------------------
 // r180 = 1 / (180 h^2), r12 = 1 / (12 h), rdt = 1 / dt, formed on the host in FP32
-__global__ __launch_bounds__(256) void k_pml(const PmlDesc* __restrict__ descs, int ndesc, int par, float r180,
-                                             float r12, float dt, float rdt, float c0) {
-  // contiguous block ranges per XCD: the stencil's y and z neighbours, read
-  // by nearby blocks, stay in one L2
-  const long long b = xcd_block(int(blockIdx.x), int(gridDim.x));
-  int lo = 0, hi = ndesc - 1;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (descs[mid].block0 <= b) lo = mid;
-    else hi = mid - 1;
-  }
-  const PmlDesc& P = descs[lo];
+// One block of 256 cells (block t of slab P).
+__device__ __forceinline__ void pml_cells(const PmlDesc& P, int t, int par, float r180, float r12, float dt,
+                                          float rdt, float c0) {
   const int W = P.W, H = P.H, D = P.D;
-  const unsigned cell = unsigned(b - P.block0) * 256u + threadIdx.x;  // slabs hold < 2^29 cells (checked at setup)
+  const unsigned cell = unsigned(t) * 256u + threadIdx.x;  // slabs hold < 2^29 cells (checked at setup)
   if (cell >= unsigned(W * H * D)) return;
   const unsigned row = cell / unsigned(W);
   const int i = int(cell - row * unsigned(W));
@@ -755,6 +748,20 @@ __global__ __launch_bounds__(256) void k_pml(const PmlDesc* __restrict__ descs, 
   ARD_G(P.phi[par ^ 1][1])[cell] = phy - dt * zy * phy + dt * (zx + zz - zy) * duy;
   ARD_G(P.phi[par ^ 1][2])[cell] = phz - dt * zz * phz + dt * (zx + zy - zz) * duz;
   if (fc != 0.f) f[cell] = 0.f;  // force cleared after the update (:199)
+}
+
+__global__ __launch_bounds__(256) void k_pml(const PmlDesc* __restrict__ descs, int ndesc, int par, float r180,
+                                             float r12, float dt, float rdt, float c0) {
+  // contiguous block ranges per XCD: the stencil's y and z neighbours, read
+  // by nearby blocks, stay in one L2
+  const long long b = xcd_block(int(blockIdx.x), int(gridDim.x));
+  int lo = 0, hi = ndesc - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (descs[mid].block0 <= b) lo = mid;
+    else hi = mid - 1;
+  }
+  pml_cells(descs[lo], int(b - descs[lo].block0), par, r180, r12, dt, rdt, c0);
 }
 
 // ---------------------------------------------------------------------------
@@ -969,7 +976,8 @@ __global__ __launch_bounds__(256) void k_pml_tiled(const PmlDesc* __restrict__ d
   }
   const PmlDesc& P = descs[lo];
   const int t = int(b - P.block0);
-  if (P.axis == 0) pml_tile<L, 0>(P, t, sm, par, r180, r12, dt, rdt, c0);
+  if (P.nb == 0) pml_cells(P, t, par, r180, r12, dt, rdt, c0);  // a slab not L cells thick
+  else if (P.axis == 0) pml_tile<L, 0>(P, t, sm, par, r180, r12, dt, rdt, c0);
   else if (P.axis == 1) pml_tile<L, 1>(P, t, sm, par, r180, r12, dt, rdt, c0);
   else pml_tile<L, 2>(P, t, sm, par, r180, r12, dt, rdt, c0);
 }
@@ -977,11 +985,10 @@ __global__ __launch_bounds__(256) void k_pml_tiled(const PmlDesc* __restrict__ d
 long long pml_slab_blocks(PmlDesc& d, int L) {
   const long long n = (long long)d.W * d.H * d.D;
   const int ext[3] = {d.W, d.H, d.D};
-  if (L < 3 || L > 8) {
+  if (L < 3 || L > 8 || ext[d.axis] != L) {  // one thread per cell
     d.nb = 0;
     return (n + 255) / 256;
   }
-  if (ext[d.axis] != L) return -1;  // the tiled update needs the slab L cells thick
   // tile extents across the normal: first and second axis other than it
   const int a0 = d.axis == 0 ? 1 : 0, a1 = d.axis == 2 ? 1 : 2;
   const int t0 = d.axis == 0 ? 16 : 32, t1 = d.axis == 0 ? 16 : 8;

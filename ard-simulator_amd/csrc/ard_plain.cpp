@@ -241,7 +241,6 @@ int setup_common(Ctx& c) {
     d.zprof = B.zprof;
     d.block0 = 0;
     const long long nblk = pml_slab_blocks(d, c.prm.pml_layers);
-    if (nblk < 0) return set_error(ARD_ERR_INVALID, "a PML slab thinner or thicker than pml_layers");
     B.desc = int(c.h_pml.size());
     c.h_pml.push_back(d);
     c.pml_blocks.push_back(nblk);
