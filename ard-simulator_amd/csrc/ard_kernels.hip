@@ -1036,8 +1036,8 @@ __device__ __forceinline__ const float* shell_addr(const PartAccess& A, float* c
   const bool sx = x < 3 || x >= W - 3;
   const bool sy = y < 3 || y >= H - 3;
   const int qx = x < 3 ? x : x - W + 6, qy = y < 3 ? y : y - H + 6, qz = z < 3 ? z : z - D + 6;
-  // shells hold 6 D H, 6 D W, 6 H W < 2^31 cells: batched partitions hold fewer than
-  // 2^30 cells (checked at setup) and are at least 3 cells thick along every axis
+  // shells hold 6 D H, 6 D W, 6 H W < 2^30 cells: batched partitions hold fewer than
+  // 2^30 cells and DCT partitions are at least 6 cells along every axis (both checked)
   const unsigned ox = (unsigned(qx * D + z) * unsigned(H) + unsigned(y));
   const unsigned oy = (unsigned(qy * D + z) * unsigned(W) + unsigned(x));
   const unsigned oz = (unsigned(qz * H + y) * unsigned(W) + unsigned(x));
