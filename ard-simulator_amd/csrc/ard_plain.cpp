@@ -221,7 +221,7 @@ int setup_common(Ctx& c) {
     if (!p.is_pml) continue;
     PmlBuffers& B = c.pml[i];
     const size_t n = size_t(p.cells());
-    if (n >= (size_t(1) << 29))  // k_pml: 32-bit byte offsets within a slab
+    if (n >= (size_t(1) << 29))  // the PML kernels: 32-bit byte offsets within a slab
       return set_error(ARD_ERR_UNSUPPORTED, "a PML slab of 2^29 or more cells");
     float* blk = nullptr;
     if ((r = c.dalloc((void**)&blk, n * 9 * 4))) return r;
