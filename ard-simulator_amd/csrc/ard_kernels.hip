@@ -1017,46 +1017,65 @@ int launch_pml(const PmlDesc* d_descs, int ndesc, long long total_blocks, int pa
 // Interface forcing (ARD/boundary.cpp:21-173). One thread per face element
 // computes the 6 outputs m = -3..2 from the 3 cells on each side.
 // ---------------------------------------------------------------------------
-// Address of cell (x, y, z) of a partition's pressure (current step) or
-// force, with selects only (no branch on the lane's cell), so that a thread's
-// six reads issue together. *in = false outside a dense field: the address is
-// then a clamped cell and the caller reads 0 (the zero sentinel).
-// Face-shell storage of the batched form: a cell belongs to the x-shell
-// [6][D][H] if it is within 3 of an x face, else to the y-shell [6][D][W],
-// else to the z-shell [6][H][W]; plane q < 3 is cell q from the low face,
-// q >= 3 cell q - 6 from the high one.
-__device__ __forceinline__ const float* shell_addr(const PartAccess& A, float* const* sh, int x, int y, int z) {
+// The three cells of one side of an interface element: normal coordinates
+// c0, c0 + 1, c0 + 2 along axis ax (0 x, 1 y, 2 z) at tangential coordinates
+// (a, b), the other two axes in x, y, z order. Face-shell storage of the
+// batched form: a cell belongs to the x-shell [6][D][H] if it is within 3 of
+// an x face, else to the y-shell [6][D][W], else to the z-shell [6][H][W];
+// plane q < 3 is cell q from the low face, q >= 3 cell q - 6 from the high
+// one. The three cells lie within 3 of the faces normal to ax, so the
+// tangential coordinates alone pick their shell: one shell w, the first
+// cell's index and a stride, formed with selects (no branch on the lane's
+// cell) so that a thread's reads issue together. A dense field: in = false
+// when (a, b) lies outside it (the index is then a clamped cell's and the
+// caller reads 0, the zero sentinel).
+struct SideRun {
+  long long off, stride;
+  int w;
+  bool in;
+};
+
+__device__ __forceinline__ SideRun side_run(const PartAccess& A, int ax, int c0, int a, int b) {
   const int W = A.W, H = A.H, D = A.D;
-  // the three shell bases are block-uniform: held in scalar registers, so the
-  // compiler selects between addresses instead of loading a lane's base
+  const int x = ax == 0 ? c0 : a, y = ax == 1 ? c0 : (ax == 0 ? a : b), z = ax == 2 ? c0 : b;
+  SideRun s;
+  if (A.kind == 2) {  // uniform across the block
+    const bool sx = x < 3 || x >= W - 3;
+    const bool sy = y < 3 || y >= H - 3;
+    const int qx = x < 3 ? x : x - W + 6, qy = y < 3 ? y : y - H + 6, qz = z < 3 ? z : z - D + 6;
+    // shells hold 6 D H, 6 D W, 6 H W < 2^30 cells: batched partitions hold fewer than
+    // 2^30 cells and DCT partitions are at least 6 cells along every axis (both checked)
+    const unsigned ox = unsigned(qx * D + z) * unsigned(H) + unsigned(y);
+    const unsigned oy = unsigned(qy * D + z) * unsigned(W) + unsigned(x);
+    const unsigned oz = unsigned(qz * H + y) * unsigned(W) + unsigned(x);
+    // a step along the normal inside each shell (the z-shell only for ax = 2,
+    // the y-shell only for ax = 1 or 2)
+    const int tx = ax == 0 ? D * H : (ax == 1 ? 1 : H);
+    const int ty = ax == 1 ? D * W : W;
+    s.w = sx ? 0 : (sy ? 1 : 2);
+    s.off = sx ? ox : (sy ? oy : oz);
+    s.stride = sx ? tx : (sy ? ty : H * W);
+    s.in = true;
+    return s;
+  }
+  s.in = !(x < 0 || x >= W || y < 0 || y >= H || z < 0 || z >= D);
+  const int xc = min(max(x, 0), W - 1), yc = min(max(y, 0), H - 1), zc = min(max(z, 0), D - 1);
+  s.off = ((long long)zc * H + yc) * W + xc;
+  s.stride = ax == 0 ? 1 : (ax == 1 ? W : (long long)W * H);
+  s.w = 0;
+  return s;
+}
+
+// Base of a side's cells: shell w of sh (batched form) or the dense field.
+// The three shell bases are block-uniform: held in scalar registers, so the
+// compiler selects between them instead of loading a lane's base.
+__device__ __forceinline__ float* run_base(const PartAccess& A, float* const* sh, float* dense, int w) {
+  if (A.kind != 2) return dense;
   float* b0 = sh[0];
   float* b1 = sh[1];
   float* b2 = sh[2];
   asm volatile("" : "+s"(b0), "+s"(b1), "+s"(b2));
-  const bool sx = x < 3 || x >= W - 3;
-  const bool sy = y < 3 || y >= H - 3;
-  const int qx = x < 3 ? x : x - W + 6, qy = y < 3 ? y : y - H + 6, qz = z < 3 ? z : z - D + 6;
-  // shells hold 6 D H, 6 D W, 6 H W < 2^30 cells: batched partitions hold fewer than
-  // 2^30 cells and DCT partitions are at least 6 cells along every axis (both checked)
-  const unsigned ox = (unsigned(qx * D + z) * unsigned(H) + unsigned(y));
-  const unsigned oy = (unsigned(qy * D + z) * unsigned(W) + unsigned(x));
-  const unsigned oz = (unsigned(qz * H + y) * unsigned(W) + unsigned(x));
-  return sx ? b0 + ox : (sy ? b1 + oy : b2 + oz);
-}
-
-__device__ __forceinline__ const float* p_addr(const PartAccess& A, int par, int x, int y, int z, bool* in) {
-  if (A.kind == 2) {  // uniform across the block
-    *in = true;
-    return shell_addr(A, A.ps, x, y, z);
-  }
-  *in = !(x < 0 || x >= A.W || y < 0 || y >= A.H || z < 0 || z >= A.D);
-  const int xc = min(max(x, 0), A.W - 1), yc = min(max(y, 0), A.H - 1), zc = min(max(z, 0), A.D - 1);
-  return A.p[par] + (((long long)zc * A.H + yc) * A.W + xc);
-}
-
-__device__ __forceinline__ float* f_addr(const PartAccess& A, int x, int y, int z) {
-  if (A.kind == 2) return const_cast<float*>(shell_addr(A, A.fs, x, y, z));
-  return A.f + (((long long)z * A.H + y) * A.W + x);
+  return w == 0 ? b0 : (w == 1 ? b1 : b2);
 }
 
 __global__ __launch_bounds__(256) void k_interface(const BndDesc* __restrict__ descs, int ndesc,
@@ -1085,29 +1104,21 @@ __global__ __launch_bounds__(256) void k_interface(const BndDesc* __restrict__ d
   const int u1 = u - B.org[1][0], v1 = v - B.org[1][1];
   const int n0 = B.type == 0 ? S0.W : (B.type == 1 ? S0.H : S0.D);
   // the element's cells: side 0 at normal n0-3..n0-1, side 1 at 0..2
-  const float* ad[6];
-  bool in[6];
-#pragma unroll
-  for (int n = 0; n < 3; ++n) {
-    if (B.type == 0) {
-      ad[n] = p_addr(S0, par, n0 - 3 + n, u0, v0, &in[n]);
-      ad[3 + n] = p_addr(S1, par, n, u1, v1, &in[3 + n]);
-    } else if (B.type == 1) {
-      ad[n] = p_addr(S0, par, u0, n0 - 3 + n, v0, &in[n]);
-      ad[3 + n] = p_addr(S1, par, u1, n, v1, &in[3 + n]);
-    } else {
-      ad[n] = p_addr(S0, par, u0, v0, n0 - 3 + n, &in[n]);
-      ad[3 + n] = p_addr(S1, par, u1, v1, n, &in[3 + n]);
-    }
-  }
+  const SideRun r0 = side_run(S0, B.type, n0 - 3, u0, v0);
+  const SideRun r1 = side_run(S1, B.type, 0, u1, v1);
+  const float* p0 = run_base(S0, S0.ps, S0.p[par], r0.w);
+  const float* p1 = run_base(S1, S1.ps, S1.p[par], r1.w);
   float pv[6];
 #pragma unroll
-  for (int q = 0; q < 6; ++q) pv[q] = *ARD_CG(ad[q]);
+  for (int n = 0; n < 3; ++n) {
+    pv[n] = ARD_CG(p0)[r0.off + n * r0.stride];
+    pv[3 + n] = ARD_CG(p1)[r1.off + n * r1.stride];
+  }
   float pl[3], pr[3];
 #pragma unroll
   for (int n = 0; n < 3; ++n) {
-    pl[n] = in[n] ? pv[n] : 0.f;
-    pr[n] = in[3 + n] ? pv[3 + n] : 0.f;
+    pl[n] = r0.in ? pv[n] : 0.f;
+    pr[n] = r1.in ? pv[3 + n] : 0.f;
   }
   const float C[6][6] = {{0, 0, -2, 2, 0, 0},       {0, -2, 27, -27, 2, 0},  {-2, 27, -270, 270, -27, 2},
                          {2, -27, 270, -270, 27, -2}, {0, 2, -27, 27, -2, 0}, {0, 0, 2, -2, 0, 0}};
@@ -1121,17 +1132,12 @@ __global__ __launch_bounds__(256) void k_interface(const BndDesc* __restrict__ d
     for (int n = 0; n < 3; ++n) s1 += C[r][n] * pl[n];
 #pragma unroll
     for (int n = 0; n < 3; ++n) s2 += C[r][n + 3] * pr[n];
-    if (m < 0) {
+    if (m < 0) {  // side 0, cell n0 + m = the run's cell r
       const float sip = B.self[0] ? (s1 + s2) : s2;
-      const float val = sip * B.k[0];
-      float* fa = B.type == 0 ? f_addr(S0, n0 + m, u0, v0)
-                              : (B.type == 1 ? f_addr(S0, u0, n0 + m, v0) : f_addr(S0, u0, v0, n0 + m));
-      *ARD_G(fa) = val;
-    } else {
+      ARD_G(run_base(S0, S0.fs, S0.f, r0.w))[r0.off + r * r0.stride] = sip * B.k[0];
+    } else {  // side 1, cell m
       const float sip = B.self[1] ? (s1 + s2) : s1;
-      const float val = sip * B.k[1];
-      float* fa = B.type == 0 ? f_addr(S1, m, u1, v1) : (B.type == 1 ? f_addr(S1, u1, m, v1) : f_addr(S1, u1, v1, m));
-      *ARD_G(fa) = val;
+      ARD_G(run_base(S1, S1.fs, S1.f, r1.w))[r1.off + m * r1.stride] = sip * B.k[1];
     }
   }
 }
