@@ -417,8 +417,10 @@ __global__ __launch_bounds__(256) void k_gemm_bf16x3(const GemmDesc* __restrict_
 template <bool B_K_CONTIG, int FOLD>
 __global__ __launch_bounds__(256) void k_gemm_mfma_fold(const GemmDesc* __restrict__ descs,
                                                         const int2* __restrict__ tiles) {
-  __shared__ float As[2][16][68];
-  __shared__ float Bs[2][2][16][68];  // inverse: even, odd rows of X; forward: sums, differences
+  // rows of 66: the staging stores (four k rows per quad of lanes) fall in
+  // distinct banks, as do the operand reads (one row per half-wave)
+  __shared__ float As[2][16][66];
+  __shared__ float Bs[2][2][16][66];  // inverse: even, odd rows of X; forward: sums, differences
   // the launch's tile table: (descriptor, tile within it), longest K first,
   // so the long tiles start early and the short ones fill in at the end
   const int2 tm = tiles[blockIdx.x];
@@ -550,7 +552,7 @@ __global__ __launch_bounds__(256) void k_gemm_mfma_fold(const GemmDesc* __restri
   f32x16 oth;
   if (FOLD == 2) {
     __syncthreads();
-    float* __restrict__ xs = &Bs[0][0][0][0];  // 4096 of its 4352 floats
+    float* __restrict__ xs = &Bs[0][0][0][0];  // 4096 of its 4224 floats
 #pragma unroll
     for (int r = 0; r < 16; ++r) xs[((bsel * 2 + (wn >> 5)) * 16 + r) * 64 + lane] = acc[r];
     __syncthreads();
