@@ -196,7 +196,7 @@ __global__ __launch_bounds__(256) void k_modal(const ModalDesc* __restrict__ des
   static_assert(RPW % 2 == 0, "rows per wave must be even");
   constexpr int TY = 4 * RPW;  // rows of the block tile
   constexpr int KB = 4;        // kz planes per batch of the x and y reductions
-  constexpr int LD = 68;       // padded row of the mode tile
+  constexpr int LD = 80;       // padded row of the mode tile: two rows 16 banks apart (x sums read 2 rows per 8 lanes)
   constexpr int NPT = ARD_MAX_SRC_PER_PART;
   constexpr int NRC = ARD_MAX_REC_PER_PART;
   static_assert(NPT == 4 && NRC == 4, "point tables hold 4 + 4 factors");
