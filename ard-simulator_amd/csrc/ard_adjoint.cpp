@@ -287,13 +287,16 @@ extern "C" int ard_adjoint(ard_ctx* ctx, int nsteps, const float* dl_drec, float
   long long pml_seg = 0;  // floats of saved PML state per step
   for (const AdjPml& A : S.pml) pml_seg += 5 * A.n;
   // checkpoints (nck + the final state) and one segment of PML states
+  // (all per-call buffers are freed by cleanup(), whatever the exit path)
   float *ck = nullptr, *seg = nullptr, *d_g = nullptr, *d_du = nullptr;
+  AdjPmlDesc* d_pd = nullptr;
   auto cleanup = [&]() {
     hipStreamSynchronize(c->stream);
     if (ck) hipFree(ck);
     if (seg) hipFree(seg);
     if (d_g) hipFree(d_g);
     if (d_du) hipFree(d_du);
+    if (d_pd) hipFree(d_pd);
   };
   hipError_t e = hipMalloc((void**)&ck, std::max<size_t>(size_t(nck + 1) * S.ck_floats * 4, 4));
   if (e == hipSuccess) e = hipMalloc((void**)&seg, std::max<size_t>(size_t(K) * pml_seg * 4, 4));
@@ -337,7 +340,6 @@ extern "C" int ard_adjoint(ard_ctx* ctx, int nsteps, const float* dl_drec, float
     pblocks.push_back(blocks);
     blocks += cdiv(A.n, 256);
   }
-  AdjPmlDesc* d_pd = nullptr;
   const size_t npd = S.pml.size();
   if (npd) {
     for (int s = 0; s < 2; ++s)
@@ -365,9 +367,9 @@ extern "C" int ard_adjoint(ard_ctx* ctx, int nsteps, const float* dl_drec, float
           pd.push_back(d);
         }
       }
-    if ((r = c->dalloc((void**)&d_pd, pd.size() * sizeof(AdjPmlDesc))) ||
-        (r = c->upload(d_pd, pd.data(), pd.size() * sizeof(AdjPmlDesc))))
-      return fail_with(r);
+    if ((e = hipMalloc((void**)&d_pd, pd.size() * sizeof(AdjPmlDesc))) != hipSuccess)
+      return fail_with(hip_fail(e, "adjoint PML descriptors"));
+    if ((r = c->upload(d_pd, pd.data(), pd.size() * sizeof(AdjPmlDesc)))) return fail_with(r);
   }
   int set = 0;
   const float b = 0.999f;

@@ -32,6 +32,9 @@
 namespace {
 
 constexpr double kPi = 3.14159265358979323846;
+// Loops over fewer elements than this run on one thread: on small scenes a
+// parallel region per loop costs more than it saves.
+constexpr long kParallelMin = 1L << 16;
 
 enum PmlType { P_LEFT = 0, P_RIGHT, P_TOP, P_BOTTOM, P_FRONT, P_BACK };
 enum BType { X_BOUNDARY = 0, Y_BOUNDARY, Z_BOUNDARY };
@@ -61,8 +64,9 @@ void transform3d(int W, int H, int D, const DctTables& tx, const DctTables& ty, 
   const std::vector<double>& mz = inverse ? tz.inv : tz.fwd;
   const size_t plane = size_t(W) * H;
   tmp.assign(a.size(), 0.0);
+  const bool par = long(a.size()) > kParallelMin;
   // x
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if (par)
   for (long r = 0; r < long(H) * D; ++r) {
     const double* in = &a[size_t(r) * W];
     double* out = &tmp[size_t(r) * W];
@@ -75,7 +79,7 @@ void transform3d(int W, int H, int D, const DctTables& tx, const DctTables& ty, 
   }
   // y
   std::fill(a.begin(), a.end(), 0.0);
-#pragma omp parallel for collapse(2) schedule(static)
+#pragma omp parallel for collapse(2) schedule(static) if (par)
   for (int z = 0; z < D; ++z)
     for (int k = 0; k < H; ++k) {
       double* out = &a[z * plane + size_t(k) * W];
@@ -87,7 +91,7 @@ void transform3d(int W, int H, int D, const DctTables& tx, const DctTables& ty, 
     }
   // z
   std::fill(tmp.begin(), tmp.end(), 0.0);
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if (par)
   for (int k = 0; k < D; ++k) {
     double* out = &tmp[size_t(k) * plane];
     for (int j = 0; j < D; ++j) {
@@ -97,6 +101,312 @@ void transform3d(int W, int H, int D, const DctTables& tx, const DctTables& ty, 
     }
   }
   a.swap(tmp);
+}
+
+// ---------------------------------------------------------------------------
+// O(n log n) REDFT10 / REDFT01, the algorithm class of the reference's FFTW
+// plans (ARD/dct_volume.cpp:17,19 plan fftwf_plan_r2r_3d with FFTW_REDFT10 /
+// FFTW_REDFT01; FFTW 3.3.5 computes them through FFTs). Restated here from the
+// published method, not from FFTW's code:
+//   * Makhoul's reordering (IEEE Trans. ASSP 28(1), 1980): for a line x of
+//     length n, v[j] = x[2j] (j < ceil(n/2)), v[n-1-j] = x[2j+1]
+//     (j < floor(n/2)); with V = DFT_n(v), REDFT10(x)_k = 2 Re(e^{-i pi k/2n} V_k).
+//     Inversely REDFT01(X) = the real part of the unnormalised inverse DFT of
+//     e^{i pi k/2n} (X_k - i X_{n-k}) (X_n = 0), read back through the same
+//     reordering.
+//   * the DFT by recursive mixed-radix Cooley-Tukey decimation in time (any n:
+//     radix 4, 2, 3, 5 butterflies, other primes as a direct DFT of that
+//     length), in double, applied to a batch of lines at once: element j of
+//     every line of the batch is one contiguous vector, so the inner loops run
+//     over the batch.
+// Pinned against the dense sums (transform3d) and scipy.fft.dct type 2/3.
+
+struct FftPlan {
+  int n = 0;
+  std::vector<int> radix;         // factorisation of n, radices of the recursion levels
+  std::vector<double> wr, wi;     // e^{-2 pi i e / n}, e in [0, n)
+  std::vector<double> pr, pi;     // Makhoul post/pre twiddles e^{-i pi k / 2n}
+  std::vector<int> perm;          // v[j] = x[perm[j]]
+  explicit FftPlan(int n_) : n(n_), wr(n_), wi(n_), pr(n_), pi(n_), perm(n_) {
+    int m = n;
+    while (m % 4 == 0) { radix.push_back(4); m /= 4; }
+    while (m % 2 == 0) { radix.push_back(2); m /= 2; }
+    for (int p = 3; m > 1; p += 2)
+      while (m % p == 0) { radix.push_back(p); m /= p; }
+    for (int e = 0; e < n; ++e) {
+      wr[e] = std::cos(2.0 * kPi * e / n);
+      wi[e] = -std::sin(2.0 * kPi * e / n);
+      pr[e] = std::cos(kPi * e / (2.0 * n));
+      pi[e] = -std::sin(kPi * e / (2.0 * n));
+    }
+    for (int j = 0; 2 * j < n; ++j) perm[j] = 2 * j;
+    for (int j = 0; 2 * j + 1 < n; ++j) perm[n - 1 - j] = 2 * j + 1;
+  }
+};
+
+// out[k] (k < len) = DFT of the len inputs in[r * istride], r < len, every
+// element a batch vector of B doubles (re and im arrays); level = index into
+// P.radix; the twiddle W_len^e is P.w[e * (P.n / len)].
+void fft_rec(const FftPlan& P, int level, int len, const double* in_r, const double* in_i, long istride,
+             double* out_r, double* out_i, int B) {
+  if (len == 1) {
+    std::memcpy(out_r, in_r, sizeof(double) * B);
+    std::memcpy(out_i, in_i, sizeof(double) * B);
+    return;
+  }
+  const int p = P.radix[level];
+  const int m = len / p;
+  for (int r = 0; r < p; ++r)  // sub-DFT r of x[r + p j] -> out[r m + k]
+    fft_rec(P, level + 1, m, in_r + r * istride * B, in_i + r * istride * B, istride * p, out_r + long(r) * m * B,
+            out_i + long(r) * m * B, B);
+  const long ts = P.n / len;  // twiddle stride for W_len
+  std::vector<double> tr(p), ti(p), gr, gi;
+  if (p > 5) { gr.resize(size_t(p) * B); gi.resize(size_t(p) * B); }
+  // Twiddled input r of column k, element b: t_r = W_len^{r k} Y_r[k].
+  for (int k = 0; k < m; ++k) {
+    for (int r = 1; r < p; ++r) {
+      const long e = (long(r) * k) * ts;
+      tr[r] = P.wr[e];
+      ti[r] = P.wi[e];
+    }
+    // X[k + m q] = sum_r t_r W_p^{r q}, written over the inputs (the same slots)
+    double* __restrict y0r = out_r + long(k) * B;
+    double* __restrict y0i = out_i + long(k) * B;
+    const long sB = long(m) * B;
+    if (p == 2) {
+      double* __restrict y1r = y0r + sB;
+      double* __restrict y1i = y0i + sB;
+      const double c1 = tr[1], s1 = ti[1];
+      for (int b = 0; b < B; ++b) {
+        const double ar = y0r[b], ai = y0i[b];
+        const double br = y1r[b] * c1 - y1i[b] * s1, bi = y1r[b] * s1 + y1i[b] * c1;
+        y0r[b] = ar + br; y0i[b] = ai + bi;
+        y1r[b] = ar - br; y1i[b] = ai - bi;
+      }
+    } else if (p == 4) {
+      double* __restrict y1r = y0r + sB;
+      double* __restrict y1i = y0i + sB;
+      double* __restrict y2r = y0r + 2 * sB;
+      double* __restrict y2i = y0i + 2 * sB;
+      double* __restrict y3r = y0r + 3 * sB;
+      double* __restrict y3i = y0i + 3 * sB;
+      const double c1 = tr[1], s1 = ti[1], c2 = tr[2], s2 = ti[2], c3 = tr[3], s3 = ti[3];
+      for (int b = 0; b < B; ++b) {
+        const double a0r = y0r[b], a0i = y0i[b];
+        const double a1r = y1r[b] * c1 - y1i[b] * s1, a1i = y1r[b] * s1 + y1i[b] * c1;
+        const double a2r = y2r[b] * c2 - y2i[b] * s2, a2i = y2r[b] * s2 + y2i[b] * c2;
+        const double a3r = y3r[b] * c3 - y3i[b] * s3, a3i = y3r[b] * s3 + y3i[b] * c3;
+        const double s02r = a0r + a2r, s02i = a0i + a2i, d02r = a0r - a2r, d02i = a0i - a2i;
+        const double s13r = a1r + a3r, s13i = a1i + a3i, d13r = a1r - a3r, d13i = a1i - a3i;
+        // W_4 = -i: X1 = d02 - i d13, X3 = d02 + i d13
+        y0r[b] = s02r + s13r; y0i[b] = s02i + s13i;
+        y2r[b] = s02r - s13r; y2i[b] = s02i - s13i;
+        y1r[b] = d02r + d13i; y1i[b] = d02i - d13r;
+        y3r[b] = d02r - d13i; y3i[b] = d02i + d13r;
+      }
+    } else if (p == 3) {
+      // W_3 = -1/2 - i sqrt(3)/2: X0 = a0 + t, X1,2 = a0 - t/2 -/+ i (sqrt(3)/2) (a1 - a2)
+      double* __restrict y1r = y0r + sB;
+      double* __restrict y1i = y0i + sB;
+      double* __restrict y2r = y0r + 2 * sB;
+      double* __restrict y2i = y0i + 2 * sB;
+      const double c1 = tr[1], s1 = ti[1], c2 = tr[2], s2 = ti[2];
+      const double h3 = std::sqrt(3.0) / 2.0;
+      for (int b = 0; b < B; ++b) {
+        const double a0r = y0r[b], a0i = y0i[b];
+        const double a1r = y1r[b] * c1 - y1i[b] * s1, a1i = y1r[b] * s1 + y1i[b] * c1;
+        const double a2r = y2r[b] * c2 - y2i[b] * s2, a2i = y2r[b] * s2 + y2i[b] * c2;
+        const double tr_ = a1r + a2r, ti_ = a1i + a2i, ur = (a1r - a2r) * h3, ui = (a1i - a2i) * h3;
+        const double mr = a0r - 0.5 * tr_, mi = a0i - 0.5 * ti_;
+        y0r[b] = a0r + tr_; y0i[b] = a0i + ti_;
+        y1r[b] = mr + ui; y1i[b] = mi - ur;
+        y2r[b] = mr - ui; y2i[b] = mi + ur;
+      }
+    } else if (p == 5) {
+      // W_5^r = cos(2 pi r/5) - i sin(2 pi r/5); pairs (1,4) and (2,3)
+      double* __restrict y1r = y0r + sB;
+      double* __restrict y1i = y0i + sB;
+      double* __restrict y2r = y0r + 2 * sB;
+      double* __restrict y2i = y0i + 2 * sB;
+      double* __restrict y3r = y0r + 3 * sB;
+      double* __restrict y3i = y0i + 3 * sB;
+      double* __restrict y4r = y0r + 4 * sB;
+      double* __restrict y4i = y0i + 4 * sB;
+      const double c1 = tr[1], s1 = ti[1], c2 = tr[2], s2 = ti[2], c3 = tr[3], s3 = ti[3], c4 = tr[4], s4 = ti[4];
+      const double k1 = std::cos(2 * kPi / 5), k2 = std::cos(4 * kPi / 5);
+      const double q1 = std::sin(2 * kPi / 5), q2 = std::sin(4 * kPi / 5);
+      for (int b = 0; b < B; ++b) {
+        const double a0r = y0r[b], a0i = y0i[b];
+        const double a1r = y1r[b] * c1 - y1i[b] * s1, a1i = y1r[b] * s1 + y1i[b] * c1;
+        const double a2r = y2r[b] * c2 - y2i[b] * s2, a2i = y2r[b] * s2 + y2i[b] * c2;
+        const double a3r = y3r[b] * c3 - y3i[b] * s3, a3i = y3r[b] * s3 + y3i[b] * c3;
+        const double a4r = y4r[b] * c4 - y4i[b] * s4, a4i = y4r[b] * s4 + y4i[b] * c4;
+        const double t1r = a1r + a4r, t1i = a1i + a4i, t2r = a2r + a3r, t2i = a2i + a3i;
+        const double t3r = a1r - a4r, t3i = a1i - a4i, t4r = a2r - a3r, t4i = a2i - a3i;
+        const double b1r = a0r + k1 * t1r + k2 * t2r, b1i = a0i + k1 * t1i + k2 * t2i;
+        const double b2r = a0r + k2 * t1r + k1 * t2r, b2i = a0i + k2 * t1i + k1 * t2i;
+        const double d1r = q1 * t3r + q2 * t4r, d1i = q1 * t3i + q2 * t4i;
+        const double d2r = q2 * t3r - q1 * t4r, d2i = q2 * t3i - q1 * t4i;
+        y0r[b] = a0r + t1r + t2r; y0i[b] = a0i + t1i + t2i;
+        // X1 = b1 - i d1, X4 = b1 + i d1, X2 = b2 - i d2, X3 = b2 + i d2
+        y1r[b] = b1r + d1i; y1i[b] = b1i - d1r;
+        y4r[b] = b1r - d1i; y4i[b] = b1i + d1r;
+        y2r[b] = b2r + d2i; y2i[b] = b2i - d2r;
+        y3r[b] = b2r - d2i; y3i[b] = b2i + d2r;
+      }
+    } else {
+      // any other prime: direct DFT of length p over the twiddled inputs
+      for (int r = 0; r < p; ++r) {
+        const double* __restrict sr = y0r + r * sB;
+        const double* __restrict si = y0i + r * sB;
+        double* __restrict g_r = gr.data() + size_t(r) * B;
+        double* __restrict g_i = gi.data() + size_t(r) * B;
+        const double c = r ? tr[r] : 1.0, sn = r ? ti[r] : 0.0;
+        for (int b = 0; b < B; ++b) {
+          g_r[b] = sr[b] * c - si[b] * sn;
+          g_i[b] = sr[b] * sn + si[b] * c;
+        }
+      }
+      const long ps = P.n / p;
+      for (int q = 0; q < p; ++q) {
+        double* __restrict orr = y0r + q * sB;
+        double* __restrict oi = y0i + q * sB;
+        std::memcpy(orr, gr.data(), sizeof(double) * B);
+        std::memcpy(oi, gi.data(), sizeof(double) * B);
+        for (int r = 1; r < p; ++r) {
+          const long e = (long(r) * q) % p * ps;
+          const double cr = P.wr[e], ci = P.wi[e];
+          const double* __restrict sr = gr.data() + size_t(r) * B;
+          const double* __restrict si = gi.data() + size_t(r) * B;
+          for (int b = 0; b < B; ++b) {
+            orr[b] += sr[b] * cr - si[b] * ci;
+            oi[b] += sr[b] * ci + si[b] * cr;
+          }
+        }
+      }
+    }
+  }
+}
+
+// REDFT10 (inverse = false) or REDFT01 (inverse = true) of `nb` lines of
+// length P.n: element j of line b at base[b * bstride + j * lstride].
+// Two real lines share one complex FFT: line b in the real part and line
+// b + Bh in the imaginary part (Bh = ceil(nb / 2)); the forward transform
+// separates their spectra as A_k = (Z_k + conj Z_{n-k}) / 2,
+// B_k = (Z_k - conj Z_{n-k}) / 2i; the inverse's outputs are real, so the two
+// lines come back as the real and imaginary parts.
+void dct_lines(const FftPlan& P, double* base, long lstride, long bstride, int nb, bool inverse,
+               std::vector<double>& scratch) {
+  const int n = P.n;
+  const int B = (nb + 1) / 2;
+  const int nhi = nb - B;  // lines in the imaginary part
+  scratch.resize(size_t(4) * n * B);
+  double* vr = scratch.data();
+  double* vi = vr + size_t(n) * B;
+  double* xr = vi + size_t(n) * B;
+  double* xi = xr + size_t(n) * B;
+  if (!inverse) {
+    for (int j = 0; j < n; ++j) {
+      const double* src = base + long(P.perm[j]) * lstride;
+      double* dr = vr + size_t(j) * B;
+      double* di = vi + size_t(j) * B;
+      for (int b = 0; b < B; ++b) dr[b] = src[b * bstride];
+      for (int b = 0; b < nhi; ++b) di[b] = src[(B + b) * bstride];
+      if (nhi < B) di[B - 1] = 0.0;
+    }
+    fft_rec(P, 0, n, vr, vi, 1, xr, xi, B);
+    for (int k = 0; k < n; ++k) {
+      const double c = P.pr[k], s = P.pi[k];
+      const int nk = k ? n - k : 0;
+      const double* zr = xr + size_t(k) * B;
+      const double* zi = xi + size_t(k) * B;
+      const double* wr = xr + size_t(nk) * B;
+      const double* wi = xi + size_t(nk) * B;
+      double* d = base + long(k) * lstride;
+      for (int b = 0; b < B; ++b) {
+        const double ar = 0.5 * (zr[b] + wr[b]), ai = 0.5 * (zi[b] - wi[b]);
+        d[b * bstride] = 2.0 * (ar * c - ai * s);
+      }
+      for (int b = 0; b < nhi; ++b) {
+        const double br = 0.5 * (zi[b] + wi[b]), bi = -0.5 * (zr[b] - wr[b]);
+        d[(B + b) * bstride] = 2.0 * (br * c - bi * s);
+      }
+    }
+  } else {
+    // V_k = e^{i pi k/2n} (X_k - i X_{n-k}) per line; Z = V_lo + i V_hi;
+    // inverse DFT as conj(DFT(conj Z))
+    for (int k = 0; k < n; ++k) {
+      const double c = P.pr[k], s = -P.pi[k];  // e^{+i pi k/2n} = c + i s
+      const double* a = base + long(k) * lstride;
+      const double* bq = k ? base + long(n - k) * lstride : nullptr;
+      double* dr = vr + size_t(k) * B;
+      double* di = vi + size_t(k) * B;
+      for (int b = 0; b < B; ++b) {
+        const double re = a[b * bstride];
+        const double im = bq ? -bq[b * bstride] : 0.0;
+        double zr = re * c - im * s, zi = re * s + im * c;
+        if (b < nhi) {
+          const double re2 = a[(B + b) * bstride];
+          const double im2 = bq ? -bq[(B + b) * bstride] : 0.0;
+          zr -= re2 * s + im2 * c;  // + i (re2 c - im2 s + i (re2 s + im2 c))
+          zi += re2 * c - im2 * s;
+        }
+        dr[b] = zr;
+        di[b] = -zi;  // conjugated
+      }
+    }
+    fft_rec(P, 0, n, vr, vi, 1, xr, xi, B);
+    for (int j = 0; j < n; ++j) {
+      const double* ar = xr + size_t(j) * B;
+      const double* ai = xi + size_t(j) * B;
+      double* d = base + long(P.perm[j]) * lstride;
+      for (int b = 0; b < B; ++b) d[b * bstride] = ar[b];
+      for (int b = 0; b < nhi; ++b) d[(B + b) * bstride] = -ai[b];  // conj of the DFT
+    }
+  }
+}
+
+// The separable 3D transform of transform3d by FFT, in place on a (z-major,
+// x fastest). Lines along y and z are batched over x (and y), which are
+// contiguous; lines along x are batched over rows.
+void transform3d_fft(int W, int H, int D, const FftPlan& px, const FftPlan& py, const FftPlan& pz, bool inverse,
+                     std::vector<double>& a) {
+  const long plane = long(W) * H;
+  auto chunk = [](int n) { return std::max(8, std::min(512, (1 << 15) / std::max(n, 1))); };
+  {  // x: lines contiguous, batch over rows
+    const int B = chunk(W);
+    const long rows = long(H) * D;
+#pragma omp parallel if (long(W) * H * D > kParallelMin)
+    {
+      std::vector<double> s;
+#pragma omp for schedule(dynamic)
+      for (long r0 = 0; r0 < rows; r0 += B)
+        dct_lines(px, a.data() + r0 * W, 1, W, int(std::min<long>(B, rows - r0)), inverse, s);
+    }
+  }
+  {  // y: for each z plane, lines of stride W, batch over x
+    const int B = chunk(H);
+    const long nchunks = (W + B - 1) / B;
+#pragma omp parallel if (long(W) * H * D > kParallelMin)
+    {
+      std::vector<double> s;
+#pragma omp for schedule(dynamic) collapse(2)
+      for (int z = 0; z < D; ++z)
+        for (long c = 0; c < nchunks; ++c)
+          dct_lines(py, a.data() + z * plane + c * B, W, 1, int(std::min<long>(B, W - c * B)), inverse, s);
+    }
+  }
+  {  // z: lines of stride W*H, batch over the plane
+    const int B = chunk(D);
+    const long nchunks = (plane + B - 1) / B;
+#pragma omp parallel if (long(W) * H * D > kParallelMin)
+    {
+      std::vector<double> s;
+#pragma omp for schedule(dynamic)
+      for (long c = 0; c < nchunks; ++c)
+        dct_lines(pz, a.data() + c * B, plane, 1, int(std::min<long>(B, plane - c * B)), inverse, s);
+    }
+  }
 }
 
 template <typename R>
@@ -130,6 +440,12 @@ struct Bnd {
   int xs, xe, ys, ye, zs, ze;
 };
 
+// Face elements of a boundary (the loop count of its forcing).
+long bnd_area(const Bnd& b) {
+  return b.type == X_BOUNDARY ? long(b.ye - b.ys) * (b.ze - b.zs)
+         : b.type == Y_BOUNDARY ? long(b.xe - b.xs) * (b.ze - b.zs) : long(b.xe - b.xs) * (b.ye - b.ys);
+}
+
 struct Source {
   int x, y, z;
   std::vector<double> wave;  // empty -> Gaussian pulse
@@ -154,7 +470,9 @@ struct Sim {
   int time_step = 0;
   int rec_local = 0;  // 0: reference addressing (global coords index the array), 1: local cell
   bool finalized = false;
+  int fft_dct = 1;  // 1: O(n log n) transforms (transform3d_fft), 0: dense sums (transform3d)
   std::vector<std::unique_ptr<DctTables>> tables;  // cached by size
+  std::vector<std::unique_ptr<FftPlan>> plans;     // cached by size
   std::vector<double> work, tmp;
 
   const DctTables& table(int n) {
@@ -162,6 +480,16 @@ struct Sim {
       if (t->n == n) return *t;
     tables.emplace_back(new DctTables(n));
     return *tables.back();
+  }
+  const FftPlan& plan(int n) {
+    for (auto& t : plans)
+      if (t->n == n) return *t;
+    plans.emplace_back(new FftPlan(n));
+    return *plans.back();
+  }
+  void transform(Part<R>& p, bool inverse) {
+    if (fft_dct) transform3d_fft(p.w, p.h, p.d, plan(p.w), plan(p.h), plan(p.d), inverse, work);
+    else transform3d(p.w, p.h, p.d, table(p.w), table(p.h), table(p.d), inverse, work, tmp);
   }
 
   // Partition ctor + DctPartition ctor tables: ARD/partition.cpp:10-52, ARD/dct_partition.cpp:7-37.
@@ -397,31 +725,36 @@ struct Sim {
 
   // DctVolume::ExecuteDct / ExecuteIdct with scaling: ARD/dct_volume.cpp:38-68.
   void dct_forward(Part<R>& p) {
-    const size_t n = p.ncell();
+    const long n = long(p.ncell());
     work.resize(n);
-    for (size_t i = 0; i < n; ++i) work[i] = double(p.f_values[i]);
-    transform3d(p.w, p.h, p.d, table(p.w), table(p.h), table(p.d), false, work, tmp);
+#pragma omp parallel for schedule(static) if (n > kParallelMin)
+    for (long i = 0; i < n; ++i) work[i] = double(p.f_values[i]);
+    transform(p, false);
     const R scale = R(1) / (R(2) * std::sqrt(R(2) * R(p.d) * R(p.w) * R(p.h)));
-    for (size_t i = 0; i < n; ++i) p.f_modes[i] = R(work[i]) * scale;
+#pragma omp parallel for schedule(static) if (n > kParallelMin)
+    for (long i = 0; i < n; ++i) p.f_modes[i] = R(work[i]) * scale;
   }
   void dct_inverse(Part<R>& p) {
-    const size_t n = p.ncell();
+    const long n = long(p.ncell());
     work.resize(n);
-    for (size_t i = 0; i < n; ++i) work[i] = double(p.p_modes[i]);
-    transform3d(p.w, p.h, p.d, table(p.w), table(p.h), table(p.d), true, work, tmp);
+#pragma omp parallel for schedule(static) if (n > kParallelMin)
+    for (long i = 0; i < n; ++i) work[i] = double(p.p_modes[i]);
+    transform(p, true);
     const R scale = R(1) / std::sqrt(R(2) * R(p.d) * R(p.w) * R(p.h));
-    for (size_t i = 0; i < n; ++i) p.p_values[i] = R(work[i]) * scale;
+#pragma omp parallel for schedule(static) if (n > kParallelMin)
+    for (long i = 0; i < n; ++i) p.p_values[i] = R(work[i]) * scale;
   }
 
   // DctPartition::Update: ARD/dct_partition.cpp:48-73 (the #else branch).
   void dct_update(Part<R>& p) {
     dct_forward(p);
-    const size_t n = p.ncell();
-    for (size_t i = 0; i < n; ++i)
+    const long n = long(p.ncell());
+#pragma omp parallel for schedule(static) if (n > kParallelMin)
+    for (long i = 0; i < n; ++i)
       p.next_modes[i] = R(0.999f) * (R(2) * p.p_modes[i] * p.cwt[i] - p.prev_modes[i] +
                                      (R(2) * p.f_modes[i] / p.w2[i]) * (R(1) - p.cwt[i]));
     p.prev_modes.swap(p.p_modes);  // prev <- current (memcpy at :69)
-    p.p_modes = p.next_modes;      // current <- next (memcpy at :70)
+    p.p_modes.swap(p.next_modes);  // current <- next (memcpy at :70; next is rewritten before any read)
     dct_inverse(p);
   }
 
@@ -435,7 +768,7 @@ struct Sim {
   void pml_update(Part<R>& P) {
     static const R c6[7] = {2, -27, 270, -490, 270, -27, 2};
     static const R c4[5] = {1, -8, 0, 8, -1};
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if (long(P.ncell()) > kParallelMin)
     for (int k = 0; k < P.d; ++k)
       for (int j = 0; j < P.h; ++j)
         for (int i = 0; i < P.w; ++i) {
@@ -511,6 +844,7 @@ struct Sim {
       const bool a_left = (bd.xs <= A.xe() && bd.xe >= A.xe());
       Part<R>& Lp = a_left ? A : B;
       Part<R>& Rp = a_left ? B : A;
+#pragma omp parallel for collapse(2) schedule(static) if (bnd_area(bd) > kParallelMin / 64)
       for (int i = bd.ys; i < bd.ye; ++i)
         for (int j = bd.zs; j < bd.ze; ++j) {
           const int ly = i - Lp.ys, ry = i - Rp.ys, lz = j - Lp.zs, rz = j - Rp.zs;
@@ -533,6 +867,7 @@ struct Sim {
       const bool a_top = (bd.ys <= A.ye() && bd.ye >= A.ye());
       Part<R>& T = a_top ? A : B;
       Part<R>& Bo = a_top ? B : A;
+#pragma omp parallel for collapse(2) schedule(static) if (bnd_area(bd) > kParallelMin / 64)
       for (int i = bd.xs; i < bd.xe; ++i)
         for (int j = bd.zs; j < bd.ze; ++j) {
           const int tx = i - T.xs, bx = i - Bo.xs, tz = j - T.zs, bz = j - Bo.zs;
@@ -553,6 +888,7 @@ struct Sim {
       const bool a_front = (bd.zs <= A.ze() && bd.ze >= A.ze());
       Part<R>& Fr = a_front ? A : B;
       Part<R>& Bk = a_front ? B : A;
+#pragma omp parallel for collapse(2) schedule(static) if (bnd_area(bd) > kParallelMin / 64)
       for (int i = bd.xs; i < bd.xe; ++i)
         for (int j = bd.ys; j < bd.ye; ++j) {
           const int fx = i - Fr.xs, bx = i - Bk.xs, fy = j - Fr.ys, by = j - Bk.ys;
@@ -632,6 +968,14 @@ int orc_set_pml_scale(void* h, const double* scale6) {
   return dispatch(static_cast<Handle*>(h), [&](auto& s) {
     if (s.finalized) return -1;
     for (int i = 0; i < 6; ++i) s.pml_scale[i] = scale6[i];
+    return 0;
+  });
+}
+
+// 1: O(n log n) transforms (default), 0: the dense per-axis sums.
+int orc_set_fft_dct(void* h, int on) {
+  return dispatch(static_cast<Handle*>(h), [&](auto& s) {
+    s.fft_dct = on ? 1 : 0;
     return 0;
   });
 }
@@ -785,10 +1129,16 @@ int orc_set_field(void* h, int part, int which, const double* in) {
 
 // Scaled 3D DCT-II (inverse=0) or DCT-III (inverse=1) exactly as
 // DctVolume::ExecuteDct/ExecuteIdct (ARD/dct_volume.cpp:38-68), in R.
-void orc_dct3d(int use_double, int w, int h, int d, const double* in, double* out, int inverse) {
-  DctTables tx(w), ty(h), tz(d);
+// fft: 1 computes the transforms by FFT (transform3d_fft), 0 by the dense sums.
+void orc_dct3d(int use_double, int w, int h, int d, const double* in, double* out, int inverse, int fft) {
   std::vector<double> a(in, in + size_t(w) * h * d), tmp;
-  transform3d(w, h, d, tx, ty, tz, inverse != 0, a, tmp);
+  if (fft) {
+    FftPlan px(w), py(h), pz(d);
+    transform3d_fft(w, h, d, px, py, pz, inverse != 0, a);
+  } else {
+    DctTables tx(w), ty(h), tz(d);
+    transform3d(w, h, d, tx, ty, tz, inverse != 0, a, tmp);
+  }
   if (use_double) {
     const double sc = inverse ? 1.0 / std::sqrt(2.0 * d * w * h) : 1.0 / (2.0 * std::sqrt(2.0 * d * w * h));
     for (size_t i = 0; i < a.size(); ++i) out[i] = a[i] * sc;

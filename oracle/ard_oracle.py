@@ -47,7 +47,8 @@ def lib():
         L.orc_step.argtypes = [vp, i, dp]
         L.orc_get_field.argtypes = [vp, i, i, dp]
         L.orc_set_field.argtypes = [vp, i, i, dp]
-        L.orc_dct3d.argtypes = [i, i, i, i, dp, dp, i]
+        L.orc_dct3d.argtypes = [i, i, i, i, dp, dp, i, i]
+        L.orc_set_fft_dct.argtypes = [vp, i]
         L.orc_mode_tables.argtypes = [i, d, d, d, i, i, i, dp, dp]
         L.orc_gaussian.argtypes = [i, d, d, d, i, dp]
         L.orc_pml_profile.argtypes = [i, d, d, i, i, i, i, i, dp]
@@ -76,6 +77,10 @@ class OracleSim:
     def set_pml_scale(self, scale6):
         a = np.ascontiguousarray(scale6, dtype=np.float64)
         assert self.L.orc_set_pml_scale(self.h, _dp(a)) == 0
+
+    def set_fft_dct(self, on):
+        """True: O(n log n) transforms (default); False: the dense per-axis sums."""
+        assert self.L.orc_set_fft_dct(self.h, int(bool(on))) == 0
 
     def set_recorder_mode(self, local):
         assert self.L.orc_set_recorder_mode(self.h, int(local)) == 0
@@ -137,12 +142,13 @@ class OracleSim:
         assert self.L.orc_set_field(self.h, part, which, _dp(a)) == 0
 
 
-def dct3d(x, inverse=False, double=True):
-    """Scaled ARD 3D DCT-II / DCT-III of x[d][h][w] (ARD/dct_volume.cpp:38-68)."""
+def dct3d(x, inverse=False, double=True, fft=False):
+    """Scaled ARD 3D DCT-II / DCT-III of x[d][h][w] (ARD/dct_volume.cpp:38-68),
+    by the dense per-axis sums (fft=False) or by FFT (fft=True)."""
     x = np.ascontiguousarray(x, dtype=np.float64)
     d, h, w = x.shape
     out = np.zeros_like(x)
-    lib().orc_dct3d(int(double), w, h, d, _dp(x), _dp(out), int(inverse))
+    lib().orc_dct3d(int(double), w, h, d, _dp(x), _dp(out), int(inverse), int(fft))
     return out
 
 
@@ -165,9 +171,10 @@ def pml_profile(dh, c0, layers, ptype, w, h, d, double=False):
     return out.reshape(3, d, h, w)
 
 
-def from_scene(scene, double=False, pml_scale=None):
-    """Build the oracle for an ard_scenes.Scene."""
+def from_scene(scene, double=False, pml_scale=None, fft=True):
+    """Build the oracle for an ard_scenes.Scene (fft: transforms by FFT or by dense sums)."""
     o = OracleSim(scene.h, scene.dt, scene.c0, scene.pml_layers, scene.absorption, double=double)
+    o.set_fft_dct(fft)
     if pml_scale is not None:
         o.set_pml_scale(pml_scale)
     o.set_recorder_mode(scene.recorder_mode)

@@ -4,7 +4,13 @@ The traces come from the CPU oracle (oracle/ard_oracle.cpp) in FP32 (the
 reference's arithmetic) and FP64; the oracle is pinned by
 tests/test_oracle_pins.py. Run from the repository root:
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py            # two rooms, scene-3, hall cell files
+    python tests/golden/make_golden.py --hall     # hall traces (2000 steps; ~30 min per precision)
+
+The hall traces (hall_fp64.npy, hall_fp32.npy: [2000][16] recorder samples of
+the seeded L-shaped hall, 74M DCT + 6.3M PML cells) are computed with the
+oracle's O(n log n) transforms; they state the FP32 forms' recorder error
+against FP64 on the full-size batched workload.
 """
 import json
 import os
@@ -20,7 +26,25 @@ import ard_scenes as S  # noqa: E402
 OUT = os.path.dirname(os.path.abspath(__file__))
 
 
+def hall_traces():
+    sc = S.hall()
+    for double in (True, False):
+        o = O.from_scene(sc, double=double)
+        chunks = []
+        for t0 in range(0, sc.steps, 100):
+            chunks.append(o.step(min(100, sc.steps - t0)))
+            print("hall %s: %d steps" % ("fp64" if double else "fp32", t0 + len(chunks[-1])), flush=True)
+        tr = np.concatenate(chunks)
+        name = "hall_%s.npy" % ("fp64" if double else "fp32")
+        np.save(os.path.join(OUT, name), tr.astype(np.float64 if double else np.float32))
+        print(name, tr.shape, float(np.abs(tr).max()), flush=True)
+        del o
+
+
 def main():
+    if "--hall" in sys.argv[1:]:
+        hall_traces()
+        return
     # the seeded hall in cell units (ard_run --cells)
     S.write_cell_files(S.hall(), os.path.join(OUT, "hall_cells"))
     for sc in (S.two_rooms(), S.scene3()):

@@ -26,3 +26,26 @@ def test_batch_rooms_match_rooms_alone(form):
         print("room %d rel-L2 %.2e" % (r, err))
         assert err <= 1e-6
     b.close()
+
+
+@pytest.mark.parametrize("what", ["sources", "recorders"])
+def test_batched_refuses_more_than_four_points_per_partition(what):
+    """The batched form carries at most 4 interior point sources and 4
+    recorders per DCT partition in its modal pass (the plain form has no such
+    limit, like the reference); a fifth is refused at finalize with
+    ARD_ERR_UNSUPPORTED, not dropped."""
+    sc = S.two_rooms()
+    sc.recorder_mode = S.REC_LOCAL
+    pts = [(8 + 3 * i, 12, 12) for i in range(5)]  # interior cells of box 0 (no face slab)
+    if what == "sources":
+        sc.sources = [S.Source(*p) for p in pts]
+    else:
+        sc.recorders = pts
+    with pytest.raises(A.ArdError, match="error -5"):
+        A.Simulation(sc, form=A.FORM_BATCHED)
+    A.Simulation(sc, form=A.FORM_PLAIN).close()  # the plain form takes five
+    if what == "sources":  # and the batched form four
+        sc.sources = sc.sources[:4]
+    else:
+        sc.recorders = sc.recorders[:4]
+    A.Simulation(sc, form=A.FORM_BATCHED).close()

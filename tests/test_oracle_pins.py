@@ -456,3 +456,231 @@ def test_direct_arrival_time():
     # first arrival (before wall reflections); the 1-based mode numbering of the
     # reference (ARD/dct_partition.cpp:24-31) speeds up the lowest modes slightly
     assert abs(np.argmax(np.abs(tr[:130])) - expect) < 6
+
+
+# --- ARD/pml_partition.cpp:113-200 with nonzero phi (all terms at once) ------
+
+def _pml_slab(ptype, layers=5, scale=1.0, absorption=0.0, box=(12, 12, 12)):
+    o = O.OracleSim(0.1, 1 / 8000, 343.0, layers, absorption, double=True)
+    o.set_pml_scale([scale] * 6)
+    o.add_dct(0, 0, 0, *box)
+    o.finalize()
+    parts = o.partitions()
+    i = [k for k, p in enumerate(parts) if p[0] == 1 and p[1] == ptype][0]
+    return o, i, parts[i][6:9]
+
+
+@pytest.mark.parametrize("ptype", range(6))
+def test_pml_update_random_fields_all_terms(ptype):
+    """Random p, p-, phi_x, phi_y, phi_z and f on a damped slab of each face type:
+    p+ = 2p - p- + dt^2 (c^2 L6 p - sum(zeta) (p - p-)/dt - (zx zy + zy zz + zx zz) p
+    + D4x phi_x + D4y phi_y + D4z phi_z + f) and
+    phi_a+ = phi_a - dt zeta_a phi_a + dt (sum of the other zetas - zeta_a) D4a p
+    (ARD/pml_partition.cpp:158-186), every stencil applied by scipy with zero
+    padding (the GetIndex sentinel, ARD/pml_partition.cpp:6-15)."""
+    o, i, (w, h, d) = _pml_slab(ptype)
+    rng = np.random.default_rng(40 + ptype)
+    p, pm, phx, phy, phz, f = (rng.standard_normal((d, h, w)) for _ in range(6))
+    for which, a in ((0, p), (7, pm), (4, phx), (5, phy), (6, phz), (1, f)):
+        o.set_field(i, which, a)
+    o.step(1)
+    dh, dt, c0 = 0.1, 1 / 8000, 343.0
+    zx, zy, zz = O.pml_profile(dh, c0, 5, ptype, w, h, d, double=True)
+    D4 = [_zero_pad_deriv(a, C4, ax) / (12 * dh) for a, ax in ((phx, 2), (phy, 1), (phz, 0))]
+    lap = sum(_zero_pad_deriv(p, C6, ax) for ax in range(3)) / (180 * dh * dh)
+    ref = 2 * p - pm + dt * dt * (c0 * c0 * lap - (zx + zy + zz) * (p - pm) / dt
+                                  - (zx * zy + zy * zz + zx * zz) * p + D4[0] + D4[1] + D4[2] + f)
+    np.testing.assert_allclose(o.field(i, 0), ref, rtol=1e-11, atol=1e-11)
+    np.testing.assert_allclose(o.field(i, 7), p)
+    dpx, dpy, dpz = (_zero_pad_deriv(p, C4, ax) / (12 * dh) for ax in (2, 1, 0))
+    np.testing.assert_allclose(o.field(i, 4), phx - dt * zx * phx + dt * (zy + zz - zx) * dpx, rtol=1e-11, atol=1e-13)
+    np.testing.assert_allclose(o.field(i, 5), phy - dt * zy * phy + dt * (zx + zz - zy) * dpy, rtol=1e-11, atol=1e-13)
+    np.testing.assert_allclose(o.field(i, 6), phz - dt * zz * phz + dt * (zx + zy - zz) * dpz, rtol=1e-11, atol=1e-13)
+
+
+def test_pml_update_polynomial_fields_analytic_derivatives():
+    """Independent of any stencil convention: on polynomial fields the 4th-order
+    first differences and the 6th-order second differences are exact, so at
+    cells whose stencils stay inside the slab the update must equal the
+    Grote-Sim right-hand side with analytic derivatives. An 8-layer P_RIGHT
+    slab (every zeta nonzero only along x) leaves x = 3, 4 interior."""
+    dh, dt, c0 = 0.1, 1 / 8000, 343.0
+    o, i, (w, h, d) = _pml_slab(O.P_RIGHT, layers=8)
+    z, y, x = np.meshgrid(np.arange(d) * dh, np.arange(h) * dh, np.arange(w) * dh, indexing="ij")
+    # p: quadratic + cross terms; phi: quadratics; derivatives by hand
+    p = 1.0 + 0.3 * x - 0.7 * y + 0.2 * z + 0.9 * x * x - 0.4 * y * y + 0.6 * z * z + 0.5 * x * y - 0.8 * y * z
+    pm = 0.4 - 0.1 * x + 0.25 * y * z
+    phx = 0.2 + 1.5 * x * x - 0.3 * y          # d/dx = 3 x
+    phy = -0.6 * y * y + 0.7 * x * z           # d/dy = -1.2 y
+    phz = 0.9 * z - 0.35 * z * z + x * y       # d/dz = 0.9 - 0.7 z
+    f = 0.05 * x - 0.02
+    for which, a in ((0, p), (7, pm), (4, phx), (5, phy), (6, phz), (1, f)):
+        o.set_field(i, which, a)
+    o.step(1)
+    zx, zy, zz = O.pml_profile(dh, c0, 8, O.P_RIGHT, w, h, d, double=True)
+    lap_p = 2 * 0.9 + 2 * (-0.4) + 2 * 0.6
+    div_phi = 3 * x + (-1.2 * y) + (0.9 - 0.7 * z)
+    ref = 2 * p - pm + dt * dt * (c0 * c0 * lap_p - (zx + zy + zz) * (p - pm) / dt
+                                  - (zx * zy + zy * zz + zx * zz) * p + div_phi + f)
+    dpx = 0.3 + 1.8 * x + 0.5 * y
+    dpy = -0.7 - 0.8 * y + 0.5 * x - 0.8 * z
+    dpz = 0.2 + 1.2 * z - 0.8 * y
+    inner = (slice(3, d - 3), slice(3, h - 3), slice(3, w - 3))
+    assert w == 8 and p[inner].size > 0
+    np.testing.assert_allclose(o.field(i, 0)[inner], ref[inner], rtol=1e-10)
+    np.testing.assert_allclose(o.field(i, 4)[inner], (phx - dt * zx * phx + dt * (zy + zz - zx) * dpx)[inner],
+                               rtol=1e-10)
+    np.testing.assert_allclose(o.field(i, 5)[inner], (phy - dt * zy * phy + dt * (zx + zz - zy) * dpy)[inner],
+                               rtol=1e-10)
+    np.testing.assert_allclose(o.field(i, 6)[inner], (phz - dt * zz * phz + dt * (zx + zy - zz) * dpz)[inner],
+                               rtol=1e-10)
+
+
+# --- ARD/boundary.cpp:78-173: Y and Z branches, PML sides of every face ------
+
+def test_y_interface_forcing_is_global_minus_neumann_operator():
+    """Boxes stacked in y (ARD/boundary.cpp:78-125): same identity as the x pin."""
+    dh, dt, c0 = 0.1, 1 / 8000, 343.0
+    o = O.OracleSim(dh, dt, c0, 5, 0.0, double=True)
+    W, H, D = 11, 10, 12
+    o.add_dct(0, 0, 0, W, H, D)
+    o.add_dct(0, H, 0, W, H, D)
+    o.finalize()
+    assert o.boundaries()[0] == (1, 0, 1, 0, W, H - 3, H + 3, 0, D)
+    rng = np.random.default_rng(7)
+    for i in range(2):
+        o.set_field(i, 2, rng.standard_normal((D, H, W)))
+        o.set_field(i, 3, rng.standard_normal((D, H, W)))
+    o.step(1)
+    pt, pb = o.field(0, 0), o.field(1, 0)
+    top, bot = _neumann_minus_global(pt, pb, axis=1)
+    k = c0 * c0 / (180 * dh * dh)
+    ft, fb = o.field(0, 1), o.field(1, 1)
+    # cells no later (PML) boundary rewrites: x and z away from the shells
+    sl = (slice(3, D - 3), slice(None), slice(3, W - 3))
+    np.testing.assert_allclose(ft[:, H - 3:, :][sl], k * top[sl], rtol=1e-10, atol=1e-6)
+    np.testing.assert_allclose(fb[:, :3, :][sl], k * bot[sl], rtol=1e-10, atol=1e-6)
+    # and the PML boundaries, registered later, won the edge cells (absorption 0)
+    assert np.all(ft[:, H - 3:, :3] == 0) and np.all(fb[:3, :3, :] == 0)
+
+
+# (type, axis, PML before the DCT box along the axis?)
+_FACE = {O.P_LEFT: (2, True), O.P_RIGHT: (2, False), O.P_TOP: (1, True), O.P_BOTTOM: (1, False),
+         O.P_FRONT: (0, True), O.P_BACK: (0, False)}
+
+
+@pytest.mark.parametrize("ptype", range(6))
+def test_pml_boundary_both_sides_every_face(ptype):
+    """A box with absorption 0.7 and random fields in it and in its PML slabs.
+    After one step, across the face of each slab type (X: ARD/boundary.cpp:30-77,
+    Y: 78-125, Z: 126-173):
+    * the PML cells carry 0.7 c^2/(180h^2) x the 6th-order operator's part that
+      reaches into the box (include_self_terms_ = false, ARD/pml_partition.cpp:20);
+    * the box cells carry 0.7 c^2/(180h^2) x (global operator over box + slab
+      minus the box's mirrored (Neumann) operator).
+    Both computed with scipy correlate1d; the Z branch writes last, so its box
+    cells hold its values everywhere on the face; X/Y faces are checked away
+    from the z shells (rewritten by the Z boundaries)."""
+    dh, dt, c0, ab = 0.1, 1 / 8000, 343.0, 0.7
+    o = O.OracleSim(dh, dt, c0, 5, ab, double=True)
+    W, H, D = 12, 11, 10
+    o.add_dct(0, 0, 0, W, H, D)
+    o.finalize()
+    parts = o.partitions()
+    rng = np.random.default_rng(60 + ptype)
+    o.set_field(0, 2, rng.standard_normal((D, H, W)))
+    o.set_field(0, 3, rng.standard_normal((D, H, W)))
+    for q, info in enumerate(parts):
+        if info[0] == 1:
+            w, h, d = info[6:9]
+            o.set_field(q, 0, rng.standard_normal((d, h, w)))
+            o.set_field(q, 7, rng.standard_normal((d, h, w)))
+    o.step(1)
+    i = [q for q, p in enumerate(parts) if p[0] == 1 and p[1] == ptype][0]
+    axis, before = _FACE[ptype]
+    pd, pp = o.field(0, 0), o.field(i, 0)
+    fd, fp = o.field(0, 1), o.field(i, 1)
+    k = ab * c0 * c0 / (180 * dh * dh)
+    n_d, n_p = pd.shape[axis], pp.shape[axis]
+    take = lambda a, idx: np.take(a, list(idx), axis=axis)
+    full = np.concatenate([pp, pd] if before else [pd, pp], axis=axis)
+    only_box = np.concatenate([np.zeros_like(pp), pd] if before else [pd, np.zeros_like(pp)], axis=axis)
+    g = scipy.ndimage.correlate1d(full, C6, axis=axis, mode="constant")
+    gb = scipy.ndimage.correlate1d(only_box, C6, axis=axis, mode="constant")
+    mirror = scipy.ndimage.correlate1d(pd, C6, axis=axis, mode="reflect")
+    if before:   # slab cells n_p-3..n_p-1, box cells 0..2
+        pml_cells, box_full, box_own = range(n_p - 3, n_p), range(n_p, n_p + 3), range(0, 3)
+    else:        # box cells n_d-3..n_d-1, slab cells 0..2
+        pml_cells, box_full, box_own = range(n_d, n_d + 3), range(n_d - 3, n_d), range(n_d - 3, n_d)
+    exp_pml = k * take(gb, pml_cells)
+    exp_box = k * (take(g, box_full) - take(mirror, box_own))
+    got_pml = take(fp, range(n_p - 3, n_p) if before else range(0, 3))
+    got_box = take(fd, box_own)
+    np.testing.assert_allclose(got_pml, exp_pml, rtol=1e-10, atol=1e-6)
+    if axis == 0:
+        np.testing.assert_allclose(got_box, exp_box, rtol=1e-10, atol=1e-6)
+    else:
+        inner = [slice(3, D - 3), slice(None), slice(None)]
+        # along the face the other tangential axis is also away from the other slabs' shells
+        other = 1 if axis == 2 else 2
+        inner[other] = slice(3, (H if other == 1 else W) - 3)
+        np.testing.assert_allclose(got_box[tuple(inner)], exp_box[tuple(inner)], rtol=1e-10, atol=1e-6)
+    assert np.abs(exp_pml).max() > 1 and np.abs(exp_box).max() > 1
+
+
+def test_mirror_symmetry_y_stacked_rooms():
+    """y -> 2H-1-y maps two y-stacked boxes onto each other (Y-boundary branch,
+    PML top/bottom slabs and their last-writer order all map consistently)."""
+    def run(src, rec):
+        sc = S.Scene("y_pair", 0.1, 1 / 8000, 343.0, [(0, 0, 0, 24, 20, 22), (0, 20, 0, 24, 20, 22)],
+                     [S.Source(*src)], [rec], 150, recorder_mode=S.REC_LOCAL)
+        return O.from_scene(sc, double=True).step(150)[:, 0]
+    a = run((9, 6, 10), (12, 28, 11))
+    b = run((9, 33, 10), (12, 11, 11))
+    assert np.abs(a).max() > 0
+    np.testing.assert_allclose(a, b, rtol=1e-8, atol=1e-10 * np.abs(a).max())
+
+
+# --- the differentiation target's damping (Grote-Sim PML near its limit) ------
+
+def test_pml_damping_scale_1p2_diverges_scale_1_does_not():
+    """The reference's PML update with the damping profile scaled by 1.2 grows
+    without bound on the 1/4-scale differentiation room, while scale 1 stays
+    bounded: the reason the gradient workload's target uses damping / 1.2."""
+    sc = S.diff_room(scale=4, steps=1200)
+    peaks = {}
+    for scale in (1.0, 1.2):
+        o = O.from_scene(sc, double=True, pml_scale=[scale] * 6)
+        tr = o.step(1200)
+        early = np.abs(tr[:400]).max()
+        late = np.abs(tr[800:]).max() if np.all(np.isfinite(tr[800:])) else np.inf
+        peaks[scale] = (early, late)
+    e1, l1 = peaks[1.0]
+    e2, l2 = peaks[1.2]
+    assert np.isfinite(l1) and l1 < 10 * e1
+    assert not np.isfinite(l2) or l2 > 1e6 * e2
+
+
+# --- the O(n log n) transforms (REDFT10/REDFT01 by FFT) ------------------------
+
+@pytest.mark.parametrize("shape", [(3, 4, 5), (1, 1, 1), (2, 1, 3), (13, 37, 40), (7, 11, 49), (20, 40, 60)])
+def test_fft_dct_matches_scipy_and_dense(shape):
+    rng = np.random.default_rng(8)
+    x = rng.standard_normal(shape)
+    n = x.size
+    for inv, t, sc in ((False, 2, 2 * np.sqrt(2 * n)), (True, 3, np.sqrt(2 * n))):
+        f = O.dct3d(x, inverse=inv, fft=True)
+        np.testing.assert_allclose(f, scipy.fft.dctn(x, type=t, norm=None) / sc, rtol=1e-12, atol=1e-12)
+        np.testing.assert_allclose(f, O.dct3d(x, inverse=inv, fft=False), rtol=1e-11, atol=1e-12)
+
+
+def test_fft_oracle_reproduces_dense_oracle_goldens():
+    """The two-room golden traces were written by the dense-sum oracle; the FFT
+    path reproduces them to rounding (FP64) and to FP32 rounding (FP32)."""
+    import os
+    gdir = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    sc = S.two_rooms()
+    for double, tol in ((True, 1e-11), (False, 1e-5)):
+        ref = np.load(os.path.join(gdir, "two_rooms_%s.npy" % ("fp64" if double else "fp32")))
+        got = O.from_scene(sc, double=double, fft=True).step(sc.steps)
+        assert np.linalg.norm(got - ref) / np.linalg.norm(ref) < tol
