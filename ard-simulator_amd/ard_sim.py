@@ -30,7 +30,7 @@ EXPORTED = [
     "ard_set_batched_tiling",
     "ard_adjoint", "ard_batch_create", "ard_batch_step", "ard_batch_fetch_recorders", "ard_batch_context",
     "ard_batch_room_partition", "ard_batch_destroy", "ard_set_recorder_cubes", "ard_fetch_recorder_cubes",
-    "ard_render_view",
+    "ard_render_view", "ard_selftest_fold",
 ]
 PHASES = ["sources", "pml", "fold", "face_fwd1", "face_fwd2", "modal", "unfold", "face_inv1", "face_inv2",
           "interface", "record", "dct_fwd", "dct_inv"]
@@ -86,6 +86,7 @@ def lib():
         L.ard_kernel_times.argtypes = [vp, fp, ip, i]
         L.ard_selftest_transform.argtypes = [i, i, i, i, i, i, fp, fp, fp]
         L.ard_set_transform_variant.argtypes = [i]
+        L.ard_selftest_fold.argtypes = [i, i, i, i, i, fp, fp]
         L.ard_batched_config.argtypes = [vp, ip]
         L.ard_set_batched_tiling.argtypes = [vp, ctypes.c_int, ctypes.c_int]
         L.ard_adjoint.argtypes = [vp, i, fp, fp, fp, i]
@@ -110,6 +111,16 @@ def _check(r):
 
 def _fp(a):
     return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+def selftest_fold(X, inverse=False, b_layout=0, device=0):
+    """Unnormalised REDFT10 (or REDFT01 if inverse) along the length-L axis of X
+    through the folded matrix-core kernel: X is [L][N] (b_layout 0) or [N][L]."""
+    X = np.ascontiguousarray(X, dtype=np.float32)
+    L, N = X.shape if b_layout == 0 else X.shape[::-1]
+    Y = np.zeros_like(X)
+    _check(lib().ard_selftest_fold(device, int(inverse), b_layout, L, N, _fp(X), _fp(Y)))
+    return Y
 
 
 def selftest_transform(A, B, variant=1, b_layout=0, device=0):

@@ -954,4 +954,56 @@ void destroy_batched(Ctx& c) {
   c.bs = nullptr;
 }
 
+// Self-test of the folded transform kernel (ard_selftest_fold): one
+// transform of length L over N columns, built exactly as the batched form
+// builds its face passes (fold_tables, a folded GemmDesc, fold_tile_table), in
+// a temporary context of its own.
+int fold_selftest(int device, int inverse, int b_layout, int L, int N, const float* X, float* Y) {
+  hipError_t e = hipSetDevice(device);
+  if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+  Ctx c;
+  c.device = device;
+  if ((e = hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking)) != hipSuccess)
+    return hip_fail(e, "hipStreamCreate");
+  int r;
+  const float *ff, *fi;
+  if ((r = fold_tables(c, L, &ff, &fi))) return r;
+  const size_t n = size_t(L) * N;
+  float *dX, *dY;
+  // the kernel's 16-byte B loads may end up to 3 floats past the array
+  if ((r = c.dalloc((void**)&dX, (n + 16) * 4)) || (r = c.dalloc((void**)&dY, n * 4))) return r;
+  if ((r = c.dzero(dX, (n + 16) * 4)) || (r = c.upload(dX, X, n * 4))) return r;
+  GemmDesc g{};
+  g.B = dX; g.C = dY;
+  g.N = N; g.N1 = N;
+  if (b_layout == 0) { g.bK = N; g.b1 = 1; g.cM = N; g.c1 = 1; }   // X, Y [L][N]
+  else { g.bK = 1; g.b1 = L; g.cM = 1; g.c1 = L; }                 // X, Y [N][L]
+  g.alpha = 1.f;
+  g.b_elems = unsigned(n);
+  g.L = L;
+  g.K = (L + 1) / 2;
+  g.Kp = fold_kp(L);
+  g.Ap = inverse ? fold_rows_inv(L) : fold_rows_fwd(L);
+  if (!inverse) {
+    g.A = ff;
+    g.Mh = fold_even_rows(L);
+    g.M = g.Mh + L / 2;
+  } else {
+    g.A = fi;
+    g.M = int(64 * ceil_div((L + 1) / 2, 32));
+  }
+  g.tiles_n = int(ceil_div(N, 64));
+  std::vector<GemmDesc> v{g};
+  GemmDesc* dd;
+  int2* dt;
+  if ((r = upload_vec(c, (void**)&dd, v.data(), sizeof(GemmDesc))) || (r = fold_tile_table(c, v, &dt))) return r;
+  const int tiles = int(ceil_div(g.M, 64)) * g.tiles_n;
+  if (launch_gemm_fold(dd, dt, tiles, b_layout == 1, inverse ? 2 : 1, c.stream))
+    return set_error(ARD_ERR_HIP, "folded transform launch");
+  if ((e = hipMemcpyAsync(Y, dY, n * 4, hipMemcpyDeviceToHost, c.stream)) != hipSuccess ||
+      (e = hipStreamSynchronize(c.stream)) != hipSuccess)
+    return hip_fail(e, "folded transform result");
+  return ARD_OK;
+}
+
 }  // namespace ard

@@ -141,6 +141,7 @@ void destroy_batched(Ctx& c);
 int batched_config(Ctx& c, int* out4);
 int batched_set_tiling(Ctx& c, int rpw, int kzc);
 int batched_modes(Ctx& c, int part, const float* M[2], int* wp);
+int fold_selftest(int device, int inverse, int b_layout, int L, int N, const float* X, float* Y);
 
 // recorder cubes (ard_output.hip)
 int check_cubes(const Ctx& c);  // host-side range check

@@ -481,7 +481,9 @@ __global__ __launch_bounds__(256) void k_gemm_mfma_fold(const GemmDesc* __restri
     if (B_K_CONTIG) {
       const int k = k0 + 4 * (tid & 3);
       if (FOLD == 1) {
-        // x[k .. k+3] and x[L-4-k .. L-1-k] (reversed); negative offsets read 0
+        // x[k .. k+3] and x[L-4-k .. L-1-k] (reversed); a 16-byte load at a
+        // negative offset reads 0 as a whole, which loses no needed element
+        // (k < ceil(L/2)) when L >= 6, the smallest partition side
         const v4f d = __builtin_bit_cast(v4f, __builtin_amdgcn_raw_buffer_load_b128(rB, int(unsigned(bcol[0] + k) * 4u), 0, 0));
         const v4f m = __builtin_bit_cast(v4f, __builtin_amdgcn_raw_buffer_load_b128(rB, int(unsigned(bcol[0] + L - 4 - k) * 4u), 0, 0));
         rb[0][0] = d.x; rb[0][1] = d.y; rb[0][2] = d.z; rb[0][3] = d.w;

@@ -188,6 +188,14 @@ int ard_kernel_times(ard_ctx* ctx, float* avg_ms, int* launches, int nphase);
  * allocates and frees its own buffers. */
 int ard_selftest_transform(int device, int variant, int b_layout, int M, int N, int K, const float* A, const float* B,
                            float* C);
+/* Self-test of the even/odd folded transform kernel (variant 3; no
+ * counterpart in the reference): one unnormalised transform of length L along
+ * the first axis of X, FFTW's REDFT10 (inverse = 0) or REDFT01 (inverse = 1)
+ * (ARD/dct_volume.cpp:17,19), for N columns. b_layout 0: X and Y are [L][N]
+ * row-major; 1: [N][L]. Built exactly as the batched form builds its face
+ * passes. L >= 6 (the smallest partition side, which the kernel assumes).
+ * Synchronous; allocates and frees its own buffers. */
+int ard_selftest_fold(int device, int inverse, int b_layout, int L, int N, const float* X, float* Y);
 /* Selects the dense transform kernels used by contexts finalized afterwards
  * (0 vector FMA, 1 matrix cores FP32, 2 matrix cores BF16 split, 3 matrix
  * cores FP32 with the even/odd folding c(k, n-1-j) = (-1)^k c(k, j), half the

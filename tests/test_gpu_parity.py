@@ -190,3 +190,40 @@ def test_hall_batched_matches_plain():
     print("hall batched vs plain: trace rel-L2 %.3e, field rel-L2 %.3e" % (e, ef))
     assert np.abs(tr[A.FORM_PLAIN]).max() > 0
     assert e <= TOL and ef <= TOL
+
+
+def odd_scene():
+    """Three boxes with odd sides along every axis (17, 13, 11, 9, 15 cells),
+    partial faces and PML runs: the odd-length paths of the transforms."""
+    sc = S.two_rooms()
+    sc.boxes = [(0, 0, 0, 17, 13, 15), (17, 0, 0, 11, 19, 15), (0, 13, 0, 17, 9, 15)]
+    sc.sources = [S.Source(5, 1, 4), S.Source(21, 9, 7)]
+    sc.recorders = [(23, 9, 7)]
+    sc.recorder_mode = S.REC_LOCAL
+    return sc
+
+
+@pytest.mark.parametrize("form,variant", [(A.FORM_PLAIN, 1), (A.FORM_BATCHED, 1), (A.FORM_BATCHED, 3),
+                                          (A.FORM_BATCHED, 0)])
+def test_odd_sides_all_fields_match_oracle(form, variant):
+    sc = odd_scene()
+    print("case odd_scene fields form", form, "variant", variant, flush=True)
+    A.set_transform_variant(variant)
+    try:
+        sim = A.Simulation(sc, form=form)
+    finally:
+        A.set_transform_variant(1)
+    if form == A.FORM_BATCHED:
+        assert sim.batched_config()[2] == variant
+    o = O.from_scene(sc)
+    assert sim.partitions() == o.partitions() and sim.boundaries() == o.boundaries()
+    assert any(p[6] % 2 and p[7] % 2 and p[8] % 2 for p in o.partitions() if p[0] == 0)
+    n = 60
+    sim.update(n)
+    ref_tr = o.step(n)
+    assert rel_l2(sim.recorders(), ref_tr) < TOL
+    for i, info in enumerate(sim.partitions()):
+        got = sim.pressure_field(i)
+        ref = o.field(i, 0)
+        err = np.abs(got - ref).max() / np.abs(ref).max()
+        assert err < TOL, (i, info, err)

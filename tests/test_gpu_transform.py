@@ -23,3 +23,36 @@ def test_transform_kernel_matches_numpy(variant, b_layout, shape):
     err = np.abs(c - ref).max() / np.abs(ref).max()
     # BF16 split: products to about 2^-16 relative (three BF16 products, FP32 sums)
     assert err < (1e-5 if variant < 2 else 1e-4), err
+
+
+# L and its folded K = ceil(L/2): odd lengths (the halved middle column, the
+# mirrored 16-byte loads, the skipped middle write), K tails of at most 8
+# columns after whole 16-column chunks (K = 3, 4, 17, 33, 136) and of more than 8
+# (K = 9, 100), even lengths (6, the smallest partition side, and the hall's 40
+# and 200)
+FOLD_L = [6, 7, 9, 11, 17, 33, 40, 65, 199, 200, 271]
+
+
+@pytest.mark.parametrize("inverse", [False, True])
+@pytest.mark.parametrize("b_layout", [0, 1])
+@pytest.mark.parametrize("L", FOLD_L)
+@pytest.mark.parametrize("N", [1, 37, 130])
+def test_folded_transform_matches_numpy(inverse, b_layout, L, N):
+    """The even/odd folded matrix-core transform (variant 3, the kernel of the
+    hall's face passes) against scipy's REDFT10 / REDFT01 (FFTW's definitions)."""
+    import scipy.fft
+
+    rng = np.random.default_rng(L * 131 + N * 7 + 2 * inverse + b_layout)
+    x = rng.standard_normal((L, N)).astype(np.float32)
+    ref = scipy.fft.dct(x.astype(np.float64), type=3 if inverse else 2, axis=0, norm=None)
+    xin = x if b_layout == 0 else np.ascontiguousarray(x.T)
+    y = A.selftest_fold(xin, inverse=inverse, b_layout=b_layout)
+    if b_layout == 1:
+        y = y.T
+    err = np.abs(y - ref).max() / np.abs(ref).max()
+    assert err < 1e-5, err
+
+
+def test_folded_transform_refuses_lengths_below_six():
+    with pytest.raises(A.ArdError, match="error -1"):
+        A.selftest_fold(np.ones((5, 4), np.float32))
