@@ -30,7 +30,7 @@ EXPORTED = [
     "ard_set_batched_tiling",
     "ard_adjoint", "ard_batch_create", "ard_batch_step", "ard_batch_fetch_recorders", "ard_batch_context",
     "ard_batch_room_partition", "ard_batch_destroy", "ard_set_recorder_cubes", "ard_fetch_recorder_cubes",
-    "ard_render_view", "ard_selftest_fold",
+    "ard_render_view", "ard_selftest_fold", "ard_dct3d",
 ]
 PHASES = ["sources", "pml", "fold", "face_fwd1", "face_fwd2", "modal", "unfold", "face_inv1", "face_inv2",
           "interface", "record", "dct_fwd", "dct_inv"]
@@ -87,6 +87,7 @@ def lib():
         L.ard_selftest_transform.argtypes = [i, i, i, i, i, i, fp, fp, fp]
         L.ard_set_transform_variant.argtypes = [i]
         L.ard_selftest_fold.argtypes = [i, i, i, i, i, fp, fp]
+        L.ard_dct3d.argtypes = [i, i, i, i, i, vp, vp, f, vp]
         L.ard_batched_config.argtypes = [vp, ip]
         L.ard_set_batched_tiling.argtypes = [vp, ctypes.c_int, ctypes.c_int]
         L.ard_adjoint.argtypes = [vp, i, fp, fp, fp, i]
@@ -111,6 +112,17 @@ def _check(r):
 
 def _fp(a):
     return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+def dct3d(inp, out, dct_type=2, scale=1.0, stream=None, device=0):
+    """out = scale * DCT-II (dct_type 2) or DCT-III (3) of the device volume inp
+    ([d][h][w] float32, contiguous; torch tensors or raw device addresses with
+    shape given by `out`'s tensor) through ard_dct3d. stream: a HIP stream handle
+    (int) or None for the null stream."""
+    d, h, w = out.shape
+    _check(lib().ard_dct3d(device, dct_type, w, h, d, ctypes.c_void_p(inp.data_ptr()),
+                           ctypes.c_void_p(out.data_ptr()), float(scale), ctypes.c_void_p(stream or 0)))
+    return out
 
 
 def selftest_fold(X, inverse=False, b_layout=0, device=0):

@@ -154,6 +154,18 @@ int ard_get_pressure(ard_ctx* ctx, int part, int x, int y, int z, float* out);
  * persist across steps; PML forces are cleared by the next update (ARD/pml_partition.cpp:199). */
 int ard_set_force(ard_ctx* ctx, int part, int x, int y, int z, float f);
 
+/* Scaled 3D DCT of a device volume: VkFFT_DCT(vkGPU, type, w, h, d, in, out).execute()
+ * (ARD/utils_VkFFT.h:38-60, ARD/utils_VkFFT.cpp:623-691), which computes FFTW's unnormalised
+ * REDFT10 (type 2) or REDFT01 (type 3) along x, y and z (ARD/dct_volume.cpp:17,19), followed by
+ * DctVolume's scaling: out = scale * transform(in). scale = 1/(2 sqrt(2 w h d)) (type 2) and
+ * 1/sqrt(2 w h d) (type 3) give DctVolume::ExecuteDct / ExecuteIdct (ARD/dct_volume.cpp:38-68);
+ * scale = 1 gives VkFFT_DCT's output. in and out are device pointers to w*h*d floats, z-major
+ * with x fastest, and must not overlap (DctVolume transforms out of place). stream is a
+ * hipStream_t (NULL: the null stream). Allocates a workspace for the call (the intermediate
+ * volume, the cosine tables, the pass descriptors: dct3d workspace bytes, about 4 w h d + 4 (w^2
+ * + h^2 + d^2)) and returns when the transform is done. */
+int ard_dct3d(int device, int type, int w, int h, int d, const float* in, float* out, float scale, void* stream);
+
 /* Blocks until all work queued on the context's stream is done. */
 int ard_synchronize(ard_ctx* ctx);
 

@@ -56,3 +56,34 @@ def test_folded_transform_matches_numpy(inverse, b_layout, L, N):
 def test_folded_transform_refuses_lengths_below_six():
     with pytest.raises(A.ArdError, match="error -1"):
         A.selftest_fold(np.ones((5, 4), np.float32))
+
+
+@pytest.mark.parametrize("shape", [(40, 60, 200), (7, 13, 9), (1, 1, 1), (30, 30, 30), (64, 3, 129)])
+@pytest.mark.parametrize("dct_type", [2, 3])
+def test_dct3d_matches_scipy(shape, dct_type):
+    """ard_dct3d (the VkFFT_DCT seam plus DctVolume's scaling) on device
+    volumes against scipy's dctn, with the reference's scales."""
+    import scipy.fft
+    import torch
+
+    rng = np.random.default_rng(sum(shape) + dct_type)
+    x = rng.standard_normal(shape).astype(np.float32)
+    n = x.size
+    scale = 1 / (2 * np.sqrt(2 * n)) if dct_type == 2 else 1 / np.sqrt(2 * n)
+    ref = scipy.fft.dctn(x.astype(np.float64), type=dct_type, norm=None) * scale
+    xi = torch.tensor(x, device="cuda")
+    out = torch.empty_like(xi)
+    A.dct3d(xi, out, dct_type=dct_type, scale=scale)
+    got = out.cpu().numpy()
+    assert np.abs(got - ref).max() / np.abs(ref).max() < 1e-5
+    assert np.array_equal(xi.cpu().numpy(), x)  # input untouched
+
+
+def test_dct3d_refuses_overlap_and_bad_type():
+    import torch
+
+    x = torch.zeros((4, 5, 6), device="cuda")
+    with pytest.raises(A.ArdError, match="error -1"):
+        A.dct3d(x, x)
+    with pytest.raises(A.ArdError, match="error -1"):
+        A.dct3d(x, torch.empty_like(x), dct_type=1)
