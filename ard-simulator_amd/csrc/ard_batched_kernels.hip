@@ -184,8 +184,9 @@ __device__ __forceinline__ float bload_nt(brsrc r, unsigned voff, unsigned soff)
 // registers. The chunk goes in batches of KB planes:
 //   planes   each wave: new modes of its rows -> global, LDS mode tile, its y sums -> LDS
 //   barrier
-//   reduce   x sums (and recorder sums) of the batch from the mode tile; the
-//            four waves' y sums added; the next batch's per-row and per-plane
+//   reduce   x sums (and recorder sums) of the batch from the mode tile; y sums
+//            of the batch from the mode tile (one (plane, kx) per thread, over
+//            the block's rows); the next batch's per-row and per-plane
 //            factors (x-face spectra, point-force factors, z tables) staged in LDS
 //   barrier
 // so the plane loop reads no scalar memory and no LDS atomics (float atomics
@@ -201,7 +202,7 @@ __global__ __launch_bounds__(256) void k_modal(const ModalDesc* __restrict__ des
   constexpr int NRC = ARD_MAX_REC_PER_PART;
   static_assert(NPT == 4 && NRC == 4, "point tables hold 4 + 4 factors");
   __shared__ __attribute__((aligned(16))) float s_m[KB][TY][LD];     // new modes of a batch
-  __shared__ __attribute__((aligned(16))) float s_ty[KB][6][4][64];  // y sums: [plane][ky parity * 3 + j][wave][lane]
+  __shared__ __attribute__((aligned(16))) float s_ry[TY][4];         // y reduction weights of the tile's rows
   __shared__ __attribute__((aligned(16))) float s_rx[64 * 3];        // x reduction weights of the tile's kx
   __shared__ __attribute__((aligned(16))) float s_gx[KB][TY][8];     // x-face spectra of the batch's rows: even 0..2, odd 4..6
   __shared__ __attribute__((aligned(16))) float s_zt[KB][8];         // z factors of the batch's planes
@@ -253,9 +254,13 @@ __global__ __launch_bounds__(256) void k_modal(const ModalDesc* __restrict__ des
 #pragma unroll
     for (int q = 0; q < NRC; ++q) s_rfx[q][lane] = q < nrec && xok ? P.rec[q].fx[kxc] : 0.f;
   }
+  if (tid < TY * 4) {  // rows past H carry zero modes, any weight will do
+    const int row = tid >> 2, j = tid & 3;
+    s_ry[row][j] = j < 3 ? ARD_CC(P.yt)[min(kyb + row, H - 1) * 8 + 3 + j] : 0.f;
+  }
   const float ux = ARD_CG(P.wx2)[kxc];
   // per-row factors of this wave's rows (wave-uniform)
-  float ey[RPW][3], ry[RPW][3], uy[RPW];
+  float ey[RPW][3], uy[RPW];
   bool rok[RPW];
   int rof[RPW], rom[RPW];  // row offsets ky * W of the face tables and ky * Wp of the modes (clamped)
 #pragma unroll
@@ -267,10 +272,7 @@ __global__ __launch_bounds__(256) void k_modal(const ModalDesc* __restrict__ des
     const cfloat* yt = ARD_CC(P.yt) + kyc * 8;
     uy[r] = yt[6];
 #pragma unroll
-    for (int j = 0; j < 3; ++j) {
-      ey[r][j] = yt[j];
-      ry[r][j] = yt[3 + j];
-    }
+    for (int j = 0; j < 3; ++j) ey[r][j] = yt[j];
   }
   // z-face force spectrum (both kz parities) and z sums of this thread's rows
   float gz[RPW][6], tz[RPW][6];
@@ -350,7 +352,6 @@ __global__ __launch_bounds__(256) void k_modal(const ModalDesc* __restrict__ des
     const float4 zb = *reinterpret_cast<const float4*>(&s_zt[kl][4]);  // Rz[kz][1..2], u_z, 0
     const float uxz = ux + zb.z;
     const unsigned po = unsigned(kz * HWp);
-    float ty[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
 #pragma unroll
     for (int r = 0; r < RPW; ++r) {
       const int PY = r & 1;
@@ -388,12 +389,7 @@ __global__ __launch_bounds__(256) void k_modal(const ModalDesc* __restrict__ des
       tz[r][PZ * 3 + 0] = fmaf(za.w, mn, tz[r][PZ * 3 + 0]);
       tz[r][PZ * 3 + 1] = fmaf(zb.x, mn, tz[r][PZ * 3 + 1]);
       tz[r][PZ * 3 + 2] = fmaf(zb.y, mn, tz[r][PZ * 3 + 2]);
-      ty[PY * 3 + 0] = fmaf(ry[r][0], mn, ty[PY * 3 + 0]);
-      ty[PY * 3 + 1] = fmaf(ry[r][1], mn, ty[PY * 3 + 1]);
-      ty[PY * 3 + 2] = fmaf(ry[r][2], mn, ty[PY * 3 + 2]);
     }
-#pragma unroll
-    for (int q = 0; q < 6; ++q) s_ty[kl][q][wave][lane] = ty[q];
   };
 
   // x and y sums of a finished batch (planes kb .. kb+n-1):
@@ -461,23 +457,24 @@ __global__ __launch_bounds__(256) void k_modal(const ModalDesc* __restrict__ des
       }
     }
   };
+  // y sums of the batch (after the barrier: every wave's rows are in the
+  // tile): thread (plane tid / 64, kx lane) sums the block's rows, even rows
+  // into q < 3 and odd rows into q >= 3; rows past H hold zeros.
   auto ysum = [&](const int kb, const int n) {
-    for (int e = tid; e < KB * 6 * 16; e += 256) {
-      const int x4 = e & 15, q = (e >> 4) % 6, pl = e / 96;
-      if (pl >= n) continue;
-      float4 v = *reinterpret_cast<const float4*>(&s_ty[pl][q][0][x4 * 4]);
+    const int pl = tid >> 6;  // KB == 4 planes, one per wave
+    if (pl >= n || !xok) return;
+    float e0 = 0.f, e1 = 0.f, e2 = 0.f, o0 = 0.f, o1 = 0.f, o2 = 0.f;
 #pragma unroll
-      for (int w = 1; w < 4; ++w) {
-        const float4 t = *reinterpret_cast<const float4*>(&s_ty[pl][q][w][x4 * 4]);
-        v.x += t.x; v.y += t.y; v.z += t.z; v.w += t.w;
-      }
-      const int kxx = bx * 64 + x4 * 4;
-      gfloat* dst = Typ + q * DW + (kb + pl) * W + kxx;
-      if (kxx + 0 < W) dst[0] = v.x;
-      if (kxx + 1 < W) dst[1] = v.y;
-      if (kxx + 2 < W) dst[2] = v.z;
-      if (kxx + 3 < W) dst[3] = v.w;
+    for (int row = 0; row < TY; row += 2) {
+      const float me = s_m[pl][row][lane], mo = s_m[pl][row + 1][lane];
+      const float4 we = *reinterpret_cast<const float4*>(&s_ry[row][0]);
+      const float4 wo = *reinterpret_cast<const float4*>(&s_ry[row + 1][0]);
+      e0 = fmaf(we.x, me, e0); e1 = fmaf(we.y, me, e1); e2 = fmaf(we.z, me, e2);
+      o0 = fmaf(wo.x, mo, o0); o1 = fmaf(wo.y, mo, o1); o2 = fmaf(wo.z, mo, o2);
     }
+    gfloat* dst = Typ + (kb + pl) * W + kx;
+    dst[0 * DW] = e0; dst[1 * DW] = e1; dst[2 * DW] = e2;
+    dst[3 * DW] = o0; dst[4 * DW] = o1; dst[5 * DW] = o2;
   };
 
   PlaneIn cur, nxt;
