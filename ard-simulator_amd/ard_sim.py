@@ -30,7 +30,7 @@ EXPORTED = [
     "ard_set_batched_tiling",
     "ard_adjoint", "ard_batch_create", "ard_batch_step", "ard_batch_fetch_recorders", "ard_batch_context",
     "ard_batch_room_partition", "ard_batch_destroy", "ard_set_recorder_cubes", "ard_fetch_recorder_cubes",
-    "ard_render_view", "ard_selftest_fold", "ard_dct3d",
+    "ard_render_view", "ard_selftest_fold", "ard_dct3d", "ard_selftest_fft",
 ]
 PHASES = ["sources", "pml", "fold", "face_fwd1", "face_fwd2", "modal", "unfold", "face_inv1", "face_inv2",
           "interface", "record", "dct_fwd", "dct_inv"]
@@ -87,6 +87,7 @@ def lib():
         L.ard_selftest_transform.argtypes = [i, i, i, i, i, i, fp, fp, fp]
         L.ard_set_transform_variant.argtypes = [i]
         L.ard_selftest_fold.argtypes = [i, i, i, i, i, fp, fp]
+        L.ard_selftest_fft.argtypes = [i, i, i, i, i, fp, fp]
         L.ard_dct3d.argtypes = [i, i, i, i, i, vp, vp, f, vp]
         L.ard_batched_config.argtypes = [vp, ip]
         L.ard_set_batched_tiling.argtypes = [vp, ctypes.c_int, ctypes.c_int]
@@ -125,13 +126,14 @@ def dct3d(inp, out, dct_type=2, scale=1.0, stream=None, device=0):
     return out
 
 
-def selftest_fold(X, inverse=False, b_layout=0, device=0):
+def selftest_fold(X, inverse=False, b_layout=0, device=0, entry="ard_selftest_fold"):
     """Unnormalised REDFT10 (or REDFT01 if inverse) along the length-L axis of X
-    through the folded matrix-core kernel: X is [L][N] (b_layout 0) or [N][L]."""
+    through the folded matrix-core kernel (or, with entry="ard_selftest_fft",
+    the FFT-style one): X is [L][N] (b_layout 0) or [N][L]."""
     X = np.ascontiguousarray(X, dtype=np.float32)
     L, N = X.shape if b_layout == 0 else X.shape[::-1]
     Y = np.zeros_like(X)
-    _check(lib().ard_selftest_fold(device, int(inverse), b_layout, L, N, _fp(X), _fp(Y)))
+    _check(getattr(lib(), entry)(device, int(inverse), b_layout, L, N, _fp(X), _fp(Y)))
     return Y
 
 

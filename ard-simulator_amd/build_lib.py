@@ -13,7 +13,7 @@ OBJ = os.path.join(HERE, "_build")
 
 HIP_SOURCES = ["ard_kernels.hip", "ard_batched_kernels.hip", "ard_batched.hip", "ard_ctx.cpp", "ard_plain.cpp",
                "ard_adjoint_kernels.hip", "ard_adjoint.cpp", "ard_batch.cpp", "ard_output.hip",
-               "ard_view.hip", "ard_dct3d.hip"]
+               "ard_view.hip", "ard_dct3d.hip", "ard_fft.hip"]
 CXX_SOURCES = ["ard_topology.cpp"]
 HOST_SOURCES = ["host/ard_host.cpp"]
 

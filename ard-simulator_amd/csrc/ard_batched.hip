@@ -84,6 +84,10 @@ struct BatchedState {
   GemmDesc* d_gemm_fold[4] = {};  // the same passes even/odd folded (transform variant 3)
   int2* d_fold_tiles[4] = {};      // their tile tables (fold_tile_table)
   int gemm_fold_tiles[4] = {};
+  GemmDesc* d_gemm_fft[4] = {};   // the same passes FFT-style (variant 4), when every length factors
+  int gemm_fft_tiles[4] = {};
+  size_t fft_lds[4] = {};
+  bool has_fft = false;
   struct ModalGroup {
     int rpw;
     bool pts;
@@ -253,6 +257,17 @@ int fold_tile_table(Ctx& c, const std::vector<GemmDesc>& v, int2** d) {
     --left;
   }
   return upload_vec(c, (void**)d, t.data(), t.size() * sizeof(int2));
+}
+
+// One face-transform launch k (0, 1 forward; 2, 3 inverse) with transform
+// kernel v: 3 the even/odd folded matrix-core passes, 4 the FFT-style passes,
+// else the dense passes of launch_gemm (0, 1, 2).
+int launch_face(const BatchedState& B, int k, int v, hipStream_t st) {
+  if (v == 3)
+    return launch_gemm_fold(B.d_gemm_fold[k], B.d_fold_tiles[k], B.gemm_fold_tiles[k], k % 2 == 0, k < 2 ? 1 : 2, st);
+  if (v == 4)
+    return launch_fft(B.d_gemm_fft[k], B.ngemm[k], B.gemm_fft_tiles[k], k % 2 == 0, k % 2 == 0, 0, B.fft_lds[k], st);
+  return launch_gemm(B.d_gemm[k], B.ngemm[k], B.gemm_tiles[k], k % 2 == 0, k % 2 == 0, 0, st, v);
 }
 
 int upload_gemms(Ctx& c, std::vector<GemmDesc>& v, GemmDesc** d, int* tiles) {
@@ -627,6 +642,27 @@ int finalize_batched(Ctx& c) {
     if ((r = upload_gemms(c, gf, &B.d_gemm_fold[k], &B.gemm_fold_tiles[k]))) return r;
     if ((r = fold_tile_table(c, gf, &B.d_fold_tiles[k]))) return r;
   }
+  // the FFT-style twins of the dense passes (variant 4): only if every face
+  // axis length factors into radices up to 13
+  B.has_fft = true;
+  {
+    std::vector<GemmDesc> ff[4];
+    for (int k = 0; k < 4 && B.has_fft; ++k) {
+      int t = 0;
+      for (GemmDesc g : gm[k]) {
+        if (fft_setup(c, g, k >= 2) != ARD_OK) { B.has_fft = false; break; }
+        g.tile0 = t;
+        t += fft_tiles(g);
+        B.fft_lds[k] = std::max(B.fft_lds[k], fft_lds_bytes(g));
+        ff[k].push_back(g);
+      }
+      B.gemm_fft_tiles[k] = t;
+    }
+    for (int k = 0; k < 4 && B.has_fft; ++k)
+      if (!ff[k].empty() && (r = upload_vec(c, (void**)&B.d_gemm_fft[k], ff[k].data(), ff[k].size() * sizeof(GemmDesc))))
+        return r;
+  }
+  if (B.gemm_variant == 4 && !B.has_fft) B.gemm_variant = 1;
   {
     std::vector<long long> st;
     long long t = 0;
@@ -720,11 +756,7 @@ int enqueue_step(Ctx& c, int par, PhaseTimer* T) {
     return launch_pml(B.d_pml, B.npml, B.pml_blocks, par, c.prm.pml_layers, c.prm.h, c.prm.dt, c.prm.c0, sp);
   });
   if (fork) r |= hipEventRecord(B.ev_join, B.side) != hipSuccess;
-  auto face = [&](int k) {
-    if (B.gemm_variant == 3)
-      return launch_gemm_fold(B.d_gemm_fold[k], B.d_fold_tiles[k], B.gemm_fold_tiles[k], k % 2 == 0, k < 2 ? 1 : 2, st);
-    return launch_gemm(B.d_gemm[k], B.ngemm[k], B.gemm_tiles[k], k % 2 == 0, k % 2 == 0, 0, st, B.gemm_variant);
-  };
+  auto face = [&](int k) { return launch_face(B, k, B.gemm_variant, st); };
   phase(ARD_PH_FACE_FWD1, [&] { return face(0); });
   phase(ARD_PH_FACE_FWD2, [&] { return face(1); });
   for (auto& g : B.modal)
@@ -807,16 +839,11 @@ int autotune(Ctx& c) {
   // the FP32 transform kernels compete on speed; BF16 split precision, when
   // requested, is kept (a precision choice, not a speed one)
   float bestg = 1e30f;
-  for (int v : {0, 1, 3}) {
+  for (int v : {0, 1, 3, 4}) {
     if (B.gemm_variant == 2) break;
+    if (v == 4 && !B.has_fft) continue;
     const float ms = time_ms([&] {
-      for (int k = 0; k < 4; ++k) {
-        if (v == 3)
-          launch_gemm_fold(B.d_gemm_fold[k], B.d_fold_tiles[k], B.gemm_fold_tiles[k], k % 2 == 0, k < 2 ? 1 : 2,
-                           c.stream);
-        else
-          launch_gemm(B.d_gemm[k], B.ngemm[k], B.gemm_tiles[k], k % 2 == 0, k % 2 == 0, 0, c.stream, v);
-      }
+      for (int k = 0; k < 4; ++k) launch_face(B, k, v, c.stream);
     }, 3);
     tried++;
     if (ms < bestg) { bestg = ms; B.gemm_variant = v; }

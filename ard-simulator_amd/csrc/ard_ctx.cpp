@@ -503,6 +503,11 @@ int ard_set_force(ard_ctx* ctx, int part, int x, int y, int z, float f) {
 int ard_batched_config(ard_ctx* ctx, int* out4) {
   Ctx* c = reinterpret_cast<Ctx*>(ctx);
   if (!c || !out4) return fail(ARD_ERR_INVALID, "null argument");
+  if (c->finalized && c->prm.form == ARD_FORM_PLAIN) {  // the plain form: its transform kernel only
+    out4[0] = out4[1] = out4[3] = 0;
+    out4[2] = c->plain_variant;
+    return ARD_OK;
+  }
   return batched_config(*c, out4);
 }
 
@@ -513,7 +518,7 @@ int ard_set_batched_tiling(ard_ctx* ctx, int rows_per_wave, int kz_per_block) {
 }
 
 int ard_set_transform_variant(int variant) {
-  if (variant < 0 || variant > 3) return fail(ARD_ERR_INVALID, "unknown transform variant");
+  if (variant < 0 || variant > 4) return fail(ARD_ERR_INVALID, "unknown transform variant");
   g_gemm_variant = variant;
   return ARD_OK;
 }
@@ -557,6 +562,12 @@ int ard_selftest_transform(int device, int variant, int b_layout, int M, int N, 
   }
   hipFree(dA); hipFree(dB); hipFree(dC); hipFree(dD);
   return r;
+}
+
+int ard_selftest_fft(int device, int inverse, int b_layout, int L, int N, const float* X, float* Y) {
+  if (!X || !Y || L < 2 || N <= 0 || b_layout < 0 || b_layout > 1) return fail(ARD_ERR_INVALID, "bad arguments");
+  if ((long long)L * N >= (1LL << 30)) return fail(ARD_ERR_INVALID, "too large");
+  return fft_selftest(device, inverse != 0, b_layout, L, N, X, Y);
 }
 
 int ard_selftest_fold(int device, int inverse, int b_layout, int L, int N, const float* X, float* Y) {

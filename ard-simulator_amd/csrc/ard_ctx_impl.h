@@ -31,6 +31,7 @@ struct DctPlain {
   float* g = nullptr;
   int gemm0 = 0;  // first of its 12 GEMM problems (2 parities x 6 passes)
   int tiles[6] = {0, 0, 0, 0, 0, 0};
+  int fft_tiles[6] = {0, 0, 0, 0, 0, 0};  // the same passes as FFT-style transforms (variant 4)
 };
 
 // PML slab buffers (ARD/pml_partition.h:8-25); shared by both forms.
@@ -70,6 +71,7 @@ struct Ctx {
   std::vector<void*> allocs;
   std::map<int, std::pair<float*, float*>> tables;  // axis length -> (REDFT10, REDFT01) matrices
   std::map<int, std::pair<float*, float*>> fold_tables;  // axis length -> even/odd folded (forward, inverse)
+  std::map<int, float*> fft_plans;                       // axis length -> FFT plan table (variant 4)
 
   // step counter and recorder trace, device-resident so captured graphs replay correctly
   int* d_step = nullptr;
@@ -87,6 +89,9 @@ struct Ctx {
   float* tmp[2] = {nullptr, nullptr};
   std::vector<GemmDesc> h_gemm;
   GemmDesc* d_gemm = nullptr;
+  int plain_variant = 1;       // transform kernel of the plain form, fixed at finalize (1, 0, 2 or 4)
+  GemmDesc* d_fft = nullptr;   // FFT twins of h_gemm (variant 4)
+  size_t fft_lds = 0;          // dynamic LDS of the plain form's FFT passes
   std::vector<PmlDesc> h_pml;      // block0 = 0 each (one launch per slab)
   PmlDesc* d_pml = nullptr;
   std::vector<long long> pml_blocks;
@@ -142,6 +147,12 @@ int batched_config(Ctx& c, int* out4);
 int batched_set_tiling(Ctx& c, int rpw, int kzc);
 int batched_modes(Ctx& c, int part, const float* M[2], int* wp);
 int fold_selftest(int device, int inverse, int b_layout, int L, int N, const float* X, float* Y);
+// FFT-style transforms (ard_fft.hip): the FFT fields of a dense pass
+// descriptor (its plan cached in the context; ARD_ERR_UNSUPPORTED for a
+// length fft_factor refuses), its tile count, and the entry self-test
+int fft_setup(Ctx& c, GemmDesc& g, bool inverse);
+int fft_tiles(const GemmDesc& g);
+int fft_selftest(int device, int inverse, int b_layout, int L, int N, const float* X, float* Y);
 
 // recorder cubes (ard_output.hip)
 int check_cubes(const Ctx& c);  // host-side range check

@@ -74,6 +74,14 @@ struct GemmDesc {
   // rows (32 of E, 32 of O) per 32 outputs.
   int Ap, Kp;
   unsigned b_elems;
+  // FFT-style per-axis transform (k_dct_fft, transform variant 4): the plan of
+  // the length K (fft_plan_table), complex lines (pairs of lines) per tile,
+  // direction (0 REDFT10, 1 REDFT01) and the radices of the stages
+  const void* plan;
+  int tlc;
+  int inv;
+  int nrad;
+  unsigned char rad[12];
 };
 
 // A PML slab (ARD/pml_partition.h). Buffers are double-buffered by step parity:

@@ -2,6 +2,8 @@
 // success, negative on a launch failure.
 #pragma once
 
+#include <vector>
+
 #include "ard_device.h"
 
 namespace ard {
@@ -25,5 +27,16 @@ int launch_interface(const BndDesc* d_descs, int ndesc, const PartAccess* d_part
                      hipStream_t s);
 int launch_sources(const SourceDesc* d_src, int n, const int* d_step, float courant, hipStream_t s);
 int launch_record(const RecDesc* d_rec, int n, float* trace, int cap, int* d_step, hipStream_t s);
+
+// FFT-style per-axis passes (ard_fft.hip, transform variant 4) on the same
+// descriptors as launch_gemm: fft_factor splits L into radices (4, 2, 3, 5, 7,
+// 11, 13; false for other prime factors or L > 4096), fft_lines gives the
+// complex lines per tile, fft_plan_table the device table of a length.
+bool fft_factor(int L, std::vector<int>& rad);
+int fft_lines(int L);
+std::vector<float> fft_plan_table(int L, const std::vector<int>& rad);
+size_t fft_lds_bytes(const GemmDesc& g);
+int launch_fft(const GemmDesc* d_descs, int ndesc, int total_tiles, bool b_k_contig, bool c_m_contig, int ep,
+               size_t lds_bytes, hipStream_t s);
 
 }  // namespace ard

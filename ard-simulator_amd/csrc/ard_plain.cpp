@@ -332,6 +332,28 @@ int finalize_plain(Ctx& c) {
   }
   if ((r = c.dalloc((void**)&c.d_gemm, c.h_gemm.size() * sizeof(GemmDesc)))) return r;
   if ((r = c.upload(c.d_gemm, c.h_gemm.data(), c.h_gemm.size() * sizeof(GemmDesc)))) return r;
+  // transform kernel of the passes, fixed here: the FFT-style passes
+  // (variant 4) when every axis length factors into radices up to 13, else
+  // the dense passes (variant 3, the folded face passes of the batched form,
+  // means matrix cores here)
+  c.plain_variant = g_gemm_variant == 3 ? 1 : g_gemm_variant;
+  if (c.plain_variant == 4 && !c.h_gemm.empty()) {
+    std::vector<GemmDesc> f = c.h_gemm;
+    bool ok = true;
+    for (size_t j = 0; j < f.size() && ok; ++j) ok = fft_setup(c, f[j], j % 6 >= 3) == ARD_OK;
+    if (ok) {
+      c.fft_lds = 0;
+      for (const GemmDesc& g : f) c.fft_lds = std::max(c.fft_lds, fft_lds_bytes(g));
+      for (size_t i = 0; i < c.dct.size(); ++i)
+        if (c.dct[i].F)
+          for (int j = 0; j < 6; ++j) c.dct[i].fft_tiles[j] = fft_tiles(f[c.dct[i].gemm0 + j]);
+      if ((r = c.dalloc((void**)&c.d_fft, f.size() * sizeof(GemmDesc))) ||
+          (r = c.upload(c.d_fft, f.data(), f.size() * sizeof(GemmDesc))))
+        return r;
+    } else {
+      c.plain_variant = 1;
+    }
+  }
 
   // partition access table for the interface kernel
   c.h_parts.assign(np, PartAccess());
@@ -409,17 +431,22 @@ int step_plain(Ctx& c, int n) {
     for (size_t i = 0; i < c.dct.size(); ++i) {
       const DctPlain& D = c.dct[i];
       if (!D.F) continue;
-      const GemmDesc* g = c.d_gemm + D.gemm0 + par * 6;
+      const int off = D.gemm0 + par * 6;
+      const bool fft = c.plain_variant == 4;
+      // pass j: x (B and C contiguous along the axis), then y, then z (with
+      // the modal update in the forward z pass)
+      auto pass = [&](int j) {
+        const bool kc = j % 3 == 0;
+        const int ep = j == 2 ? 1 : 0;
+        return fft ? launch_fft(c.d_fft + off + j, 1, D.fft_tiles[j], kc, kc, ep, c.fft_lds, c.stream)
+                   : launch_gemm(c.d_gemm + off + j, 1, D.tiles[j], kc, kc, ep, c.stream, c.plain_variant);
+      };
       int r = 0;
       ev = T.begin(c.stream, ARD_PH_DCT_FWD);
-      r |= launch_gemm(g + 0, 1, D.tiles[0], true, true, 0, c.stream);
-      r |= launch_gemm(g + 1, 1, D.tiles[1], false, false, 0, c.stream);
-      r |= launch_gemm(g + 2, 1, D.tiles[2], false, false, 1, c.stream);
+      for (int j = 0; j < 3; ++j) r |= pass(j);
       T.end(c.stream, ev);
       ev = T.begin(c.stream, ARD_PH_DCT_INV);
-      r |= launch_gemm(g + 3, 1, D.tiles[3], true, true, 0, c.stream);
-      r |= launch_gemm(g + 4, 1, D.tiles[4], false, false, 0, c.stream);
-      r |= launch_gemm(g + 5, 1, D.tiles[5], false, false, 0, c.stream);
+      for (int j = 3; j < 6; ++j) r |= pass(j);
       T.end(c.stream, ev);
       if (r) return set_error(ARD_ERR_HIP, "DCT pass launch");
     }

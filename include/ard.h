@@ -208,16 +208,26 @@ int ard_selftest_transform(int device, int variant, int b_layout, int M, int N, 
  * passes. L >= 6 (the smallest partition side, which the kernel assumes).
  * Synchronous; allocates and frees its own buffers. */
 int ard_selftest_fold(int device, int inverse, int b_layout, int L, int N, const float* X, float* Y);
+/* Self-test of the FFT-style transform kernel (variant 4; no counterpart in the
+ * reference, whose FFTW plans it replaces, ARD/dct_volume.cpp:17,19): the same
+ * transform as ard_selftest_fold, for L >= 2 whose prime factors are at most 13
+ * (ARD_ERR_UNSUPPORTED otherwise). Synchronous; allocates and frees its own
+ * buffers. */
+int ard_selftest_fft(int device, int inverse, int b_layout, int L, int N, const float* X, float* Y);
 /* Selects the dense transform kernels used by contexts finalized afterwards
  * (0 vector FMA, 1 matrix cores FP32, 2 matrix cores BF16 split, 3 matrix
  * cores FP32 with the even/odd folding c(k, n-1-j) = (-1)^k c(k, j), half the
  * multiply-adds; 3 applies to the batched form's face passes, other passes
  * use 1). Default 1. The batched form's autotuner may choose among 0, 1 and 3
- * (same precision); it keeps 2. */
+ * (same precision); it keeps 2. 4: FFT-style passes (Makhoul's reordering and
+ * mixed-radix FFTs in LDS, O(L log L) per line) for every pass, in either form,
+ * when every axis length's prime factors are at most 13 (else 1). */
 int ard_set_transform_variant(int variant);
 /* Tiling of the batched form after ard_finalize (chosen by the autotuner when
  * ard_params.autotune is set): out[4] = rows per wave, kz planes per block,
- * transform variant, number of candidates timed. No counterpart in the reference. */
+ * transform variant, number of candidates timed. For a finalized plain-form
+ * context: out[2] = the transform variant of its passes, the rest 0. No
+ * counterpart in the reference. */
 int ard_batched_config(ard_ctx* ctx, int* out4);
 /* Sets the batched form's modal-pass tiling after ard_finalize: rows per wave
  * (2, 4, 6 or 8) and kz planes per block (16 to 4096, rounded to an even

@@ -227,3 +227,52 @@ def test_odd_sides_all_fields_match_oracle(form, variant):
         ref = o.field(i, 0)
         err = np.abs(got - ref).max() / np.abs(ref).max()
         assert err < TOL, (i, info, err)
+
+
+@pytest.mark.parametrize("form", FORMS)
+def test_fft_transforms_scene3_and_two_rooms_goldens(form):
+    """The FFT-style transform passes (variant 4) in both forms against the FP64
+    golden traces: every pass of the plain form, the face passes of the
+    batched form."""
+    A.set_transform_variant(4)
+    try:
+        for sc, name in ((S.two_rooms(), "two_rooms"), (S.scene3(), "scene3")):
+            print("case %s fft form %d" % (name, form), flush=True)
+            sim = A.Simulation(sc, form=form)
+            assert sim.batched_config()[2] == 4
+            sim.update(sc.steps)
+            e64 = rel_l2(sim.recorders(), np.load(os.path.join(GOLD, "%s_fp64.npy" % name)))
+            print("%s fft form %d rel-L2 vs fp64 %.3e" % (name, form, e64))
+            sim.close()
+            assert e64 <= TOL
+    finally:
+        A.set_transform_variant(1)
+
+
+def fft_odd_scene():
+    """Odd sides whose prime factors are at most 13 (15, 13, 21, 11, 25, 9):
+    the FFT-style passes' odd lengths and radix 3, 5, 7, 11 and 13 stages."""
+    sc = odd_scene()
+    sc.boxes = [(0, 0, 0, 15, 13, 21), (15, 0, 0, 11, 25, 21), (0, 13, 0, 15, 9, 21)]
+    sc.sources = [S.Source(5, 1, 4), S.Source(19, 12, 10)]
+    sc.recorders = [(21, 9, 7)]
+    return sc
+
+
+@pytest.mark.parametrize("form", FORMS)
+def test_fft_transforms_odd_sides_fields(form):
+    sc = fft_odd_scene()
+    A.set_transform_variant(4)
+    try:
+        sim = A.Simulation(sc, form=form)
+    finally:
+        A.set_transform_variant(1)
+    assert sim.batched_config()[2] == 4
+    o = O.from_scene(sc)
+    n = 60
+    sim.update(n)
+    o.step(n)
+    for i, info in enumerate(sim.partitions()):
+        ref = o.field(i, 0)
+        err = np.abs(sim.pressure_field(i) - ref).max() / np.abs(ref).max()
+        assert err < TOL, (i, info, err)

@@ -87,3 +87,32 @@ def test_dct3d_refuses_overlap_and_bad_type():
         A.dct3d(x, x)
     with pytest.raises(A.ArdError, match="error -1"):
         A.dct3d(x, torch.empty_like(x), dct_type=1)
+
+
+FFT_L = [2, 3, 5, 6, 7, 13, 20, 40, 60, 98, 121, 140, 169, 200, 220, 260, 540, 1024]
+
+
+@pytest.mark.parametrize("inverse", [False, True])
+@pytest.mark.parametrize("b_layout", [0, 1])
+@pytest.mark.parametrize("L", FFT_L)
+@pytest.mark.parametrize("N", [1, 37, 130])
+def test_fft_transform_matches_numpy(inverse, b_layout, L, N):
+    """The FFT-style transform (variant 4: Makhoul's reordering, two real lines
+    per complex FFT, mixed-radix stages 4/2/3/5/7/11/13 in LDS) against scipy's
+    REDFT10 / REDFT01, on every radix, odd lengths and partial tiles."""
+    import scipy.fft
+
+    rng = np.random.default_rng(L * 17 + N + 2 * inverse + b_layout)
+    x = rng.standard_normal((L, N)).astype(np.float32)
+    ref = scipy.fft.dct(x.astype(np.float64), type=3 if inverse else 2, axis=0, norm=None)
+    xin = x if b_layout == 0 else np.ascontiguousarray(x.T)
+    y = A.selftest_fold(xin, inverse=inverse, b_layout=b_layout, entry="ard_selftest_fft")
+    if b_layout == 1:
+        y = y.T
+    err = np.abs(y - ref).max() / np.abs(ref).max()
+    assert err < 2e-6, err
+
+
+def test_fft_transform_refuses_large_prime_factors():
+    with pytest.raises(A.ArdError, match="error -5"):
+        A.selftest_fold(np.ones((17, 4), np.float32), entry="ard_selftest_fft")
