@@ -171,25 +171,32 @@ def test_set_force_and_get_pressure(form):
     assert sim.time_step == 15
 
 
-def test_hall_batched_matches_plain():
-    """Full-size hall (74M DCT cells): the two forms agree once the pulses have
-    reached the recorders (before that the traces are rounding noise)."""
+@pytest.mark.parametrize("form", FORMS)
+def test_hall_matches_fp64_oracle(form):
+    """The full-size hall (74M DCT + 6.3M PML cells, 8 sources, 16 recorders)
+    over the workload's 2000 steps against the FP64 oracle's trace
+    (tests/golden/hall_fp64.npy, the oracle with its O(n log n) transforms):
+    the metric's condition, recorder rel-L2 <= 1e-4 against FP64, in both
+    forms; the batched form also at the hall's tiling and transform choice."""
     sc = S.hall()
-    n = 400
-    print("case hall plain vs batched, %d steps" % n, flush=True)
-    tr = {}
-    fields = {}
-    for form in FORMS:
+    ref = np.load(os.path.join(GOLD, "hall_fp64.npy"))
+    assert ref.shape == (sc.steps, len(sc.recorders))
+    print("case hall vs fp64, form %d, %d steps" % (form, sc.steps), flush=True)
+    if form == A.FORM_BATCHED:
+        A.set_transform_variant(3)
+    try:
         sim = A.Simulation(sc, form=form)
-        sim.update(n)
-        tr[form] = sim.recorders()
-        fields[form] = sim.pressure_field(3)
-        sim.close()
-    e = rel_l2(tr[A.FORM_BATCHED], tr[A.FORM_PLAIN])
-    ef = rel_l2(fields[A.FORM_BATCHED], fields[A.FORM_PLAIN])
-    print("hall batched vs plain: trace rel-L2 %.3e, field rel-L2 %.3e" % (e, ef))
-    assert np.abs(tr[A.FORM_PLAIN]).max() > 0
-    assert e <= TOL and ef <= TOL
+    finally:
+        A.set_transform_variant(1)
+    sim.update(sc.steps)
+    tr = sim.recorders(0, sc.steps)
+    fld = sim.pressure_field(3)
+    sim.close()
+    e = rel_l2(tr, ref)
+    per_rec = [rel_l2(tr[:, r], ref[:, r]) for r in range(ref.shape[1])]
+    print("hall form %d rel-L2 vs fp64 %.3e (worst recorder %.3e)" % (form, e, max(per_rec)))
+    assert np.all(np.isfinite(fld)) and np.abs(tr).max() > 0
+    assert e <= TOL
 
 
 def odd_scene():
