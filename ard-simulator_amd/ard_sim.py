@@ -30,7 +30,7 @@ EXPORTED = [
     "ard_set_batched_tiling",
     "ard_adjoint", "ard_batch_create", "ard_batch_step", "ard_batch_fetch_recorders", "ard_batch_context",
     "ard_batch_room_partition", "ard_batch_destroy", "ard_set_recorder_cubes", "ard_fetch_recorder_cubes",
-    "ard_render_view", "ard_selftest_fold", "ard_dct3d", "ard_selftest_fft",
+    "ard_render_view", "ard_selftest_fold", "ard_dct3d", "ard_dct3d_noalloc", "ard_selftest_fft",
 ]
 PHASES = ["sources", "pml", "fold", "face_fwd1", "face_fwd2", "modal", "unfold", "face_inv1", "face_inv2",
           "interface", "record", "dct_fwd", "dct_inv"]

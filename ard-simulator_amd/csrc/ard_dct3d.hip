@@ -9,12 +9,21 @@
 // workspace holds the intermediate volume, the three cosine matrices of the
 // pass (FFTW's REDFT10 or REDFT01 kernel, ARD/dct_volume.cpp:17,19, built on
 // the device in FP64 and stored FP32) and the three pass descriptors. It is
-// allocated per call.
+// allocated per call by ard_dct3d and is the file-scope device array
+// `scratch` for ard_dct3d_noalloc, which allocates nothing.
 #include <hip/hip_runtime.h>
+
+#include <mutex>
 
 #include "../../include/ard.h"
 #include "ard_ctx_impl.h"
 #include "ard_kernels.h"
+
+// ard_dct3d_noalloc's workspace: one copy per GPU (at its own address on
+// each), held until the process ends. In host code this name is a stand-in,
+// not a device address: host code uses the per-device pointer that
+// hipGetSymbolAddress returns (g_scratch below), never the name itself.
+static __device__ __align__(256) char scratch[ARD_DCT3D_SCRATCH_BYTES];
 
 namespace ard {
 namespace {
@@ -144,3 +153,39 @@ extern "C" int ard_dct3d(int device, int type, int w, int h, int d, const float*
   return e == hipSuccess ? ARD_OK : hip_fail(e, "hipStreamSynchronize");
 }
 
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+char* g_scratch[kMaxDevices] = {};  // per device: the address of `scratch` there
+}  // namespace
+
+// No counterpart in the reference: ard_dct3d for callers that cannot allocate
+// (graph capture, serving loops). The workspace is `scratch`; its device
+// address is taken once per device, with that device current, on the first
+// call there. Asynchronous on `stream`. Every call on one GPU shares that
+// GPU's scratch, so calls that may overlap on one GPU (other streams, other
+// host threads) race: serialise them. Everything the workspace holds is
+// rewritten by each call before it is read (tables, descriptors, volume).
+extern "C" int ard_dct3d_noalloc(int device, int type, int w, int h, int d, const float* in, float* out, float scale,
+                                 void* stream) {
+  using namespace ard;
+  int r;
+  if ((r = dct3d_check(type, w, h, d, in, out))) return r;
+  if (dct3d_workspace_bytes(w, h, d) > size_t(ARD_DCT3D_SCRATCH_BYTES))
+    return set_error(ARD_ERR_RANGE, "volume too large for ard_dct3d_noalloc's workspace (ARD_DCT3D_SCRATCH_BYTES)");
+  if (device < 0 || device >= kMaxDevices) return set_error(ARD_ERR_INVALID, "no such device");
+  hipError_t e = hipSetDevice(device);
+  if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+  char* ws;
+  {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch[device]) {
+      void* p = nullptr;
+      if ((e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch))) != hipSuccess || !p)
+        return hip_fail(e == hipSuccess ? hipErrorInvalidSymbol : e, "hipGetSymbolAddress(scratch)");
+      g_scratch[device] = static_cast<char*>(p);
+    }
+    ws = g_scratch[device];
+  }
+  return dct3d_enqueue(ws, type, w, h, d, in, out, scale, static_cast<hipStream_t>(stream));
+}

@@ -166,6 +166,18 @@ int ard_set_force(ard_ctx* ctx, int part, int x, int y, int z, float f);
  * + h^2 + d^2)) and returns when the transform is done. */
 int ard_dct3d(int device, int type, int w, int h, int d, const float* in, float* out, float scale, void* stream);
 
+/* ard_dct3d for callers that cannot allocate (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc). No counterpart in the reference. Same arguments and
+ * result as ard_dct3d; the workspace is a static device array of ARD_DCT3D_SCRATCH_BYTES per GPU
+ * (held until the process ends), so volumes whose workspace exceeds it (about 4 w h d +
+ * 4 (w^2 + h^2 + d^2) bytes: 250^3 cells fit, 256^3 do not) return ARD_ERR_RANGE. Calls no hipMalloc or
+ * hipFree; asynchronous on `stream` (it returns once the work is enqueued). Every call on one
+ * GPU shares its workspace: calls that may overlap on one GPU (other streams, other threads)
+ * race, so serialise them. */
+#define ARD_DCT3D_SCRATCH_BYTES (64u << 20)
+int ard_dct3d_noalloc(int device, int type, int w, int h, int d, const float* in, float* out, float scale,
+                      void* stream);
+
 /* Blocks until all work queued on the context's stream is done. */
 int ard_synchronize(ard_ctx* ctx);
 
