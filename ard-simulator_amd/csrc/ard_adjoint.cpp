@@ -194,7 +194,7 @@ int setup_adjoint(Ctx& c) {
   std::vector<BndDesc> bd = c.h_bnd;
   std::vector<uint8_t> mask;
   std::vector<long long> mstart;
-  final_writer_masks(c, bd, c.bnd_elems, mask, mstart);
+  final_writer_masks(c.topo, bd, c.bnd_elems, mask, mstart);
   uint8_t* dm = nullptr;
   if (!mask.empty() && ((r = c.dalloc((void**)&dm, mask.size())) || (r = c.upload(dm, mask.data(), mask.size()))))
     return r;

@@ -432,7 +432,7 @@ int build_masks(Ctx& c, std::vector<BndDesc>& descs, const std::vector<long long
   BatchedState& B = *c.bs;
   std::vector<uint8_t> mask;
   std::vector<long long> mstart;
-  final_writer_masks(c, descs, elems, mask, mstart);
+  final_writer_masks(c.topo, descs, elems, mask, mstart);
   uint8_t* dm = nullptr;
   int r = upload_vec(c, (void**)&dm, mask.data(), mask.size());
   if (r) return r;

@@ -10,6 +10,7 @@
 
 #include "../../include/ard.h"
 #include "ard_device.h"
+#include "ard_masks.h"
 #include "ard_topology.h"
 
 namespace ard {
@@ -126,11 +127,8 @@ void modal_tables(int W, int H, int D, double h, double dt, double c0, std::vect
 // shared set-up (ard_plain.cpp)
 int setup_common(Ctx& c);               // step counter, PML slabs, sources/recorders tables
 int build_boundary_descs(Ctx& c, std::vector<BndDesc>& out, std::vector<long long>& elems);
-int shell_of(int W, int H, int D, int x, int y, int z, long long* off);
 std::vector<float> pml_profile(const PartGeom& p, int layers, double h, double c0, double scale);
 void destroy_adjoint(Ctx& c);
-void final_writer_masks(const Ctx& c, const std::vector<BndDesc>& descs, const std::vector<long long>& elems,
-                        std::vector<uint8_t>& mask, std::vector<long long>& mstart);
 int ensure_trace(Ctx& c, int steps_needed);
 
 // plain form (ard_plain.cpp)

@@ -6,6 +6,7 @@ tests/test_oracle_pins.py. Run from the repository root:
 
     python tests/golden/make_golden.py            # two rooms, scene-3, hall cell files
     python tests/golden/make_golden.py --hall     # hall traces (2000 steps; ~30 min per precision)
+    python tests/golden/make_golden.py --rooms64  # 64-room batch, FP64 (1500 steps; ~15 min)
 
 The hall traces (hall_fp64.npy, hall_fp32.npy: [2000][16] recorder samples of
 the seeded L-shaped hall, 74M DCT + 6.3M PML cells) are computed with the
@@ -41,9 +42,27 @@ def hall_traces():
         del o
 
 
+def rooms64_trace():
+    """The 64 independent rooms (sides U[2,6] m, 1-4 boxes each) in one oracle
+    scene: rooms 2 m apart never touch, so each evolves as alone. FP64,
+    [1500][64] (one recorder per room, room order)."""
+    sc = S.rooms64()
+    o = O.from_scene(sc, double=True)
+    chunks = []
+    for t0 in range(0, sc.steps, 100):
+        chunks.append(o.step(min(100, sc.steps - t0)))
+        print("rooms64 fp64: %d steps" % (t0 + len(chunks[-1])), flush=True)
+    tr = np.concatenate(chunks)
+    np.save(os.path.join(OUT, "rooms64_fp64.npy"), tr.astype(np.float64))
+    print("rooms64_fp64.npy", tr.shape, float(np.abs(tr).max()), flush=True)
+
+
 def main():
     if "--hall" in sys.argv[1:]:
         hall_traces()
+        return
+    if "--rooms64" in sys.argv[1:]:
+        rooms64_trace()
         return
     # the seeded hall in cell units (ard_run --cells)
     S.write_cell_files(S.hall(), os.path.join(OUT, "hall_cells"))
