@@ -31,6 +31,7 @@ EXPORTED = [
     "ard_adjoint", "ard_batch_create", "ard_batch_step", "ard_batch_fetch_recorders", "ard_batch_context",
     "ard_batch_room_partition", "ard_batch_destroy", "ard_set_recorder_cubes", "ard_fetch_recorder_cubes",
     "ard_render_view", "ard_selftest_fold", "ard_dct3d", "ard_dct3d_noalloc", "ard_selftest_fft",
+    "ard_selftest_fold_bf16",
 ]
 PHASES = ["sources", "pml", "fold", "face_fwd1", "face_fwd2", "modal", "unfold", "face_inv1", "face_inv2",
           "interface", "record", "dct_fwd", "dct_inv"]
@@ -88,6 +89,7 @@ def lib():
         L.ard_set_transform_variant.argtypes = [i]
         L.ard_selftest_fold.argtypes = [i, i, i, i, i, fp, fp]
         L.ard_selftest_fft.argtypes = [i, i, i, i, i, fp, fp]
+        L.ard_selftest_fold_bf16.argtypes = [i, i, i, i, i, fp, fp]
         L.ard_dct3d.argtypes = [i, i, i, i, i, vp, vp, f, vp]
         L.ard_batched_config.argtypes = [vp, ip]
         L.ard_set_batched_tiling.argtypes = [vp, ctypes.c_int, ctypes.c_int]
@@ -343,6 +345,18 @@ class Batch:
 
     def update(self, n=1):
         _check(self.L.ard_batch_step(self.batch, n))
+
+    def set_kernel_timing(self, on):
+        """Per-phase kernel timing of the batch's context (ard_set_kernel_timing)."""
+        _check(self.L.ard_set_kernel_timing(self.ctx, int(on)))
+
+    def kernel_times(self):
+        """{phase: (average ms per launch, launches)} since timing was switched on."""
+        n = len(PHASES)
+        ms = np.zeros(n, dtype=np.float32)
+        cnt = np.zeros(n, dtype=np.int32)
+        _check(self.L.ard_kernel_times(self.ctx, _fp(ms), cnt.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), n))
+        return {PHASES[i]: (float(ms[i]), int(cnt[i])) for i in range(n) if cnt[i]}
 
     def synchronize(self):
         _check(self.L.ard_synchronize(self.ctx))

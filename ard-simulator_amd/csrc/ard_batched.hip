@@ -10,7 +10,10 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
+#include <cstdint>
+#include <cstring>
 #include <map>
 #include <memory>
 #include <string>
@@ -84,6 +87,7 @@ struct BatchedState {
   GemmDesc* d_gemm_fold[4] = {};  // the same passes even/odd folded (transform variant 3)
   int2* d_fold_tiles[4] = {};      // their tile tables (fold_tile_table)
   int gemm_fold_tiles[4] = {};
+  GemmDesc* d_gemm_bf16[4] = {};  // the folded passes in BF16 split precision (variant 2), d_fold_tiles' tiles
   GemmDesc* d_gemm_fft[4] = {};   // the same passes FFT-style (variant 4), when every length factors
   int gemm_fft_tiles[4] = {};
   size_t fft_lds[4] = {};
@@ -193,12 +197,12 @@ int fold_even_rows(int L) { return int(ceil_div((L + 1) / 2, 32) * 32); }
 int fold_rows_fwd(int L) { return int(ceil_div(fold_even_rows(L) + L / 2, 64) * 64); }
 int fold_rows_inv(int L) { return int(ceil_div((L + 1) / 2, 32) * 32); }
 
-int fold_tables(Ctx& c, int L, const float** fwd, const float** inv) {
-  auto it = c.fold_tables.find(L);
-  if (it == c.fold_tables.end()) {
-    const int Kh = (L + 1) / 2, Me = (L + 1) / 2, Mo = L / 2, Kp = fold_kp(L);
-    const int Mh = fold_even_rows(L), Rf = fold_rows_fwd(L), Ri = fold_rows_inv(L);
-    std::vector<float> f(size_t(Rf) * Kp, 0.f), v(size_t(2) * Ri * Kp, 0.f);
+void fold_table_host(int L, std::vector<float>& f, std::vector<float>& v) {
+  const int Kh = (L + 1) / 2, Me = (L + 1) / 2, Mo = L / 2, Kp = fold_kp(L);
+  const int Mh = fold_even_rows(L), Rf = fold_rows_fwd(L), Ri = fold_rows_inv(L);
+  f.assign(size_t(Rf) * Kp, 0.f);
+  v.assign(size_t(2) * Ri * Kp, 0.f);
+  {
     auto c2 = [&](int k, int j) { return 2.0 * std::cos(kPi * k * (j + 0.5) / L); };
     auto c3 = [&](int j, int k) { return k == 0 ? 1.0 : 2.0 * std::cos(kPi * k * (j + 0.5) / L); };
     for (int j = 0; j < Kh; ++j) {
@@ -210,6 +214,51 @@ int fold_tables(Ctx& c, int L, const float** fwd, const float** inv) {
         if (2 * k + 1 < L) v[size_t(Ri + j) * Kp + k] = float(c3(j, 2 * k + 1));
       }
     }
+  }
+}
+
+// BF16 round to nearest even of a finite float, as the device's bf16_rn
+uint16_t bf16_host(float x) {
+  uint32_t u;
+  std::memcpy(&u, &x, 4);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return uint16_t(u >> 16);
+}
+
+// The folded tables' BF16 high and low parts (x = hi + lo), for
+// k_gemm_bf16x3_fold: out = {forward hi, forward lo, inverse hi, inverse lo}.
+int fold_tables_bf16(Ctx& c, int L, const void* out[4]) {
+  auto it = c.fold_bf16.find(L);
+  if (it == c.fold_bf16.end()) {
+    std::vector<float> f, v;
+    fold_table_host(L, f, v);
+    std::array<void*, 4> d{};
+    int r;
+    int q = 0;
+    for (const std::vector<float>* t : {&f, &v}) {
+      std::vector<uint16_t> hi(t->size()), lo(t->size());
+      for (size_t i = 0; i < t->size(); ++i) {
+        hi[i] = bf16_host((*t)[i]);
+        const uint32_t hb = uint32_t(hi[i]) << 16;
+        float hf;
+        std::memcpy(&hf, &hb, 4);
+        lo[i] = bf16_host((*t)[i] - hf);
+      }
+      if ((r = upload_vec(c, &d[q], hi.data(), hi.size() * 2)) || (r = upload_vec(c, &d[q + 1], lo.data(), lo.size() * 2)))
+        return r;
+      q += 2;
+    }
+    it = c.fold_bf16.emplace(L, d).first;
+  }
+  for (int i = 0; i < 4; ++i) out[i] = it->second[i];
+  return ARD_OK;
+}
+
+int fold_tables(Ctx& c, int L, const float** fwd, const float** inv) {
+  auto it = c.fold_tables.find(L);
+  if (it == c.fold_tables.end()) {
+    std::vector<float> f, v;
+    fold_table_host(L, f, v);
     float *df, *di;
     int r;
     if ((r = upload_vec(c, (void**)&df, f.data(), f.size() * 4)) || (r = upload_vec(c, (void**)&di, v.data(), v.size() * 4)))
@@ -267,6 +316,9 @@ int launch_face(const BatchedState& B, int k, int v, hipStream_t st) {
     return launch_gemm_fold(B.d_gemm_fold[k], B.d_fold_tiles[k], B.gemm_fold_tiles[k], k % 2 == 0, k < 2 ? 1 : 2, st);
   if (v == 4)
     return launch_fft(B.d_gemm_fft[k], B.ngemm[k], B.gemm_fft_tiles[k], k % 2 == 0, k % 2 == 0, 0, B.fft_lds[k], st);
+  if (v == 2)
+    return launch_gemm_bf16_fold(B.d_gemm_bf16[k], B.d_fold_tiles[k], B.gemm_fold_tiles[k], k % 2 == 0, k < 2 ? 1 : 2,
+                                 st);
   return launch_gemm(B.d_gemm[k], B.ngemm[k], B.gemm_tiles[k], k % 2 == 0, k % 2 == 0, 0, st, v);
 }
 
@@ -641,6 +693,15 @@ int finalize_batched(Ctx& c) {
     if ((r = upload_gemms(c, gm[k], &B.d_gemm[k], &B.gemm_tiles[k]))) return r;
     if ((r = upload_gemms(c, gf, &B.d_gemm_fold[k], &B.gemm_fold_tiles[k]))) return r;
     if ((r = fold_tile_table(c, gf, &B.d_fold_tiles[k]))) return r;
+    // and their BF16 split twins: the tables' high and low parts
+    std::vector<GemmDesc> gb = gf;
+    for (GemmDesc& g : gb) {
+      const void* t[4];
+      if ((r = fold_tables_bf16(c, g.L, t))) return r;
+      g.A = static_cast<const float*>(k < 2 ? t[0] : t[2]);
+      g.Alo = k < 2 ? t[1] : t[3];
+    }
+    if (!gb.empty() && (r = upload_vec(c, (void**)&B.d_gemm_bf16[k], gb.data(), gb.size() * sizeof(GemmDesc)))) return r;
   }
   // the FFT-style twins of the dense passes (variant 4): only if every face
   // axis length factors into radices up to 13
@@ -985,7 +1046,7 @@ void destroy_batched(Ctx& c) {
 // transform of length L over N columns, built exactly as the batched form
 // builds its face passes (fold_tables, a folded GemmDesc, fold_tile_table), in
 // a temporary context of its own.
-int fold_selftest(int device, int inverse, int b_layout, int L, int N, const float* X, float* Y) {
+int fold_selftest(int device, int inverse, int b_layout, int L, int N, const float* X, float* Y, bool bf16) {
   hipError_t e = hipSetDevice(device);
   if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
   Ctx c;
@@ -1019,13 +1080,19 @@ int fold_selftest(int device, int inverse, int b_layout, int L, int N, const flo
     g.A = fi;
     g.M = int(64 * ceil_div((L + 1) / 2, 32));
   }
+  if (bf16) {
+    const void* t[4];
+    if ((r = fold_tables_bf16(c, L, t))) return r;
+    g.A = static_cast<const float*>(inverse ? t[2] : t[0]);
+    g.Alo = inverse ? t[3] : t[1];
+  }
   g.tiles_n = int(ceil_div(N, 64));
   std::vector<GemmDesc> v{g};
   GemmDesc* dd;
   int2* dt;
   if ((r = upload_vec(c, (void**)&dd, v.data(), sizeof(GemmDesc))) || (r = fold_tile_table(c, v, &dt))) return r;
   const int tiles = int(ceil_div(g.M, 64)) * g.tiles_n;
-  if (launch_gemm_fold(dd, dt, tiles, b_layout == 1, inverse ? 2 : 1, c.stream))
+  if ((bf16 ? launch_gemm_bf16_fold : launch_gemm_fold)(dd, dt, tiles, b_layout == 1, inverse ? 2 : 1, c.stream))
     return set_error(ARD_ERR_HIP, "folded transform launch");
   if ((e = hipMemcpyAsync(Y, dY, n * 4, hipMemcpyDeviceToHost, c.stream)) != hipSuccess ||
       (e = hipStreamSynchronize(c.stream)) != hipSuccess)

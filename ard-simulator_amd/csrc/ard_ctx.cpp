@@ -574,7 +574,13 @@ int ard_selftest_fold(int device, int inverse, int b_layout, int L, int N, const
   // L >= 6: partition sides are at least 6 cells, and the kernel relies on it
   if (!X || !Y || L < 6 || N <= 0 || b_layout < 0 || b_layout > 1) return fail(ARD_ERR_INVALID, "bad arguments");
   if ((long long)L * N + 16 >= (1LL << 30)) return fail(ARD_ERR_INVALID, "too large");
-  return fold_selftest(device, inverse != 0, b_layout, L, N, X, Y);
+  return fold_selftest(device, inverse != 0, b_layout, L, N, X, Y, false);
+}
+
+int ard_selftest_fold_bf16(int device, int inverse, int b_layout, int L, int N, const float* X, float* Y) {
+  if (!X || !Y || L < 6 || N <= 0 || b_layout < 0 || b_layout > 1) return fail(ARD_ERR_INVALID, "bad arguments");
+  if ((long long)L * N + 16 >= (1LL << 30)) return fail(ARD_ERR_INVALID, "too large");
+  return fold_selftest(device, inverse != 0, b_layout, L, N, X, Y, true);
 }
 
 }  // extern "C"

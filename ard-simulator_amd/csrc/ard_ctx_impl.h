@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <array>
 #include <map>
 #include <string>
 #include <utility>
@@ -73,6 +74,7 @@ struct Ctx {
   std::map<int, std::pair<float*, float*>> tables;  // axis length -> (REDFT10, REDFT01) matrices
   std::map<int, std::pair<float*, float*>> fold_tables;  // axis length -> even/odd folded (forward, inverse)
   std::map<int, float*> fft_plans;                       // axis length -> FFT plan table (variant 4)
+  std::map<int, std::array<void*, 4>> fold_bf16;         // axis length -> folded tables' BF16 hi / lo parts
 
   // step counter and recorder trace, device-resident so captured graphs replay correctly
   int* d_step = nullptr;
@@ -144,7 +146,7 @@ void destroy_batched(Ctx& c);
 int batched_config(Ctx& c, int* out4);
 int batched_set_tiling(Ctx& c, int rpw, int kzc);
 int batched_modes(Ctx& c, int part, const float* M[2], int* wp);
-int fold_selftest(int device, int inverse, int b_layout, int L, int N, const float* X, float* Y);
+int fold_selftest(int device, int inverse, int b_layout, int L, int N, const float* X, float* Y, bool bf16);
 // FFT-style transforms (ard_fft.hip): the FFT fields of a dense pass
 // descriptor (its plan cached in the context; ARD_ERR_UNSUPPORTED for a
 // length fft_factor refuses), its tile count, and the entry self-test

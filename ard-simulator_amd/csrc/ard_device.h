@@ -76,6 +76,9 @@ struct GemmDesc {
   // rows (32 of E, 32 of O) per 32 outputs.
   int Ap, Kp;
   unsigned b_elems;
+  // BF16 split folded transforms (k_gemm_bf16x3_fold): A holds the tables'
+  // BF16 high parts, Alo their low parts (same layout, 2-byte elements)
+  const void* Alo;
   // FFT-style per-axis transform (k_dct_fft, transform variant 4): the plan of
   // the length K (fft_plan_table), complex lines (pairs of lines) per tile,
   // direction (0 REDFT10, 1 REDFT01) and the radices of the stages

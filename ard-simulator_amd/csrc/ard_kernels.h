@@ -16,6 +16,11 @@ int launch_gemm(const GemmDesc* d_descs, int ndesc, int total_tiles, bool b_k_co
 // d_tiles[b] = (descriptor, tile within it) of block b (fold_tile_table).
 int launch_gemm_fold(const GemmDesc* d_descs, const int2* d_tiles, int total_tiles, bool b_k_contig, int fold,
                      hipStream_t s);
+// The same folded transforms in BF16 split precision (three BF16 products,
+// FP32 accumulate) on v_mfma_f32_32x32x16_bf16; the descriptors' A / Alo are
+// the folded tables' BF16 high and low parts.
+int launch_gemm_bf16_fold(const GemmDesc* d_descs, const int2* d_tiles, int total_tiles, bool b_k_contig, int fold,
+                          hipStream_t s);
 // Blocks of one slab's update for PML thickness L (pml_layers); sets d.nb.
 // Slabs L cells thick along their normal, for L from 3 to 8, take the
 // LDS-tiled update; others (thicker layers, or slabs of another thickness

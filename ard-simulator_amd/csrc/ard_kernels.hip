@@ -610,6 +610,191 @@ __global__ __launch_bounds__(256) void k_gemm_mfma_fold(const GemmDesc* __restri
   }
 }
 
+// ---------------------------------------------------------------------------
+// The even/odd folded transforms in BF16 split precision on gfx950's
+// v_mfma_f32_32x32x16_bf16: x = hi + lo in BF16, A B ~ A_hi B_hi + A_hi B_lo
+// + A_lo B_hi, FP32 accumulation (a product's relative error about 2^-16),
+// three instructions per 16 columns of the folded K against eight FP32
+// v_mfma_f32_32x32x2_f32. Operand layout of the instruction: lane l holds
+// A[l % 32][8 (l / 32) + i] and B[8 (l / 32) + i][l % 32], i < 8; the
+// accumulator layout is that of the other 32 x 32 instructions.
+// A (the folded cosine tables, g.A / g.Alo: BF16 hi and lo parts split on the
+// host, [Ap][Kp] forward or [2][Ap][Kp] inverse) goes from memory straight
+// into the operand registers (16 bytes per lane and part); B is folded and
+// split once while staged in LDS as [n][k] rows (24 halfwords: the 16-byte
+// operand reads of one phase land in distinct banks). Tiles, B addressing,
+// the tile table and the outputs are those of k_gemm_mfma_fold.
+// ---------------------------------------------------------------------------
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef unsigned short u16;
+
+template <bool B_K_CONTIG, int FOLD>
+__global__ __launch_bounds__(256) void k_gemm_bf16x3_fold(const GemmDesc* __restrict__ descs,
+                                                          const int2* __restrict__ tiles) {
+  // [hi / lo][buffer][set][n][k]: 24 KB, also the inverse's exchange space
+  __shared__ __attribute__((aligned(16))) u16 Bs[2][2][2][64][24];
+  auto& Bh = Bs[0];
+  auto& Bl = Bs[1];
+  const int2 tm = tiles[blockIdx.x];
+  const GemmDesc& g = descs[tm.x];
+  const int M = g.M, N = g.N, K = g.K, N1 = g.N1, L = g.L, Kp = g.Kp;
+  const int t = tm.y;
+  const int tiles_m = (M + 63) >> 6;
+  const int m0 = (t % tiles_m) * 64;
+  const int n0 = (t / tiles_m) * 64;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+  const float sg0 = FOLD == 1 && m0 >= g.Mh ? -1.f : 1.f;
+  const float sg1 = FOLD == 1 && m0 + 32 >= g.Mh ? -1.f : 1.f;
+  const int bsel = wm >> 5;
+  // A: this lane's operand row of the wave's 32 rows, 8 k per 16-column step
+  const int arow = FOLD == 2 ? bsel * g.Ap + (m0 >> 1) + (lane & 31) : m0 + wm + (lane & 31);
+  const unsigned aoff = unsigned(arow * Kp + 8 * (lane >> 5)) * 2u;
+  const brsrc rAh = make_rsrc(g.A, unsigned((FOLD == 2 ? 2 : 1) * g.Ap * Kp) * 2u);
+  const brsrc rAl = make_rsrc(g.Alo, unsigned((FOLD == 2 ? 2 : 1) * g.Ap * Kp) * 2u);
+  const brsrc rB = make_rsrc(g.B, (g.b_elems + 4u) * 4u);
+  const int bK = int(g.bK);
+  int bcol;
+  if (B_K_CONTIG) {
+    const int n = min(n0 + (tid >> 2), N - 1);
+    bcol = int((n % N1) * g.b1 + (n / N1) * g.b2);
+  } else {
+    const int n = min(n0 + lane, N - 1);
+    bcol = int((n % N1) * g.b1 + (n / N1) * g.b2);
+  }
+  const int kt = tid >> 6;
+  f32x16 acc;
+#pragma unroll
+  for (int r = 0; r < 16; ++r) acc[r] = 0.f;
+
+  v4f rah, ral;     // this lane's A operand (hi, lo) of the next step
+  float rb[2][4];   // B elements, as in k_gemm_mfma_fold
+  auto gload = [&](const int k0) {
+    rah = __builtin_bit_cast(v4f, __builtin_amdgcn_raw_buffer_load_b128(rAh, int(aoff), int(k0 * 2), 0));
+    ral = __builtin_bit_cast(v4f, __builtin_amdgcn_raw_buffer_load_b128(rAl, int(aoff), int(k0 * 2), 0));
+    if (B_K_CONTIG) {
+      const int k = k0 + 4 * (tid & 3);
+      if (FOLD == 1) {
+        const v4f d = __builtin_bit_cast(v4f, __builtin_amdgcn_raw_buffer_load_b128(rB, int(unsigned(bcol + k) * 4u), 0, 0));
+        const v4f m = __builtin_bit_cast(v4f, __builtin_amdgcn_raw_buffer_load_b128(rB, int(unsigned(bcol + L - 4 - k) * 4u), 0, 0));
+        rb[0][0] = d.x; rb[0][1] = d.y; rb[0][2] = d.z; rb[0][3] = d.w;
+        rb[1][0] = m.w; rb[1][1] = m.z; rb[1][2] = m.y; rb[1][3] = m.x;
+      } else {
+        const v4f f0 = __builtin_bit_cast(v4f, __builtin_amdgcn_raw_buffer_load_b128(rB, int(unsigned(bcol + 2 * k) * 4u), 0, 0));
+        const v4f f1 = __builtin_bit_cast(v4f, __builtin_amdgcn_raw_buffer_load_b128(rB, int(unsigned(bcol + 2 * k + 4) * 4u), 0, 0));
+        rb[0][0] = f0.x; rb[1][0] = f0.y; rb[0][1] = f0.z; rb[1][1] = f0.w;
+        rb[0][2] = f1.x; rb[1][2] = f1.y; rb[0][3] = f1.z; rb[1][3] = f1.w;
+      }
+    } else {
+#pragma unroll
+      for (int w = 0; w < 2; ++w)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int k = k0 + kt + 4 * r;
+          const int row = FOLD == 1 ? (w == 0 ? k : L - 1 - k) : 2 * k + w;
+          rb[w][r] = bload(rB, unsigned(row * bK + bcol) * 4u, 0u);
+        }
+    }
+  };
+  // B staged as [set][n][k], split into hi / lo once
+  auto sstore = [&](const int buf) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int k = B_K_CONTIG ? 4 * (tid & 3) + r : kt + 4 * r;
+      const int n = B_K_CONTIG ? (tid >> 2) : lane;
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const float v = FOLD == 1 ? rb[0][r] + (h ? sg1 : sg0) * rb[1][r] : rb[h][r];
+        const short hi = bf16_rn(v);
+        Bh[buf][h][n][k] = u16(hi);
+        Bl[buf][h][n][k] = u16(bf16_rn(v - bf16_f(hi)));
+      }
+    }
+  };
+  gload(0);
+  int buf = 0;
+  for (int k0 = 0; k0 < K; k0 += 16, buf ^= 1) {
+    const v4f ah = rah, al = ral;
+    sstore(buf);
+    __syncthreads();
+    if (k0 + 16 < K) gload(k0 + 16);
+    const v4f bh = *reinterpret_cast<const v4f*>(&Bh[buf][bsel][wn + (lane & 31)][8 * (lane >> 5)]);
+    const v4f bl = *reinterpret_cast<const v4f*>(&Bl[buf][bsel][wn + (lane & 31)][8 * (lane >> 5)]);
+    const bf16x8 Ah = __builtin_bit_cast(bf16x8, ah), Al = __builtin_bit_cast(bf16x8, al);
+    const bf16x8 Bhh = __builtin_bit_cast(bf16x8, bh), Bll = __builtin_bit_cast(bf16x8, bl);
+    // K-contiguous B: the output is contiguous along m, so the instruction's
+    // columns take m: C^T = B^T A^T
+    if (B_K_CONTIG) {
+      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Bll, Ah, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Bhh, Al, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Bhh, Ah, acc, 0, 0, 0);
+    } else {
+      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Al, Bhh, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ah, Bll, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ah, Bhh, acc, 0, 0, 0);
+    }
+  }
+  // outputs exactly as k_gemm_mfma_fold
+  f32x16 oth;
+  if (FOLD == 2) {
+    __syncthreads();
+    float* __restrict__ xs = reinterpret_cast<float*>(&Bs[0][0][0][0][0]);  // 4096 of its 6144 floats
+#pragma unroll
+    for (int r = 0; r < 16; ++r) xs[((bsel * 2 + (wn >> 5)) * 16 + r) * 64 + lane] = acc[r];
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < 16; ++r) oth[r] = xs[(((bsel ^ 1) * 2 + (wn >> 5)) * 16 + r) * 64 + lane];
+  }
+  gfloat* __restrict__ C = ARD_G(g.C);
+  const int mb = FOLD == 2 ? (m0 >> 1) : m0 + wm;
+  const int mlim = FOLD == 2 ? (L + 1) / 2 : M;
+  const int jl = (B_K_CONTIG ? mb : n0 + wn) + (lane & 31);
+  if (jl >= (B_K_CONTIG ? mlim : N)) return;
+  long long cn = 0;
+  if (!B_K_CONTIG) cn = (long long)(jl % N1) * g.c1 + (long long)(jl / N1) * g.c2;
+  const int nb = n0 + wn + 4 * (lane >> 5);
+  int nq = 0, nr = 0;
+  if (B_K_CONTIG) {
+    nq = int(float(nb) * (1.f / float(N1)));
+    nr = nb - nq * N1;
+    while (nr < 0) { nr += N1; --nq; }
+    while (nr >= N1) { nr -= N1; ++nq; }
+  }
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    const int il = (B_K_CONTIG ? n0 + wn : mb) + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+    if (il >= (B_K_CONTIG ? N : mlim)) continue;
+    const int i = B_K_CONTIG ? jl : il;
+    if (B_K_CONTIG) {
+      int q = nq, rr = nr + 8 * (r >> 2) + (r & 3);
+      while (rr >= N1) { rr -= N1; ++q; }
+      cn = (long long)rr * g.c1 + (long long)q * g.c2;
+    }
+    if (FOLD == 1) {
+      const int k = i < g.Mh ? 2 * i : 2 * (i - g.Mh) + 1;
+      if (k < L) C[(long long)k * g.cM + cn] = g.alpha * acc[r];
+    } else {
+      const float e = bsel == 0 ? acc[r] : oth[r], o = bsel == 0 ? oth[r] : acc[r];
+      if (bsel == 0) C[(long long)i * g.cM + cn] = g.alpha * (e + o);
+      else if (L - 1 - i != i) C[(long long)(L - 1 - i) * g.cM + cn] = g.alpha * (e - o);
+    }
+  }
+}
+
+int launch_gemm_bf16_fold(const GemmDesc* d_descs, const int2* d_tiles, int total_tiles, bool b_k_contig, int fold,
+                          hipStream_t s) {
+  if (total_tiles <= 0) return 0;
+  dim3 grid(total_tiles), block(256);
+  if (fold == 1 && b_k_contig) hipLaunchKernelGGL((k_gemm_bf16x3_fold<true, 1>), grid, block, 0, s, d_descs, d_tiles);
+  else if (fold == 1) hipLaunchKernelGGL((k_gemm_bf16x3_fold<false, 1>), grid, block, 0, s, d_descs, d_tiles);
+  else if (fold == 2 && b_k_contig) hipLaunchKernelGGL((k_gemm_bf16x3_fold<true, 2>), grid, block, 0, s, d_descs, d_tiles);
+  else if (fold == 2) hipLaunchKernelGGL((k_gemm_bf16x3_fold<false, 2>), grid, block, 0, s, d_descs, d_tiles);
+  else return -1;
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
 int launch_gemm_fold(const GemmDesc* d_descs, const int2* d_tiles, int total_tiles, bool b_k_contig, int fold,
                      hipStream_t s) {
   if (total_tiles <= 0) return 0;

@@ -220,6 +220,11 @@ int ard_selftest_transform(int device, int variant, int b_layout, int M, int N, 
  * passes. L >= 6 (the smallest partition side, which the kernel assumes).
  * Synchronous; allocates and frees its own buffers. */
 int ard_selftest_fold(int device, int inverse, int b_layout, int L, int N, const float* X, float* Y);
+/* The same self-test for the folded transform in BF16 split precision on the
+ * matrix cores (variant 2's batched face passes: every operand x = hi + lo in
+ * BF16, three products on v_mfma_f32_32x32x16_bf16, FP32 accumulate). No
+ * counterpart in the reference. */
+int ard_selftest_fold_bf16(int device, int inverse, int b_layout, int L, int N, const float* X, float* Y);
 /* Self-test of the FFT-style transform kernel (variant 4; no counterpart in the
  * reference, whose FFTW plans it replaces, ARD/dct_volume.cpp:17,19): the same
  * transform as ard_selftest_fold, for L >= 2 whose prime factors are at most 13

@@ -116,3 +116,25 @@ def test_fft_transform_matches_numpy(inverse, b_layout, L, N):
 def test_fft_transform_refuses_large_prime_factors():
     with pytest.raises(A.ArdError, match="error -5"):
         A.selftest_fold(np.ones((17, 4), np.float32), entry="ard_selftest_fft")
+
+
+@pytest.mark.parametrize("inverse", [False, True])
+@pytest.mark.parametrize("b_layout", [0, 1])
+@pytest.mark.parametrize("L", FOLD_L)
+@pytest.mark.parametrize("N", [1, 37, 130])
+def test_folded_bf16_transform_matches_numpy(inverse, b_layout, L, N):
+    """The folded transform in BF16 split precision (variant 2's face passes:
+    three BF16 products per 16 folded columns on v_mfma_f32_32x32x16_bf16,
+    FP32 accumulate) against scipy's REDFT10 / REDFT01; a product carries
+    about 2^-16 relative error."""
+    import scipy.fft
+
+    rng = np.random.default_rng(L * 131 + N * 7 + 2 * inverse + b_layout + 1)
+    x = rng.standard_normal((L, N)).astype(np.float32)
+    ref = scipy.fft.dct(x.astype(np.float64), type=3 if inverse else 2, axis=0, norm=None)
+    xin = x if b_layout == 0 else np.ascontiguousarray(x.T)
+    y = A.selftest_fold(xin, inverse=inverse, b_layout=b_layout, entry="ard_selftest_fold_bf16")
+    if b_layout == 1:
+        y = y.T
+    err = np.abs(y - ref).max() / np.abs(ref).max()
+    assert err < 1e-4, err
