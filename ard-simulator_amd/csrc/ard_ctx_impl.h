@@ -94,6 +94,7 @@ struct Ctx {
   GemmDesc* d_gemm = nullptr;
   int plain_variant = 1;       // transform kernel of the plain form, fixed at finalize (1, 0, 2 or 4)
   GemmDesc* d_fft = nullptr;   // FFT twins of h_gemm (variant 4)
+  const float* plain_graph_trace = nullptr;  // trace buffer the plain step graphs were captured with
   size_t fft_lds = 0;          // dynamic LDS of the plain form's FFT passes
   std::vector<PmlDesc> h_pml;      // block0 = 0 each (one launch per slab)
   PmlDesc* d_pml = nullptr;

@@ -294,59 +294,94 @@ int finalize_plain(Ctx& c) {
 // One Simulation::Update per iteration (ARD/simulation.cpp:300-317): sources,
 // every partition's update, then the boundaries in list order (the later
 // writer wins on shared cells), then the recorder sample.
-int step_plain(Ctx& c, int n) {
+// One plain step of parity par: sources, the partitions in list order, PML
+// slabs, boundaries in list order (the last writer wins on shared cells), then
+// the recorder sample. With a timer, each phase is bracketed by events.
+static int enqueue_plain(Ctx& c, int par, PhaseTimer* T) {
   const int nr = int(c.topo.recorders.size());
-  PhaseTimer& T = c.timer;
-  for (int s = 0; s < n; ++s) {
-    const int par = c.time_step & 1;
-    int ev = T.begin(c.stream, ARD_PH_SOURCES);
-    if (launch_sources(c.d_src, c.nsrc, c.d_step, c.courant, c.stream)) return set_error(ARD_ERR_HIP, "sources");
-    T.end(c.stream, ev);
-    for (size_t i = 0; i < c.dct.size(); ++i) {
-      const DctPlain& D = c.dct[i];
-      if (!D.F) continue;
-      const int off = D.gemm0 + par * 6;
-      const bool fft = c.plain_variant == 4;
-      // pass j: x (B and C contiguous along the axis), then y, then z (with
-      // the modal update in the forward z pass)
-      auto pass = [&](int j) {
-        const bool kc = j % 3 == 0;
-        const int ep = j == 2 ? 1 : 0;
-        return fft ? launch_fft(c.d_fft + off + j, 1, D.fft_tiles[j], kc, kc, ep, c.fft_lds, c.stream)
-                   : launch_gemm(c.d_gemm + off + j, 1, D.tiles[j], kc, kc, ep, c.stream, c.plain_variant);
-      };
-      int r = 0;
-      ev = T.begin(c.stream, ARD_PH_DCT_FWD);
-      for (int j = 0; j < 3; ++j) r |= pass(j);
-      T.end(c.stream, ev);
-      ev = T.begin(c.stream, ARD_PH_DCT_INV);
-      for (int j = 3; j < 6; ++j) r |= pass(j);
-      T.end(c.stream, ev);
-      if (r) return set_error(ARD_ERR_HIP, "DCT pass launch");
-    }
-    for (size_t i = 0; i < c.h_pml.size(); ++i) {
-      ev = T.begin(c.stream, ARD_PH_PML);
-      if (launch_pml(c.d_pml + i, 1, c.pml_blocks[i], par, c.prm.pml_layers, c.prm.h, c.prm.dt, c.prm.c0,
-                     c.stream))
-        return set_error(ARD_ERR_HIP, "PML launch");
-      T.end(c.stream, ev);
-    }
-    for (size_t b = 0; b < c.h_bnd.size(); ++b) {
-      ev = T.begin(c.stream, ARD_PH_INTERFACE);
-      if (launch_interface(c.d_bnd + b, 1, c.d_parts, par ^ 1, c.bnd_elems[b], c.stream))
-        return set_error(ARD_ERR_HIP, "interface launch");
-      T.end(c.stream, ev);
-    }
-    ev = T.begin(c.stream, ARD_PH_RECORD);
-    if (enqueue_cubes(c, par, c.stream)) return set_error(ARD_ERR_HIP, "recorder cube launch");
-    if (launch_record(c.d_rec, nr, c.d_trace, c.trace_cap, c.d_step, c.stream))
-      return set_error(ARD_ERR_HIP, "recorder launch");
-    T.end(c.stream, ev);
-    c.time_step++;
-    if (T.on && T.used > 2048) {
-      int r = T.collect();
+  auto begin = [&](int ph) { return T ? T->begin(c.stream, ph) : -1; };
+  auto end = [&](int ev) { if (T) T->end(c.stream, ev); };
+  int ev = begin(ARD_PH_SOURCES);
+  if (launch_sources(c.d_src, c.nsrc, c.d_step, c.courant, c.stream)) return set_error(ARD_ERR_HIP, "sources");
+  end(ev);
+  for (size_t i = 0; i < c.dct.size(); ++i) {
+    const DctPlain& D = c.dct[i];
+    if (!D.F) continue;
+    const int off = D.gemm0 + par * 6;
+    const bool fft = c.plain_variant == 4;
+    // pass j: x (B and C contiguous along the axis), then y, then z (with
+    // the modal update in the forward z pass)
+    auto pass = [&](int j) {
+      const bool kc = j % 3 == 0;
+      const int ep = j == 2 ? 1 : 0;
+      return fft ? launch_fft(c.d_fft + off + j, 1, D.fft_tiles[j], kc, kc, ep, c.fft_lds, c.stream)
+                 : launch_gemm(c.d_gemm + off + j, 1, D.tiles[j], kc, kc, ep, c.stream, c.plain_variant);
+    };
+    int r = 0;
+    ev = begin(ARD_PH_DCT_FWD);
+    for (int j = 0; j < 3; ++j) r |= pass(j);
+    end(ev);
+    ev = begin(ARD_PH_DCT_INV);
+    for (int j = 3; j < 6; ++j) r |= pass(j);
+    end(ev);
+    if (r) return set_error(ARD_ERR_HIP, "DCT pass launch");
+  }
+  for (size_t i = 0; i < c.h_pml.size(); ++i) {
+    ev = begin(ARD_PH_PML);
+    if (launch_pml(c.d_pml + i, 1, c.pml_blocks[i], par, c.prm.pml_layers, c.prm.h, c.prm.dt, c.prm.c0, c.stream))
+      return set_error(ARD_ERR_HIP, "PML launch");
+    end(ev);
+  }
+  for (size_t b = 0; b < c.h_bnd.size(); ++b) {
+    ev = begin(ARD_PH_INTERFACE);
+    if (launch_interface(c.d_bnd + b, 1, c.d_parts, par ^ 1, c.bnd_elems[b], c.stream))
+      return set_error(ARD_ERR_HIP, "interface launch");
+    end(ev);
+  }
+  ev = begin(ARD_PH_RECORD);
+  if (enqueue_cubes(c, par, c.stream)) return set_error(ARD_ERR_HIP, "recorder cube launch");
+  if (launch_record(c.d_rec, nr, c.d_trace, c.trace_cap, c.d_step, c.stream))
+    return set_error(ARD_ERR_HIP, "recorder launch");
+  end(ev);
+  return ARD_OK;
+}
+
+// n steps: each a replay of the step graph of its parity (the launches of
+// enqueue_plain captured once: no per-launch host overhead; rebuilt when the
+// trace buffers move), or, with kernel timing on, direct launches.
+int step_plain(Ctx& c, int n) {
+  if (c.timer.on) {
+    for (int s = 0; s < n; ++s) {
+      int r = enqueue_plain(c, c.time_step & 1, &c.timer);
       if (r) return r;
+      c.time_step++;
+      if (c.timer.used > 2048 && (r = c.timer.collect())) return r;
     }
+    return ARD_OK;
+  }
+  if (c.plain_graph_trace != c.d_trace || !c.graph_exec[0]) {
+    for (int par = 0; par < 2; ++par) {
+      if (c.graph_exec[par]) {
+        hipGraphExecDestroy(c.graph_exec[par]);
+        c.graph_exec[par] = nullptr;
+      }
+      hipGraph_t g;
+      hipError_t e = hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal);
+      if (e != hipSuccess) return hip_fail(e, "hipStreamBeginCapture");
+      int r = enqueue_plain(c, par, nullptr);
+      e = hipStreamEndCapture(c.stream, &g);
+      if (r) return r;
+      if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+      e = hipGraphInstantiate(&c.graph_exec[par], g, nullptr, nullptr, 0);
+      hipGraphDestroy(g);
+      if (e != hipSuccess) return hip_fail(e, "hipGraphInstantiate");
+    }
+    c.plain_graph_trace = c.d_trace;
+  }
+  for (int s = 0; s < n; ++s) {
+    hipError_t e = hipGraphLaunch(c.graph_exec[c.time_step & 1], c.stream);
+    if (e != hipSuccess) return hip_fail(e, "hipGraphLaunch");
+    c.time_step++;
   }
   return ARD_OK;
 }

@@ -283,3 +283,23 @@ def test_fft_transforms_odd_sides_fields(form):
         ref = o.field(i, 0)
         err = np.abs(sim.pressure_field(i) - ref).max() / np.abs(ref).max()
         assert err < TOL, (i, info, err)
+
+
+@pytest.mark.parametrize("form", FORMS)
+def test_step_graph_replays_match_direct_launches(form):
+    """The step graphs (replayed per step) against the same steps launched
+    directly (kernel timing on): identical recorder samples, checked after
+    each chunk of replays with host synchronisation and fetches in between."""
+    sc = small_scene()
+    g = A.Simulation(sc, form=form)
+    d = A.Simulation(sc, form=form)
+    d.set_kernel_timing(True)
+    for chunk in range(6):
+        g.update(7)
+        d.update(7)
+        g.synchronize()
+        d.synchronize()
+        t = 7 * (chunk + 1)
+        assert np.array_equal(g.recorders(0, t), d.recorders(0, t)), chunk
+    for i in range(len(g.partitions())):
+        assert np.array_equal(g.pressure_field(i), d.pressure_field(i)), i
