@@ -194,7 +194,11 @@ def test_hall_matches_fp64_oracle(form):
     sim.close()
     e = rel_l2(tr, ref)
     per_rec = [rel_l2(tr[:, r], ref[:, r]) for r in range(ref.shape[1])]
-    print("hall form %d rel-L2 vs fp64 %.3e (worst recorder %.3e)" % (form, e, max(per_rec)))
+    # the FP32 oracle (the reference's operation order) for scale: its own
+    # error against FP64 on the same trace
+    ref32 = np.load(os.path.join(GOLD, "hall_fp32.npy")).astype(np.float64)
+    print("hall form %d rel-L2 vs fp64 %.3e (worst recorder %.3e); vs fp32 oracle %.3e; fp32 oracle vs fp64 %.3e"
+          % (form, e, max(per_rec), rel_l2(tr, ref32), rel_l2(ref32, ref)))
     assert np.all(np.isfinite(fld)) and np.abs(tr).max() > 0
     assert e <= TOL
 

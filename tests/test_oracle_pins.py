@@ -684,3 +684,18 @@ def test_fft_oracle_reproduces_dense_oracle_goldens():
         ref = np.load(os.path.join(gdir, "two_rooms_%s.npy" % ("fp64" if double else "fp32")))
         got = O.from_scene(sc, double=double, fft=True).step(sc.steps)
         assert np.linalg.norm(got - ref) / np.linalg.norm(ref) < tol
+
+
+def test_hall_goldens_fp32_within_1e4_of_fp64():
+    """The hall's two golden traces (2000 steps, 16 recorders, both by the FFT
+    path): the FP32 oracle, in the reference's operation order, lies within
+    the metric's 1e-4 of the FP64 oracle; the traces carry the pulse (the
+    direct sound reaches every recorder within the window)."""
+    import os
+    gdir = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    r64 = np.load(os.path.join(gdir, "hall_fp64.npy"))
+    r32 = np.load(os.path.join(gdir, "hall_fp32.npy")).astype(np.float64)
+    sc = S.hall()
+    assert r64.shape == r32.shape == (sc.steps, len(sc.recorders))
+    assert np.linalg.norm(r32 - r64) / np.linalg.norm(r64) < 1e-4
+    assert np.all(np.abs(r64).max(axis=0) > 1e-3 * np.abs(r64).max())
