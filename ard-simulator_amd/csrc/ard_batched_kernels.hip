@@ -182,17 +182,21 @@ __device__ __forceinline__ float bload_nt(brsrc r, unsigned voff, unsigned soff)
 // compile time) and walks the chunk's kz planes in pairs (parity known), with
 // the next plane's loads in flight during the current one. z sums stay in
 // registers. The chunk goes in batches of KB planes:
-//   planes   each wave: new modes of its rows -> global, LDS mode tile, its y sums -> LDS
+//   planes   each wave: new modes of its rows -> global and the LDS mode tile
+//   x sums   each wave from its own rows of the tile (no barrier), with the
+//            recorder sums
 //   barrier
-//   reduce   x sums (and recorder sums) of the batch from the mode tile; y sums
-//            of the batch from the mode tile (one (plane, kx) per thread, over
-//            the block's rows); the next batch's per-row and per-plane
-//            factors (x-face spectra, point-force factors, z tables) staged in LDS
+//   y sums   of the batch from the tile (one (plane, kx) per thread, over the
+//            block's rows); the next batch's per-plane factors (x-face
+//            spectra, z tables, point z factors), loaded a batch ahead, to LDS
 //   barrier
 // so the plane loop reads no scalar memory and no LDS atomics (float atomics
-// on LDS serialise).
+// on LDS serialise), and the loads of the next plane stay in flight across
+// the batch's sums and barriers.
+// At most 4 rows per wave the kernel fits 3 waves per SIMD (168 VGPRs); the
+// attribute keeps the compiler there. Taller tiles hold more and get 2.
 template <int RPW, bool PTS>
-__global__ __launch_bounds__(256) void k_modal(const ModalDesc* __restrict__ descs,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RPW <= 4 ? 3 : 1))) void k_modal(const ModalDesc* __restrict__ descs,
                                                const long long* __restrict__ starts, int ndesc, int par) {
   static_assert(RPW % 2 == 0, "rows per wave must be even");
   constexpr int TY = 4 * RPW;  // rows of the block tile
@@ -206,7 +210,8 @@ __global__ __launch_bounds__(256) void k_modal(const ModalDesc* __restrict__ des
   __shared__ __attribute__((aligned(16))) float s_rx[64 * 3];        // x reduction weights of the tile's kx
   __shared__ __attribute__((aligned(16))) float s_gx[KB][TY][8];     // x-face spectra of the batch's rows: even 0..2, odd 4..6
   __shared__ __attribute__((aligned(16))) float s_zt[KB][8];         // z factors of the batch's planes
-  __shared__ __attribute__((aligned(16))) float s_pc[PTS ? KB : 1][TY][NPT];  // point-force y*z factors
+  __shared__ __attribute__((aligned(16))) float s_pty[PTS ? TY : 1][8];  // point y factors of the tile's rows: sources 0..3, recorders 4..7
+  __shared__ __attribute__((aligned(16))) float s_ptz[PTS ? KB : 1][8];  // point z factors of the batch's planes, likewise
   __shared__ __attribute__((aligned(16))) float s_rfx[PTS ? NRC : 1][64];     // recorder x factors
   __shared__ float s_rec[4][NRC];
   const int bid = xcd_block(int(blockIdx.x), int(gridDim.x));
@@ -293,30 +298,45 @@ __global__ __launch_bounds__(256) void k_modal(const ModalDesc* __restrict__ des
   for (int q = 0; q < NRC; ++q) rec_acc[q] = 0.f;
 
   // factors of one batch staged in LDS: x-face spectra (6 of 8 per row),
-  // z tables (8 per plane), point-force y*z factors (4 per row). Loaded into
-  // registers one batch ahead, stored after the batch's first barrier.
-  constexpr int NGX = KB * TY * 8, NZT = KB * 8, NPC = PTS ? KB * TY * NPT : 0;
-  constexpr int NSTG = (NGX + NZT + NPC + 255) / 256;
+  // z tables (8 per plane), point z factors (8 per plane) and point y
+  // factors (8 per row: the same every batch). Loaded into registers one
+  // batch ahead, stored after the batch's first barrier; no arithmetic on
+  // them before then, so no wait for their loads.
+  constexpr int NGX = KB * TY * 8, NZT = KB * 8, NPC = PTS ? KB * 8 : 0, NPY = PTS ? TY * 8 : 0;
+  constexpr int NSTG = (NGX + NZT + NPC + NPY + 255) / 256;
   float stg[NSTG];
+  const gfloat* const gxt = pinned(P.Gx);
+  const gfloat* const ztt = pinned(P.zt);
+  const gfloat* const pzt = pinned(PTS ? P.ptz : P.zt);
+  const gfloat* const pyt = pinned(PTS ? P.pty : P.zt);
+  // one load per slot, every lane, with only its address selected: a load
+  // under a branch would make the compiler wait for all loads in flight at
+  // the branch's join
   auto stage_load = [&](const int kb) {
 #pragma unroll
     for (int t = 0; t < NSTG; ++t) {
       const int e = tid + 256 * t;
-      float v = 0.f;
+      const gfloat* src = ztt;
+      int idx = 0;  // past the staged tables: any valid address, value unused
       if (e < NGX) {
+        // slots 0..2 even spectra, 4..6 odd; slots 3 and 7 are never read
         const int sl = e & 7, row = (e >> 3) % TY, kz = min(kb + e / (8 * TY), kz1 - 1);
         const int ky = min(kyb + row, H - 1);
-        const int q = sl < 3 ? sl : sl - 1;  // slots 0..2 even, 4..6 odd
-        v = (sl != 3 && sl != 7) ? ARD_CG(P.Gx)[(kz * H + ky) * 6 + q] : 0.f;
+        src = gxt;
+        idx = (kz * H + ky) * 6 + (sl < 3 ? sl : sl < 7 ? sl - 1 : 5);
       } else if (e < NGX + NZT) {
-        const int f = e - NGX, kz = min(kb + (f >> 3), kz1 - 1);
-        v = ARD_CG(P.zt)[kz * 8 + (f & 7)];
-      } else if (PTS && nsrc > 0 && e < NGX + NZT + NPC) {  // point tables exist only with points
-        const int f = e - NGX - NZT, q = f % NPT, row = (f / NPT) % TY, kz = min(kb + f / (NPT * TY), kz1 - 1);
-        const int ky = min(kyb + row, H - 1);
-        v = ARD_CG(P.pty)[ky * 8 + q] * ARD_CG(P.ptz)[kz * 8 + q];
+        const int f = e - NGX;
+        idx = min(kb + (f >> 3), kz1 - 1) * 8 + (f & 7);
+      } else if (PTS && e < NGX + NZT + NPC) {
+        const int f = e - NGX - NZT;
+        src = pzt;
+        idx = min(kb + (f >> 3), kz1 - 1) * 8 + (f & 7);
+      } else if (PTS && e < NGX + NZT + NPC + NPY) {
+        const int f = e - NGX - NZT - NPC;
+        src = pyt;
+        idx = min(kyb + (f >> 3), H - 1) * 8 + (f & 7);
       }
-      stg[t] = v;
+      stg[t] = src[idx];
     }
   };
   auto stage_store = [&]() {
@@ -325,7 +345,8 @@ __global__ __launch_bounds__(256) void k_modal(const ModalDesc* __restrict__ des
       const int e = tid + 256 * t;
       if (e < NGX) (&s_gx[0][0][0])[e] = stg[t];
       else if (e < NGX + NZT) (&s_zt[0][0])[e - NGX] = stg[t];
-      else if (PTS && e < NGX + NZT + NPC) (&s_pc[0][0][0])[e - NGX - NZT] = stg[t];
+      else if (PTS && e < NGX + NZT + NPC) (&s_ptz[0][0])[e - NGX - NZT] = stg[t];
+      else if (PTS && e < NGX + NZT + NPC + NPY) (&s_pty[0][0])[e - NGX - NZT - NPC] = stg[t];
     }
   };
 
@@ -345,8 +366,9 @@ __global__ __launch_bounds__(256) void k_modal(const ModalDesc* __restrict__ des
     for (int q = 0; q < 6; ++q) in.gy[q] = bload(rGy, vx, unsigned(q * DW + kz * W) * 4u);
   };
 
-  // one kz plane of this wave's rows
-  auto plane = [&](const PlaneIn& in, const int kz, const int kl, auto pzc) {
+  // one kz plane of this wave's rows; a plane past the batch (live false)
+  // stores nothing and adds zeros
+  auto plane = [&](const PlaneIn& in, const int kz, const int kl, auto pzc, const bool live) {
     constexpr int PZ = decltype(pzc)::value;
     const float4 za = *reinterpret_cast<const float4*>(&s_zt[kl][0]);  // Ez[kz][0..2], Rz[kz][0]
     const float4 zb = *reinterpret_cast<const float4*>(&s_zt[kl][4]);  // Rz[kz][1..2], u_z, 0
@@ -375,14 +397,15 @@ __global__ __launch_bounds__(256) void k_modal(const ModalDesc* __restrict__ des
       fh = fmaf(za.y, gz[r][PZ * 3 + 1], fh);
       fh = fmaf(za.z, gz[r][PZ * 3 + 2], fh);
       if (PTS && nsrc > 0) {
-        const float4 pc = *reinterpret_cast<const float4*>(&s_pc[PTS ? kl : 0][row][0]);
-        fh = fmaf(sfx[0], pc.x, fh);
-        fh = fmaf(sfx[1], pc.y, fh);
-        fh = fmaf(sfx[2], pc.z, fh);
-        fh = fmaf(sfx[3], pc.w, fh);
+        const float4 py = *reinterpret_cast<const float4*>(&s_pty[PTS ? row : 0][0]);
+        const float4 pz = *reinterpret_cast<const float4*>(&s_ptz[PTS ? kl : 0][0]);
+        fh = fmaf(sfx[0] * py.x, pz.x, fh);
+        fh = fmaf(sfx[1] * py.y, pz.y, fh);
+        fh = fmaf(sfx[2] * py.z, pz.z, fh);
+        fh = fmaf(sfx[3] * py.w, pz.w, fh);
       }
       const float mv = fmaf(a, in.mc[r], fmaf(-0.999f, in.mp[r], g * fh));
-      const bool ok = rok[r] && xok;
+      const bool ok = rok[r] && xok && live;
       if (ok) __builtin_nontemporal_store(mv, &Mp[po + unsigned(rom[r]) + unsigned(kx)]);  // streamed, not reused in this step
       const float mn = ok ? mv : 0.f;
       s_m[kl][row][lane] = mn;
@@ -433,15 +456,13 @@ __global__ __launch_bounds__(256) void k_modal(const ModalDesc* __restrict__ des
       ARD_QUAD_SUM(ev0) ARD_QUAD_SUM(ev1) ARD_QUAD_SUM(ev2) ARD_QUAD_SUM(od0) ARD_QUAD_SUM(od1) ARD_QUAD_SUM(od2)
       const int ky = kyb + row;
       if (PTS && nrec > 0) {
-        const int kyc = min(ky, H - 1), kz = min(kb + kl, D - 1);
 #pragma unroll
         for (int q = 0; q < NRC; ++q) {
           if (q < nrec) {
             ARD_QUAD_SUM(rs[q])
             // rows past H and planes past the batch hold no modes of this step
-            const float w = (kl < n && ky < H && qtr == 0)
-                                ? ARD_CG(P.pty)[kyc * 8 + 4 + q] * ARD_CG(P.ptz)[kz * 8 + 4 + q]
-                                : 0.f;
+            const float w =
+                (kl < n && ky < H && qtr == 0) ? s_pty[PTS ? row : 0][4 + q] * s_ptz[PTS ? kl : 0][4 + q] : 0.f;
             rec_acc[q] = fmaf(w, rs[q], rec_acc[q]);
           }
         }
@@ -487,14 +508,16 @@ __global__ __launch_bounds__(256) void k_modal(const ModalDesc* __restrict__ des
     stage_load(kb + KB);  // next batch's factors (past the chunk: clamped, unused)
     for (int kl = 0; kl < n; kl += 2) {
       const int kz = kb + kl;
-      // the next plane's loads go out before this plane's arithmetic
-      // (past the chunk's end they re-read its last plane)
+      // the next plane's loads go out before this plane's arithmetic (past
+      // the chunk's end they re-read its last plane). Both planes of a pair
+      // always run, the second one idle past an odd batch's end: no path
+      // leaves a load in flight that the code after the loop does not
+      // expect, so the next batch's first plane stays in flight through the
+      // x and y sums and the barriers.
       load(nxt, min(kz + 1, kz1 - 1));
-      plane(cur, kz, kl, std::integral_constant<int, 0>());
-      if (kl + 1 < n) {
-        load(cur, min(kz + 2, kz1 - 1));
-        plane(nxt, kz + 1, kl + 1, std::integral_constant<int, 1>());
-      }
+      plane(cur, kz, kl, std::integral_constant<int, 0>(), true);
+      load(cur, min(kz + 2, kz1 - 1));
+      plane(nxt, kz + 1, kl + 1, std::integral_constant<int, 1>(), kl + 1 < n);
     }
     // the x sums read only this wave's rows of the mode tile: no block barrier
     __builtin_amdgcn_wave_barrier();

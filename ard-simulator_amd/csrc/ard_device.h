@@ -27,6 +27,16 @@ typedef __attribute__((address_space(1))) float gfloat;
 typedef __attribute__((address_space(4))) const float cfloat;
 #define ARD_CC(p) ((cfloat*)(p))
 
+// A uniform pointer read once from a descriptor and held in scalar registers
+// from then on: the empty asm makes the value opaque, so the compiler cannot
+// re-read it from the descriptor at each use (a dependent load pair whose
+// wait would also wait for every load issued before it).
+__device__ __forceinline__ const gfloat* pinned(const void* p) {
+  unsigned long long v = reinterpret_cast<unsigned long long>(p);
+  asm volatile("" : "+s"(v));
+  return reinterpret_cast<const gfloat*>(v);
+}
+
 // Raw buffer resource over n bytes (gfx9 word 3: 32-bit float data, no
 // swizzle): loads at byte offsets at or past n return 0.
 typedef float v4f __attribute__((ext_vector_type(4)));
