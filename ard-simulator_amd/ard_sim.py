@@ -57,6 +57,14 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise ArdError("libard.so is not built (run __graft_entry__.build())")
+        # One HIP runtime per process: torch ships its own libamdhip64 (the
+        # soname libard.so needs). Loaded first, it serves both, so torch's
+        # device pointers and streams are valid here; loaded after ROCm's,
+        # it would be a second runtime that finds no GPU.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = ctypes.CDLL(LIB_PATH)
         vp, i, f = ctypes.c_void_p, ctypes.c_int, ctypes.c_float
         fp = ctypes.POINTER(ctypes.c_float)
