@@ -120,7 +120,13 @@ struct BatchedState {
   // fold and forward face transforms, which they share no data with
   hipStream_t side = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  // kMultiSteps steps captured in one graph, per starting parity: consecutive
+  // graph launches leave the GPU idle a few microseconds between them
+  static constexpr int kMultiSteps = 8;
+  hipGraphExec_t graph_multi[2] = {nullptr, nullptr};
   ~BatchedState() {
+    for (auto& g : graph_multi)
+      if (g) hipGraphExecDestroy(g);
     if (ev_fork) hipEventDestroy(ev_fork);
     if (ev_join) hipEventDestroy(ev_join);
     if (side) hipStreamDestroy(side);
@@ -838,24 +844,31 @@ int enqueue_step(Ctx& c, int par, PhaseTimer* T) {
 }
 
 int build_graphs(Ctx& c) {
-  for (int par = 0; par < 2; ++par) {
-    if (c.graph_exec[par]) {
-      hipGraphExecDestroy(c.graph_exec[par]);
-      c.graph_exec[par] = nullptr;
+  BatchedState& B = *c.bs;
+  // per parity: one step, and kMultiSteps steps (an even count, so the graph
+  // ends at the parity it starts with)
+  for (int multi = 0; multi < 2; ++multi) {
+    for (int par = 0; par < 2; ++par) {
+      hipGraphExec_t& ex = multi ? B.graph_multi[par] : c.graph_exec[par];
+      if (ex) {
+        hipGraphExecDestroy(ex);
+        ex = nullptr;
+      }
+      hipGraph_t g;
+      hipError_t e = hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal);
+      if (e != hipSuccess) return hip_fail(e, "hipStreamBeginCapture");
+      int r = ARD_OK;
+      for (int s = 0; s < (multi ? BatchedState::kMultiSteps : 1) && !r; ++s) r = enqueue_step(c, (par + s) & 1, nullptr);
+      e = hipStreamEndCapture(c.stream, &g);
+      if (r) return r;
+      if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+      e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+      hipGraphDestroy(g);
+      if (e != hipSuccess) return hip_fail(e, "hipGraphInstantiate");
     }
-    hipGraph_t g;
-    hipError_t e = hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal);
-    if (e != hipSuccess) return hip_fail(e, "hipStreamBeginCapture");
-    int r = enqueue_step(c, par, nullptr);
-    e = hipStreamEndCapture(c.stream, &g);
-    if (r) return r;
-    if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
-    e = hipGraphInstantiate(&c.graph_exec[par], g, nullptr, nullptr, 0);
-    hipGraphDestroy(g);
-    if (e != hipSuccess) return hip_fail(e, "hipGraphInstantiate");
   }
-  c.bs->graph_trace = c.d_trace;
-  c.bs->graphs_valid = true;
+  B.graph_trace = c.d_trace;
+  B.graphs_valid = true;
   return ARD_OK;
 }
 
@@ -949,11 +962,13 @@ int step_batched(Ctx& c, int n) {
     int r = build_graphs(c);
     if (r) return r;
   }
-  for (int s = 0; s < n; ++s) {
+  for (int s = 0; s < n;) {
     const int par = c.time_step & 1;
-    hipError_t e = hipGraphLaunch(c.graph_exec[par], c.stream);
+    const int k = n - s >= BatchedState::kMultiSteps ? BatchedState::kMultiSteps : 1;
+    hipError_t e = hipGraphLaunch(k > 1 ? B.graph_multi[par] : c.graph_exec[par], c.stream);
     if (e != hipSuccess) return hip_fail(e, "hipGraphLaunch");
-    c.time_step++;
+    c.time_step += k;
+    s += k;
   }
   return ARD_OK;
 }

@@ -291,19 +291,21 @@ def test_fft_transforms_odd_sides_fields(form):
 
 @pytest.mark.parametrize("form", FORMS)
 def test_step_graph_replays_match_direct_launches(form):
-    """The step graphs (replayed per step) against the same steps launched
+    """The step graphs (replayed one step at a time, and the batched form's
+    8-step graphs, from even and odd steps) against the same steps launched
     directly (kernel timing on): identical recorder samples, checked after
     each chunk of replays with host synchronisation and fetches in between."""
     sc = small_scene()
     g = A.Simulation(sc, form=form)
     d = A.Simulation(sc, form=form)
     d.set_kernel_timing(True)
-    for chunk in range(6):
-        g.update(7)
-        d.update(7)
+    t = 0
+    for chunk, n in enumerate((7, 8, 17, 3, 16, 9)):
+        g.update(n)
+        d.update(n)
         g.synchronize()
         d.synchronize()
-        t = 7 * (chunk + 1)
+        t += n
         assert np.array_equal(g.recorders(0, t), d.recorders(0, t)), chunk
     for i in range(len(g.partitions())):
         assert np.array_equal(g.pressure_field(i), d.pressure_field(i)), i
