@@ -369,6 +369,12 @@ class Batch:
     def synchronize(self):
         _check(self.L.ard_synchronize(self.ctx))
 
+    def batched_config(self):
+        """(rows per wave, kz planes per block, transform variant, candidates timed)."""
+        out = (ctypes.c_int * 4)()
+        _check(self.L.ard_batched_config(self.ctx, out))
+        return tuple(out)
+
     def recorders(self, room, t0, nt):
         out = np.zeros((nt, max(self.nrec[room], 1)), dtype=np.float32)
         _check(self.L.ard_batch_fetch_recorders(self.batch, room, _fp(out), t0, nt))
