@@ -138,3 +138,18 @@ def test_folded_bf16_transform_matches_numpy(inverse, b_layout, L, N):
         y = y.T
     err = np.abs(y - ref).max() / np.abs(ref).max()
     assert err < 1e-4, err
+
+
+@pytest.mark.parametrize("shape", [(64, 200, 300), (130, 270, 77), (40, 40, 1000)])
+def test_vector_and_matrix_core_fp32_are_bit_identical(shape):
+    """The FP32 vector-FMA kernel and the FP32 matrix-core kernel accumulate
+    each output's products in K order with one rounding per multiply-add: the
+    same bits, as the recorder traces of the precision study show."""
+    M, K, N = shape
+    rng = np.random.default_rng(M + K + N)
+    a = rng.standard_normal((M, K)).astype(np.float32)
+    b = rng.standard_normal((K, N)).astype(np.float32)
+    c0 = A.selftest_transform(a, b, variant=0)
+    c1 = A.selftest_transform(a, b, variant=1)
+    assert np.array_equal(c0, c1)
+    assert np.abs(c1 - a.astype(np.float64) @ b).max() <= 1e-5 * np.abs(a.astype(np.float64) @ b).max()
