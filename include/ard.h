@@ -96,7 +96,12 @@ int ard_add_source(ard_ctx* ctx, int x, int y, int z, const float* wave, int n);
 /* Recorder at global cell (x,y,z); owner found as Recorder::FindPartition (ARD/recorder.cpp:26-38). */
 int ard_add_recorder(ard_ctx* ctx, int x, int y, int z);
 
-/* Allocates device state and tables. After it, topology calls fail with ARD_ERR_STATE. */
+/* Allocates device state and tables. After it, topology calls fail with ARD_ERR_STATE.
+ * Limits of the batched form (the reference has none; the plain form has none either):
+ * at most 4 point sources in a DCT partition's interior (more than 3 cells from its faces;
+ * sources in the face slabs are not counted), at most 4 recorders per DCT partition, and
+ * fewer than 2^30 cells per DCT partition. Beyond them: ARD_ERR_UNSUPPORTED (split the
+ * partition or use the plain form). */
 int ard_finalize(ard_ctx* ctx);
 
 /* Topology queries. part_info[9] = is_pml, pml_type, parent, x0,y0,z0, w,h,d.
