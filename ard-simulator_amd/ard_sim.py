@@ -19,6 +19,7 @@ LIB_PATH = os.path.join(_HERE, "libard.so")
 ARD_OK = 0
 FORM_PLAIN, FORM_BATCHED = 0, 1
 REC_REFERENCE, REC_LOCAL = 0, 1
+OPT_RING = 2  # ard_set_batched_option
 
 EXPORTED = [
     "ard_ctx_create", "ard_ctx_destroy", "ard_last_error", "ard_set_pml_scale", "ard_add_dct_partition",
@@ -27,11 +28,11 @@ EXPORTED = [
     "ard_boundary_info", "ard_recorder_partition", "ard_step", "ard_time_step", "ard_fetch_recorders",
     "ard_copy_pressure_field", "ard_get_pressure", "ard_set_force", "ard_synchronize", "ard_set_kernel_timing",
     "ard_kernel_times", "ard_selftest_transform", "ard_set_transform_variant", "ard_batched_config",
-    "ard_set_batched_tiling",
+    "ard_set_batched_tiling", "ard_set_batched_option", "ard_batched_option",
     "ard_adjoint", "ard_batch_create", "ard_batch_step", "ard_batch_fetch_recorders", "ard_batch_context",
     "ard_batch_room_partition", "ard_batch_destroy", "ard_set_recorder_cubes", "ard_fetch_recorder_cubes",
     "ard_render_view", "ard_selftest_fold", "ard_dct3d", "ard_dct3d_noalloc", "ard_selftest_fft",
-    "ard_selftest_fold_bf16",
+    "ard_selftest_fold_bf16", "ard_check_finite", "ard_find_boundary",
 ]
 PHASES = ["sources", "pml", "fold", "face_fwd1", "face_fwd2", "modal", "unfold", "face_inv1", "face_inv2",
           "interface", "record", "dct_fwd", "dct_inv"]
@@ -101,6 +102,10 @@ def lib():
         L.ard_dct3d.argtypes = [i, i, i, i, i, vp, vp, f, vp]
         L.ard_batched_config.argtypes = [vp, ip]
         L.ard_set_batched_tiling.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+        L.ard_set_batched_option.argtypes = [vp, i, i]
+        L.ard_check_finite.argtypes = [vp, ip]
+        L.ard_find_boundary.argtypes = [ip, ip, ip]
+        L.ard_batched_option.argtypes = [vp, i, ip]
         L.ard_adjoint.argtypes = [vp, i, fp, fp, fp, i]
         L.ard_batch_create.argtypes = [ctypes.POINTER(vp), i, ctypes.POINTER(vp)]
         L.ard_batch_step.argtypes = [vp, i]
@@ -145,6 +150,17 @@ def selftest_fold(X, inverse=False, b_layout=0, device=0, entry="ard_selftest_fo
     Y = np.zeros_like(X)
     _check(getattr(lib(), entry)(device, int(inverse), b_layout, L, N, _fp(X), _fp(Y)))
     return Y
+
+
+def find_boundary(box_a, box_b):
+    """Boundary::FindBoundary on two boxes (x0, y0, z0, w, h, d) in cells:
+    (type, xs, xe, ys, ye, zs, ze) or None (ard_find_boundary)."""
+    a = (ctypes.c_int * 6)(*[int(v) for v in box_a])
+    b = (ctypes.c_int * 6)(*[int(v) for v in box_b])
+    out = (ctypes.c_int * 9)()
+    r = lib().ard_find_boundary(a, b, out)
+    _check(r)
+    return None if r == 0 else (out[0],) + tuple(out[3:9])
 
 
 def selftest_transform(A, B, variant=1, b_layout=0, device=0):
@@ -318,6 +334,22 @@ class Simulation:
         out = (ctypes.c_int * 4)()
         _check(self.L.ard_batched_config(self.ctx, out))
         return tuple(out)
+
+    def check_finite(self):
+        """Lowest partition id whose state holds a NaN or an infinity, or -1 (ard_check_finite)."""
+        out = ctypes.c_int(0)
+        _check(self.L.ard_check_finite(self.ctx, ctypes.byref(out)))
+        return out.value
+
+    def set_batched_option(self, option, value):
+        """Sets a schedule option of a finalized batched context (OPT_RING)."""
+        _check(self.L.ard_set_batched_option(self.ctx, int(option), int(value)))
+
+    def batched_option(self, option):
+        """A schedule option of a finalized batched context (OPT_RING)."""
+        out = ctypes.c_int(0)
+        _check(self.L.ard_batched_option(self.ctx, int(option), ctypes.byref(out)))
+        return out.value
 
 
 class Batch:

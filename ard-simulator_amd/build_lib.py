@@ -2,6 +2,7 @@
 
 Usage: python build_lib.py   (called by __graft_entry__.build()).
 """
+import concurrent.futures
 import os
 import subprocess
 import sys
@@ -13,7 +14,7 @@ OBJ = os.path.join(HERE, "_build")
 
 HIP_SOURCES = ["ard_kernels.hip", "ard_batched_kernels.hip", "ard_batched.hip", "ard_ctx.cpp", "ard_plain.cpp",
                "ard_adjoint_kernels.hip", "ard_adjoint.cpp", "ard_batch.cpp", "ard_output.hip",
-               "ard_view.hip", "ard_dct3d.hip", "ard_fft.hip"]
+               "ard_view.hip", "ard_dct3d.hip", "ard_fft.hip", "ard_check.hip"]
 CXX_SOURCES = ["ard_topology.cpp", "ard_masks.cpp"]
 HOST_SOURCES = ["host/ard_host.cpp"]
 
@@ -35,21 +36,26 @@ def run(cmd):
     return out
 
 
-def build(verbose=False):
+def build(verbose=False, jobs=None):
     os.makedirs(OBJ, exist_ok=True)
-    objs = []
+    cmds, objs = [], []
     for src in HIP_SOURCES:
         obj = os.path.join(OBJ, src + ".o")
-        run([HIPCC, "-x", "hip", "-c", os.path.join(CSRC, src), "-o", obj] + COMMON + DEVICE)
+        cmds.append([HIPCC, "-x", "hip", "-c", os.path.join(CSRC, src), "-o", obj] + COMMON + DEVICE)
         objs.append(obj)
     for src in CXX_SOURCES:
         obj = os.path.join(OBJ, src + ".o")
-        run(["g++", "-c", os.path.join(CSRC, src), "-o", obj] + COMMON)
+        cmds.append(["g++", "-c", os.path.join(CSRC, src), "-o", obj] + COMMON)
         objs.append(obj)
     for src in HOST_SOURCES:  # the C++ class mirror, over the C-ABI
         obj = os.path.join(OBJ, os.path.basename(src) + ".o")
-        run(["g++", "-c", os.path.join(HERE, src), "-o", obj] + COMMON)
+        cmds.append(["g++", "-c", os.path.join(HERE, src), "-o", obj] + COMMON)
         objs.append(obj)
+    # one compiler process per source, at most `jobs` (default: CPUs, up to 8) at a time
+    jobs = jobs or min(8, os.cpu_count() or 1)
+    with concurrent.futures.ThreadPoolExecutor(max_workers=jobs) as ex:
+        for _ in ex.map(run, cmds):
+            pass
     tmp = OUT + ".tmp"
     run([HIPCC, "-shared", "-o", tmp] + objs + ["--offload-arch=" + ARCH, "-fPIC"])
     os.replace(tmp, OUT)

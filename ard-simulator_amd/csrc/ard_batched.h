@@ -92,8 +92,9 @@ struct BRecDesc {
 int launch_fold(const FoldDesc* d, const long long* starts, int n, long long total, hipStream_t s);
 int launch_unfold(const UnfoldDesc* d, const long long* starts, int n, long long total, hipStream_t s);
 int launch_bsources(const BSourceDesc* d, int n, const int* d_step, float courant, hipStream_t s);
-int launch_modal(int rpw, bool pts, const ModalDesc* d, const long long* starts, int n, long long blocks, int par,
-                 hipStream_t s);
+// ring: planes of mode rows in flight per wave through LDS (0: one, in registers; 2 or 3 with 4 rows per wave)
+int launch_modal(int rpw, bool pts, int ring, const ModalDesc* d, const long long* starts, int n, long long blocks,
+                 int par, hipStream_t s);
 int launch_brecord(const BRecDesc* d, int n, float* trace, int cap, int* d_step, hipStream_t s);
 
 }  // namespace ard

@@ -14,7 +14,6 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
-#include <map>
 #include <memory>
 #include <string>
 #include <vector>
@@ -79,28 +78,40 @@ struct PointSrc {
 struct BatchedState {
   std::vector<BPart> parts;  // by partition id (DCT entries used)
   std::vector<PointSrc> srcs;
-  // device descriptor tables
-  FoldDesc* d_fold = nullptr; long long* d_fold_start = nullptr; int nfold = 0; long long fold_total = 0;
-  UnfoldDesc* d_unfold = nullptr; long long* d_unfold_start = nullptr; int nunfold = 0; long long unfold_total = 0;
-  GemmDesc* d_gemm[4] = {};   // FT1, FT2, IT1, IT2
-  int ngemm[4] = {}, gemm_tiles[4] = {};
-  GemmDesc* d_gemm_fold[4] = {};  // the same passes even/odd folded (transform variant 3)
-  int2* d_fold_tiles[4] = {};      // their tile tables (fold_tile_table)
-  int gemm_fold_tiles[4] = {};
-  GemmDesc* d_gemm_bf16[4] = {};  // the folded passes in BF16 split precision (variant 2), d_fold_tiles' tiles
-  GemmDesc* d_gemm_fft[4] = {};   // the same passes FFT-style (variant 4), when every length factors
-  int gemm_fft_tiles[4] = {};
-  size_t fft_lds[4] = {};
-  bool has_fft = false;
   struct ModalGroup {
-    int rpw;
-    bool pts;
+    int rpw = 4;
+    int ring = 0;  // planes of mode rows in flight per wave through LDS (0: one, in registers)
+    bool pts = true;
     ModalDesc* d_desc[2] = {};  // same table for both parities (the kernel takes the parity)
     long long* d_start = nullptr;
     int n = 0;
     long long blocks = 0;
   };
-  std::vector<ModalGroup> modal;
+  // Launch tables of the DCT partitions' phases: fold, the four face passes
+  // in every transform kernel, the modal pass and the unfold.
+  struct Stage {
+    FoldDesc* fold = nullptr; long long* fold_start = nullptr; int nfold = 0; long long fold_total = 0;
+    UnfoldDesc* unfold = nullptr; long long* unfold_start = nullptr; int nunfold = 0; long long unfold_total = 0;
+    GemmDesc* gemm[4] = {};        // FT1, FT2, IT1, IT2 (dense kernels, variants 0 and 1)
+    int ngemm[4] = {}, gemm_tiles[4] = {};
+    GemmDesc* gemm_fold[4] = {};   // the same passes even/odd folded (transform variant 3)
+    int2* fold_tiles[4] = {};      // their tile tables (fold_tile_table)
+    int gemm_fold_tiles[4] = {};
+    GemmDesc* gemm_bf16[4] = {};   // the folded passes in BF16 split precision (variant 2), fold_tiles' tiles
+    GemmDesc* gemm_fft[4] = {};    // the same passes FFT-style (variant 4), when every length factors
+    int gemm_fft_tiles[4] = {};
+    ModalGroup modal;
+  };
+  Stage stage;
+  // host copies of the per-partition descriptors in descriptor order (three
+  // per DCT partition for fold, unfold and each face pass; one modal)
+  std::vector<int> dct_ids;
+  std::vector<FoldDesc> h_fold;
+  std::vector<UnfoldDesc> h_unfold;
+  std::vector<GemmDesc> h_gemm[4], h_gemm_fold[4], h_gemm_bf16[4], h_gemm_fft[4];
+  std::vector<ModalDesc> h_modal;
+  size_t fft_lds[4] = {};
+  bool has_fft = false;
   PmlDesc* d_pml = nullptr; long long pml_blocks = 0; int npml = 0;
   BndDesc* d_bnd = nullptr; long long bnd_total = 0; int nbnd = 0;
   PartAccess* d_parts = nullptr;
@@ -116,6 +127,7 @@ struct BatchedState {
   bool graphs_valid = false;
   // tiling of the modal pass and transform kernel, chosen at finalize
   int rpw = 4, kz_chunk = 48, gemm_variant = 1, tuned = 0;
+  int ring = 0;  // modal pass: planes of mode rows in flight per wave through LDS (4 rows per wave)
   // second stream and events: in the step graph the PML slabs run beside the
   // fold and forward face transforms, which they share no data with
   hipStream_t side = nullptr;
@@ -314,18 +326,18 @@ int fold_tile_table(Ctx& c, const std::vector<GemmDesc>& v, int2** d) {
   return upload_vec(c, (void**)d, t.data(), t.size() * sizeof(int2));
 }
 
-// One face-transform launch k (0, 1 forward; 2, 3 inverse) with transform
-// kernel v: 3 the even/odd folded matrix-core passes, 4 the FFT-style passes,
-// else the dense passes of launch_gemm (0, 1, 2).
-int launch_face(const BatchedState& B, int k, int v, hipStream_t st) {
+// One face-transform launch k (0, 1 forward; 2, 3 inverse) of a group of
+// partitions with transform kernel v: 3 the even/odd folded matrix-core
+// passes, 4 the FFT-style passes, else the dense passes of launch_gemm (0, 1, 2).
+int launch_face(const BatchedState& B, const BatchedState::Stage& S, int k, int v, hipStream_t st) {
   if (v == 3)
-    return launch_gemm_fold(B.d_gemm_fold[k], B.d_fold_tiles[k], B.gemm_fold_tiles[k], k % 2 == 0, k < 2 ? 1 : 2, st);
+    return launch_gemm_fold(S.gemm_fold[k], S.fold_tiles[k], S.gemm_fold_tiles[k], k % 2 == 0, k < 2 ? 1 : 2, st);
   if (v == 4)
-    return launch_fft(B.d_gemm_fft[k], B.ngemm[k], B.gemm_fft_tiles[k], k % 2 == 0, k % 2 == 0, 0, B.fft_lds[k], st);
+    return launch_fft(S.gemm_fft[k], S.ngemm[k], S.gemm_fft_tiles[k], k % 2 == 0, k % 2 == 0, 0, B.fft_lds[k], st);
   if (v == 2)
-    return launch_gemm_bf16_fold(B.d_gemm_bf16[k], B.d_fold_tiles[k], B.gemm_fold_tiles[k], k % 2 == 0, k < 2 ? 1 : 2,
+    return launch_gemm_bf16_fold(S.gemm_bf16[k], S.fold_tiles[k], S.gemm_fold_tiles[k], k % 2 == 0, k < 2 ? 1 : 2,
                                  st);
-  return launch_gemm(B.d_gemm[k], B.ngemm[k], B.gemm_tiles[k], k % 2 == 0, k % 2 == 0, 0, st, v);
+  return launch_gemm(S.gemm[k], S.ngemm[k], S.gemm_tiles[k], k % 2 == 0, k % 2 == 0, 0, st, v);
 }
 
 int upload_gemms(Ctx& c, std::vector<GemmDesc>& v, GemmDesc** d, int* tiles) {
@@ -336,6 +348,78 @@ int upload_gemms(Ctx& c, std::vector<GemmDesc>& v, GemmDesc** d, int* tiles) {
   }
   *tiles = t;
   return upload_vec(c, (void**)d, v.data(), v.size() * sizeof(GemmDesc));
+}
+
+// Uploads the stage's launch tables that depend on the geometry only: fold
+// and the four face passes in every transform kernel.
+int build_stage(Ctx& c) {
+  BatchedState& B = *c.bs;
+  BatchedState::Stage& S = B.stage;
+  int r;
+  if (B.dct_ids.empty()) return ARD_OK;  // nothing to launch
+  std::vector<long long> st;
+  long long t = 0;
+  for (auto& x : B.h_fold) { st.push_back(t); t += ceil_div(3 * x.plane / x.vec, 256) * 256; }
+  S.nfold = int(B.h_fold.size());
+  S.fold_total = t;
+  if ((r = upload_vec(c, (void**)&S.fold, B.h_fold.data(), B.h_fold.size() * sizeof(FoldDesc)))) return r;
+  if ((r = upload_vec(c, (void**)&S.fold_start, st.data(), st.size() * 8))) return r;
+  for (int k = 0; k < 4; ++k) {
+    std::vector<GemmDesc> g = B.h_gemm[k];
+    S.ngemm[k] = int(g.size());
+    if ((r = upload_gemms(c, g, &S.gemm[k], &S.gemm_tiles[k]))) return r;
+    std::vector<GemmDesc> gf = B.h_gemm_fold[k];
+    if ((r = upload_gemms(c, gf, &S.gemm_fold[k], &S.gemm_fold_tiles[k]))) return r;
+    if ((r = fold_tile_table(c, gf, &S.fold_tiles[k]))) return r;
+    const std::vector<GemmDesc>& gb = B.h_gemm_bf16[k];
+    if ((r = upload_vec(c, (void**)&S.gemm_bf16[k], gb.data(), gb.size() * sizeof(GemmDesc)))) return r;
+    if (B.has_fft) {
+      std::vector<GemmDesc> ff = B.h_gemm_fft[k];
+      int tf = 0;
+      for (GemmDesc& x : ff) {
+        x.tile0 = tf;
+        tf += fft_tiles(x);
+      }
+      S.gemm_fft_tiles[k] = tf;
+      if ((r = upload_vec(c, (void**)&S.gemm_fft[k], ff.data(), ff.size() * sizeof(GemmDesc)))) return r;
+    }
+  }
+  return ARD_OK;
+}
+
+// Uploads the stage's tables that depend on the modal tiling and the point
+// forces: the modal pass and the unfold.
+int build_stage_tiling(Ctx& c) {
+  BatchedState& B = *c.bs;
+  BatchedState::Stage& S = B.stage;
+  int r;
+  if (B.dct_ids.empty()) return ARD_OK;
+  std::vector<long long> su;
+  long long t = 0;
+  for (auto& x : B.h_unfold) { su.push_back(t); t += ceil_div(3 * x.plane / x.vec, 256) * 256; }
+  S.nunfold = int(B.h_unfold.size());
+  S.unfold_total = t;
+  if ((r = upload_vec(c, (void**)&S.unfold, B.h_unfold.data(), B.h_unfold.size() * sizeof(UnfoldDesc)))) return r;
+  if ((r = upload_vec(c, (void**)&S.unfold_start, su.data(), su.size() * 8))) return r;
+  // one launch for all partitions: the point terms are uniform branches, and
+  // one grid avoids the tail of a second, smaller launch
+  BatchedState::ModalGroup g;
+  g.rpw = B.rpw;
+  g.ring = g.rpw == 4 ? B.ring : 0;
+  g.pts = true;
+  std::vector<long long> starts;
+  long long blocks = 0;
+  for (const ModalDesc& d : B.h_modal) {
+    starts.push_back(blocks);
+    blocks += (long long)d.nbx * d.nby * d.nbz;
+  }
+  g.n = int(B.h_modal.size());
+  g.blocks = blocks;
+  if ((r = upload_vec(c, (void**)&g.d_desc[0], B.h_modal.data(), B.h_modal.size() * sizeof(ModalDesc)))) return r;
+  g.d_desc[1] = g.d_desc[0];
+  if ((r = upload_vec(c, (void**)&g.d_start, starts.data(), starts.size() * 8))) return r;
+  S.modal = g;
+  return ARD_OK;
 }
 
 int build_sources(Ctx& c) {
@@ -384,7 +468,7 @@ int build_sources(Ctx& c) {
 int build_modal(Ctx& c) {
   BatchedState& B = *c.bs;
   int r;
-  std::map<std::pair<int, int>, std::vector<ModalDesc>> groups;  // (rows per wave, has points)
+  B.h_modal.clear();
   for (int i = 0; i < int(c.topo.parts.size()); ++i) {
     const PartGeom& p = c.topo.parts[i];
     if (p.is_pml) continue;
@@ -458,29 +542,9 @@ int build_modal(Ctx& c) {
       d.pty = dy;
       d.ptz = dz;
     }
-    // one launch for all partitions: the point terms are uniform branches, and
-    // one grid avoids the tail of a second, smaller launch
-    groups[std::make_pair(P.rpw, 1)].push_back(d);
+    B.h_modal.push_back(d);
   }
-  B.modal.clear();
-  for (auto& kv : groups) {
-    BatchedState::ModalGroup g;
-    g.rpw = kv.first.first;
-    g.pts = kv.first.second != 0;
-    std::vector<long long> starts;
-    long long blocks = 0;
-    for (auto& d : kv.second) {
-      starts.push_back(blocks);
-      blocks += (long long)d.nbx * d.nby * d.nbz;
-    }
-    g.n = int(kv.second.size());
-    g.blocks = blocks;
-    if ((r = upload_vec(c, (void**)&g.d_desc[0], kv.second.data(), kv.second.size() * sizeof(ModalDesc)))) return r;
-    g.d_desc[1] = g.d_desc[0];
-    if ((r = upload_vec(c, (void**)&g.d_start, starts.data(), starts.size() * 8))) return r;
-    B.modal.push_back(g);
-  }
-  return ARD_OK;
+  return build_stage_tiling(c);
 }
 
 // Last-writer masks of the grouped interface launch (final_writer_masks),
@@ -512,11 +576,13 @@ int build_masks(Ctx& c, std::vector<BndDesc>& descs, const std::vector<long long
 int apply_tiling(Ctx& c, int rpw, int kzc) {
   BatchedState& B = *c.bs;
   int r;
-  std::vector<UnfoldDesc> unfolds;
-  for (int i = 0; i < int(c.topo.parts.size()); ++i) {
-    const PartGeom& p = c.topo.parts[i];
-    if (p.is_pml) continue;
-    BPart& P = B.parts[i];
+  B.rpw = rpw;
+  B.kz_chunk = kzc;
+  std::vector<UnfoldDesc>& unfolds = B.h_unfold;
+  unfolds.clear();
+  for (int i : B.dct_ids) {
+    const PartGeom& p = c.topo.parts[size_t(i)];
+    BPart& P = B.parts[size_t(i)];
     P.rpw = rpw;
     P.nbx = int(ceil_div(p.w, 64));
     P.nby = int(ceil_div(p.h, 4 * rpw));
@@ -527,14 +593,7 @@ int apply_tiling(Ctx& c, int rpw, int kzc) {
       unfolds.push_back(UnfoldDesc{P.T[w], P.S[w], P.plane[w], w == 0 ? P.nbx : (w == 1 ? P.nby : P.nbz),
                                    vec_of(P.plane[w], {P.T[w], P.S[w]})});
   }
-  std::vector<long long> su;
-  long long t = 0;
-  for (auto& u : unfolds) { su.push_back(t); t += ceil_div(3 * u.plane / u.vec, 256) * 256; }
-  B.nunfold = int(unfolds.size());
-  B.unfold_total = t;
-  if ((r = upload_vec(c, (void**)&B.d_unfold, unfolds.data(), unfolds.size() * sizeof(UnfoldDesc)))) return r;
-  if ((r = upload_vec(c, (void**)&B.d_unfold_start, su.data(), su.size() * 8))) return r;
-  if ((r = build_modal(c))) return r;
+  if ((r = build_modal(c))) return r;  // and the stage's modal and unfold tables
   std::vector<BRecDesc> rd;
   for (size_t q = 0; q < c.topo.recorders.size(); ++q) {
     const BPart& P = B.parts[c.topo.recorders[q].part];
@@ -542,8 +601,6 @@ int apply_tiling(Ctx& c, int rpw, int kzc) {
     rd.push_back(BRecDesc{P.rec_part + slot, P.nbx * P.nby * P.nbz});
   }
   if (!rd.empty() && (r = upload_vec(c, (void**)&B.d_rec, rd.data(), rd.size() * sizeof(BRecDesc)))) return r;
-  B.rpw = rpw;
-  B.kz_chunk = kzc;
   B.graphs_valid = false;
   hipError_t e = hipStreamSynchronize(c.stream);
   return e == hipSuccess ? ARD_OK : hip_fail(e, "hipStreamSynchronize");
@@ -564,12 +621,12 @@ int finalize_batched(Ctx& c) {
   const int np = int(c.topo.parts.size());
   B.parts.assign(np, BPart());
   for (size_t i = 0; i < c.topo.recorders.size(); ++i) B.parts[c.topo.recorders[i].part].recs.push_back(int(i));
-  std::vector<FoldDesc> folds;
-  std::vector<UnfoldDesc> unfolds;
-  std::vector<GemmDesc> gm[4];
+  std::vector<FoldDesc>& folds = B.h_fold;
+  std::vector<GemmDesc>* gm = B.h_gemm;
   for (int i = 0; i < np; ++i) {
     const PartGeom& p = c.topo.parts[i];
     if (p.is_pml) continue;
+    B.dct_ids.push_back(i);
     BPart& P = B.parts[i];
     if (int(P.recs.size()) > ARD_MAX_REC_PER_PART)
       return set_error(ARD_ERR_UNSUPPORTED, "more than 4 recorders in one partition");
@@ -674,7 +731,6 @@ int finalize_batched(Ctx& c) {
     gm[3].push_back(gemm_n_contig(ih, p.h, P.U[2], P.P[2], p.w));
   }
   for (int k = 0; k < 4; ++k) {
-    B.ngemm[k] = int(gm[k].size());
     // the folded twin of each pass: forward passes (k = 0, 1) fold the input,
     // inverse passes (k = 2, 3) produce the output pair
     std::vector<GemmDesc> gf = gm[k];
@@ -696,9 +752,7 @@ int finalize_batched(Ctx& c) {
       }
       g.tiles_n = int(ceil_div(g.N, 64));
     }
-    if ((r = upload_gemms(c, gm[k], &B.d_gemm[k], &B.gemm_tiles[k]))) return r;
-    if ((r = upload_gemms(c, gf, &B.d_gemm_fold[k], &B.gemm_fold_tiles[k]))) return r;
-    if ((r = fold_tile_table(c, gf, &B.d_fold_tiles[k]))) return r;
+    B.h_gemm_fold[k] = gf;
     // and their BF16 split twins: the tables' high and low parts
     std::vector<GemmDesc> gb = gf;
     for (GemmDesc& g : gb) {
@@ -707,38 +761,21 @@ int finalize_batched(Ctx& c) {
       g.A = static_cast<const float*>(k < 2 ? t[0] : t[2]);
       g.Alo = k < 2 ? t[1] : t[3];
     }
-    if (!gb.empty() && (r = upload_vec(c, (void**)&B.d_gemm_bf16[k], gb.data(), gb.size() * sizeof(GemmDesc)))) return r;
+    B.h_gemm_bf16[k] = gb;
   }
   // the FFT-style twins of the dense passes (variant 4): only if every face
   // axis length factors into radices up to 13
   B.has_fft = true;
-  {
-    std::vector<GemmDesc> ff[4];
-    for (int k = 0; k < 4 && B.has_fft; ++k) {
-      int t = 0;
-      for (GemmDesc g : gm[k]) {
-        if (fft_setup(c, g, k >= 2) != ARD_OK) { B.has_fft = false; break; }
-        g.tile0 = t;
-        t += fft_tiles(g);
-        B.fft_lds[k] = std::max(B.fft_lds[k], fft_lds_bytes(g));
-        ff[k].push_back(g);
-      }
-      B.gemm_fft_tiles[k] = t;
+  for (int k = 0; k < 4 && B.has_fft; ++k) {
+    B.h_gemm_fft[k].clear();
+    for (GemmDesc g : gm[k]) {
+      if (fft_setup(c, g, k >= 2) != ARD_OK) { B.has_fft = false; break; }
+      B.fft_lds[k] = std::max(B.fft_lds[k], fft_lds_bytes(g));
+      B.h_gemm_fft[k].push_back(g);
     }
-    for (int k = 0; k < 4 && B.has_fft; ++k)
-      if (!ff[k].empty() && (r = upload_vec(c, (void**)&B.d_gemm_fft[k], ff[k].data(), ff[k].size() * sizeof(GemmDesc))))
-        return r;
   }
   if (B.gemm_variant == 4 && !B.has_fft) B.gemm_variant = 1;
-  {
-    std::vector<long long> st;
-    long long t = 0;
-    for (auto& f : folds) { st.push_back(t); t += ceil_div(3 * f.plane / f.vec, 256) * 256; }
-    B.nfold = int(folds.size());
-    B.fold_total = t;
-    if ((r = upload_vec(c, (void**)&B.d_fold, folds.data(), folds.size() * sizeof(FoldDesc)))) return r;
-    if ((r = upload_vec(c, (void**)&B.d_fold_start, st.data(), st.size() * 8))) return r;
-  }
+  if ((r = build_stage(c))) return r;
   // PML slabs: one grouped launch
   {
     std::vector<PmlDesc> pd = c.h_pml;
@@ -801,46 +838,72 @@ int enqueue_step(Ctx& c, int par, PhaseTimer* T) {
   BatchedState& B = *c.bs;
   hipStream_t st = c.stream;
   int r = 0;
-  auto phase = [&](int ph, auto&& launch) {
-    const int ev = T ? T->begin(st, ph) : -1;
+  auto phase = [&](int ph, hipStream_t s, auto&& launch) {
+    const int ev = T ? T->begin(s, ph) : -1;
     r |= launch();
-    if (T) T->end(st, ev);
+    if (T) T->end(s, ev);
   };
-  phase(ARD_PH_SOURCES, [&] { return launch_bsources(B.d_src, B.nsrc, c.d_step, c.courant, st); });
+  auto ok = [&](hipError_t e) { r |= e != hipSuccess; };
+  phase(ARD_PH_SOURCES, st, [&] { return launch_bsources(B.d_src, B.nsrc, c.d_step, c.courant, st); });
   // captured: PML on the side stream, joined before the interface (which
   // reads the new PML pressure and writes PML forces); timed: all in order on
-  // one stream. Forking after the fold or after the first forward transform
-  // was slower (the fold is stretched less, but the PML then meets the modal
-  // pass or loses its overlap).
+  // one stream, phase by phase over all partitions.
   const bool fork = !T && B.side;
-  if (fork) r |= hipEventRecord(B.ev_fork, st) != hipSuccess;
+  const BatchedState::Stage& S = B.stage;
+  if (fork) ok(hipEventRecord(B.ev_fork, st));
   // the fold is enqueued first, so that its blocks are dispatched before the
-  // PML slabs' fill the CUs; the PML still depends on the sources only
-  phase(ARD_PH_FOLD, [&] { return launch_fold(B.d_fold, B.d_fold_start, B.nfold, B.fold_total, st); });
-  if (fork) r |= hipStreamWaitEvent(B.side, B.ev_fork, 0) != hipSuccess;
-  hipStream_t sp = fork ? B.side : st;
-  phase(ARD_PH_PML, [&] {
-    return launch_pml(B.d_pml, B.npml, B.pml_blocks, par, c.prm.pml_layers, c.prm.h, c.prm.dt, c.prm.c0, sp);
-  });
-  if (fork) r |= hipEventRecord(B.ev_join, B.side) != hipSuccess;
-  auto face = [&](int k) { return launch_face(B, k, B.gemm_variant, st); };
-  phase(ARD_PH_FACE_FWD1, [&] { return face(0); });
-  phase(ARD_PH_FACE_FWD2, [&] { return face(1); });
-  for (auto& g : B.modal)
-    phase(ARD_PH_MODAL, [&] { return launch_modal(g.rpw, g.pts, g.d_desc[par], g.d_start, g.n, g.blocks, par, st); });
-  phase(ARD_PH_UNFOLD, [&] { return launch_unfold(B.d_unfold, B.d_unfold_start, B.nunfold, B.unfold_total, st); });
-  phase(ARD_PH_FACE_INV1, [&] { return face(2); });
-  phase(ARD_PH_FACE_INV2, [&] { return face(3); });
+  // PML slabs' fill the CUs; the PML still depends on the sources only.
+  // Forking after the fold or after the first forward transform was slower
+  // (the fold is stretched less, but the PML then meets the modal pass or
+  // loses its overlap).
+  phase(ARD_PH_FOLD, st, [&] { return launch_fold(S.fold, S.fold_start, S.nfold, S.fold_total, st); });
+  if (fork) ok(hipStreamWaitEvent(B.side, B.ev_fork, 0));
+  {
+    hipStream_t sp = fork ? B.side : st;
+    phase(ARD_PH_PML, sp, [&] {
+      return launch_pml(B.d_pml, B.npml, B.pml_blocks, par, c.prm.pml_layers, c.prm.h, c.prm.dt, c.prm.c0, sp);
+    });
+  }
+  if (fork) ok(hipEventRecord(B.ev_join, B.side));
+  auto face = [&](int k) {
+    static const int ph[4] = {ARD_PH_FACE_FWD1, ARD_PH_FACE_FWD2, ARD_PH_FACE_INV1, ARD_PH_FACE_INV2};
+    phase(ph[k], st, [&] { return launch_face(B, S, k, B.gemm_variant, st); });
+  };
+  face(0);
+  face(1);
+  {
+    const BatchedState::ModalGroup& g = S.modal;
+    phase(ARD_PH_MODAL, st, [&] { return launch_modal(g.rpw, g.pts, g.ring, g.d_desc[par], g.d_start, g.n, g.blocks, par, st); });
+  }
+  phase(ARD_PH_UNFOLD, st, [&] { return launch_unfold(S.unfold, S.unfold_start, S.nunfold, S.unfold_total, st); });
+  face(2);
+  face(3);
   // the PML stencil overlaps the fold and the face transforms and is joined
   // here, at its first consumer (long finished by then, so the join costs the
-  // step no wait): the interface reads the PML pressure just written, buffer par ^ 1
-  if (fork) r |= hipStreamWaitEvent(st, B.ev_join, 0) != hipSuccess;
-  phase(ARD_PH_INTERFACE, [&] { return launch_interface(B.d_bnd, B.nbnd, B.d_parts, par ^ 1, B.bnd_total, st); });
-  phase(ARD_PH_RECORD, [&] {
+  // step no wait)
+  if (fork) ok(hipStreamWaitEvent(st, B.ev_join, 0));
+  // the interface reads the PML pressure just written, buffer par ^ 1
+  phase(ARD_PH_INTERFACE, st, [&] { return launch_interface(B.d_bnd, B.nbnd, B.d_parts, par ^ 1, B.bnd_total, st); });
+  phase(ARD_PH_RECORD, st, [&] {
     if (enqueue_cubes(c, par, st)) return 1;
     return launch_brecord(B.d_rec, int(c.topo.recorders.size()), c.d_trace, c.trace_cap, c.d_step, st);
   });
   return r ? set_error(ARD_ERR_HIP, "batched step launch") : ARD_OK;
+}
+
+// Captures nsteps steps from step parity par into an executable graph.
+int capture_steps(Ctx& c, int par, int nsteps, hipGraphExec_t* ex) {
+  hipGraph_t g;
+  hipError_t e = hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal);
+  if (e != hipSuccess) return hip_fail(e, "hipStreamBeginCapture");
+  int r = ARD_OK;
+  for (int s = 0; s < nsteps && !r; ++s) r = enqueue_step(c, (par + s) & 1, nullptr);
+  e = hipStreamEndCapture(c.stream, &g);
+  if (r) return r;
+  if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+  e = hipGraphInstantiate(ex, g, nullptr, nullptr, 0);
+  hipGraphDestroy(g);
+  return e == hipSuccess ? ARD_OK : hip_fail(e, "hipGraphInstantiate");
 }
 
 int build_graphs(Ctx& c) {
@@ -854,17 +917,8 @@ int build_graphs(Ctx& c) {
         hipGraphExecDestroy(ex);
         ex = nullptr;
       }
-      hipGraph_t g;
-      hipError_t e = hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal);
-      if (e != hipSuccess) return hip_fail(e, "hipStreamBeginCapture");
-      int r = ARD_OK;
-      for (int s = 0; s < (multi ? BatchedState::kMultiSteps : 1) && !r; ++s) r = enqueue_step(c, (par + s) & 1, nullptr);
-      e = hipStreamEndCapture(c.stream, &g);
+      int r = capture_steps(c, par, multi ? BatchedState::kMultiSteps : 1, &ex);
       if (r) return r;
-      if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
-      e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
-      hipGraphDestroy(g);
-      if (e != hipSuccess) return hip_fail(e, "hipGraphInstantiate");
     }
   }
   B.graph_trace = c.d_trace;
@@ -895,21 +949,29 @@ int autotune(Ctx& c) {
     hipEventElapsedTime(&ms, e0, e1);
     return ms / reps;
   };
-  int best_rpw = 4, best_kzc = 48, tried = 0;
+  int best_rpw = 4, best_kzc = 48, best_ring = 0, tried = 0;
   float best = 1e30f;
   int r = ARD_OK;
   for (int rpw : {2, 4, 6, 8}) {
     for (int kzc : {16, 32, 48, 96}) {
-      if ((r = apply_tiling(c, rpw, kzc))) break;
-      const float ms = time_ms([&] {
-        for (auto& g : B.modal) launch_modal(g.rpw, g.pts, g.d_desc[0], g.d_start, g.n, g.blocks, 0, c.stream);
-        launch_unfold(B.d_unfold, B.d_unfold_start, B.nunfold, B.unfold_total, c.stream);
-      }, 3);
-      tried++;
-      if (ms < best) { best = ms; best_rpw = rpw; best_kzc = kzc; }
+      for (int ring : {0, 2}) {
+        if (ring && rpw != 4) continue;
+        B.ring = ring;
+        if ((r = apply_tiling(c, rpw, kzc))) break;
+        const BatchedState::Stage& S = B.stage;
+        const float ms = time_ms([&] {
+          const BatchedState::ModalGroup& g = S.modal;
+          launch_modal(g.rpw, g.pts, g.ring, g.d_desc[0], g.d_start, g.n, g.blocks, 0, c.stream);
+          launch_unfold(S.unfold, S.unfold_start, S.nunfold, S.unfold_total, c.stream);
+        }, 3);
+        tried++;
+        if (ms < best) { best = ms; best_rpw = rpw; best_kzc = kzc; best_ring = ring; }
+      }
+      if (r) break;
     }
     if (r) break;
   }
+  B.ring = best_ring;
   // the FP32 transform kernels compete on speed; BF16 split precision, when
   // requested, is kept (a precision choice, not a speed one)
   float bestg = 1e30f;
@@ -917,16 +979,36 @@ int autotune(Ctx& c) {
     if (B.gemm_variant == 2) break;
     if (v == 4 && !B.has_fft) continue;
     const float ms = time_ms([&] {
-      for (int k = 0; k < 4; ++k) launch_face(B, k, v, c.stream);
+      for (int k = 0; k < 4; ++k) launch_face(B, B.stage, k, v, c.stream);
     }, 3);
     tried++;
     if (ms < bestg) { bestg = ms; B.gemm_variant = v; }
   }
+  if (!r) r = apply_tiling(c, best_rpw, best_kzc);
+  B.graphs_valid = false;
   hipEventDestroy(e0);
   hipEventDestroy(e1);
   if (r) return r;
-  if ((r = apply_tiling(c, best_rpw, best_kzc))) return r;
   B.tuned = tried;
+  return ARD_OK;
+}
+
+int batched_set_option(Ctx& c, int option, int value) {
+  if (!c.bs) return set_error(ARD_ERR_STATE, "not a finalized batched context");
+  BatchedState& B = *c.bs;
+  if (option == ARD_OPT_RING) {
+    if (value != 0 && value != 2) return set_error(ARD_ERR_INVALID, "ring depth must be 0 or 2");
+    B.ring = value;
+    B.graphs_valid = false;
+    return build_stage_tiling(c);
+  }
+  return set_error(ARD_ERR_INVALID, "unknown batched option");
+}
+
+int batched_option(Ctx& c, int option, int* value) {
+  if (!c.bs) return set_error(ARD_ERR_STATE, "not a finalized batched context");
+  if (option == ARD_OPT_RING) *value = c.bs->rpw == 4 ? c.bs->ring : 0;
+  else return set_error(ARD_ERR_INVALID, "unknown batched option");
   return ARD_OK;
 }
 
@@ -1022,6 +1104,18 @@ int batched_modes(Ctx& c, int part, const float* M[2], int* wp) {
   M[1] = P.M[1];
   *wp = P.wp;
   return ARD_OK;
+}
+
+void batched_state_arrays(Ctx& c, int part, std::vector<std::pair<const float*, long long>>& out) {
+  const BPart& P = c.bs->parts[size_t(part)];
+  const PartGeom& g = c.topo.parts[size_t(part)];
+  const long long np = (long long)g.d * g.h * P.wp;  // padded mode rows (the padding holds zeros)
+  out.emplace_back(P.M[0], np);
+  out.emplace_back(P.M[1], np);
+  for (int w = 0; w < 3; ++w) {
+    out.emplace_back(P.F[w], 6 * P.plane[w]);
+    out.emplace_back(P.P[w], 6 * P.plane[w]);
+  }
 }
 
 // Partition::set_force on a DCT partition: a face-slab cell is written in

@@ -195,10 +195,23 @@ __device__ __forceinline__ float bload_nt(brsrc r, unsigned voff, unsigned soff)
 // the batch's sums and barriers.
 // At most 4 rows per wave the kernel fits 3 waves per SIMD (168 VGPRs); the
 // attribute keeps the compiler there. Taller tiles hold more and get 2.
-template <int RPW, bool PTS>
+//
+// RING > 0 (4 rows per wave): the mode rows (M, M-) of the next two planes
+// stream into a per-wave LDS ring by direct-to-LDS loads (buffer_load_dwordx4
+// ... lds: lane l's 16 bytes land at 16 l, so one instruction moves the wave's
+// 4 rows x 64 kx of one array), instead of one plane ahead in registers: two
+// planes in flight per wave at no register cost. The ring's two slots (even
+// and odd planes) are separate LDS arrays: the compiler tracks direct-to-LDS
+// loads per LDS variable and waits, before a read of a slot, for the loads
+// into that slot only. Block barriers inside the plane loop wait for this
+// wave's LDS accesses only (a fence would wait for the ring's loads too), and
+// the ring drains before the block ends (no load may write LDS after the block
+// has released it).
+template <int RPW, bool PTS, int RING>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RPW <= 4 ? 3 : 1))) void k_modal(const ModalDesc* __restrict__ descs,
                                                const long long* __restrict__ starts, int ndesc, int par) {
   static_assert(RPW % 2 == 0, "rows per wave must be even");
+  static_assert(RING == 0 || (RPW == 4 && RING == 2), "the ring holds 4 rows per wave, 2 planes (one per plane parity)");
   constexpr int TY = 4 * RPW;  // rows of the block tile
   constexpr int KB = 4;        // kz planes per batch of the x and y reductions
   constexpr int LD = 80;       // padded row of the mode tile: two rows 16 banks apart (x sums read 2 rows per 8 lanes)
@@ -214,6 +227,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RPW <= 4 ? 
   __shared__ __attribute__((aligned(16))) float s_ptz[PTS ? KB : 1][8];  // point z factors of the batch's planes, likewise
   __shared__ __attribute__((aligned(16))) float s_rfx[PTS ? NRC : 1][64];     // recorder x factors
   __shared__ float s_rec[4][NRC];
+  // the ring's two slots (even and odd planes) are separate arrays, so that
+  // the compiler, which tracks direct-to-LDS loads per LDS variable, does not
+  // make a read of one slot wait for the loads into the other
+  __shared__ __attribute__((aligned(16))) float s_ring0[RING > 0 ? 4 : 1][2][RPW][64];
+  __shared__ __attribute__((aligned(16))) float s_ring1[RING > 0 ? 4 : 1][2][RPW][64];
   const int bid = xcd_block(int(blockIdx.x), int(gridDim.x));
   const int di = find_desc_ll(starts, ndesc, bid);
   const ModalDesc& P = descs[di];
@@ -350,26 +368,71 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RPW <= 4 ? 
     }
   };
 
-  // streamed operands of one plane: modes of the rows (clamped rows) and both
-  // y-parity spectra
+  // streamed operands of one plane: modes of the rows (clamped rows; in the
+  // LDS ring when RING > 0) and both y-parity spectra
   struct PlaneIn {
-    float mc[RPW], mp[RPW], gy[6];
+    float mc[RING > 0 ? 1 : RPW], mp[RING > 0 ? 1 : RPW], gy[6];
   };
   auto load = [&](PlaneIn& in, const int kz) {
     const unsigned po = unsigned(kz * HWp);
+    if constexpr (RING == 0) {
 #pragma unroll
-    for (int r = 0; r < RPW; ++r) {
-      in.mc[r] = bload_nt(rMc, vx, (po + unsigned(rom[r])) * 4u);
-      in.mp[r] = bload_nt(rMp, vx, (po + unsigned(rom[r])) * 4u);
+      for (int r = 0; r < RPW; ++r) {
+        in.mc[r] = bload_nt(rMc, vx, (po + unsigned(rom[r])) * 4u);
+        in.mp[r] = bload_nt(rMp, vx, (po + unsigned(rom[r])) * 4u);
+      }
     }
 #pragma unroll
     for (int q = 0; q < 6; ++q) in.gy[q] = bload(rGy, vx, unsigned(q * DW + kz * W) * 4u);
+  };
+  // ring loads: lane l moves kx = 4 (l % 16) .. +3 of row l / 16 (clamped
+  // rows; kx past W read padding or the next row, never stored)
+  const int rsel = lane >> 4;
+  const unsigned rvoff =
+      unsigned((rsel == 0 ? rom[0] : rsel == 1 ? rom[1 % RPW] : rsel == 2 ? rom[2 % RPW] : rom[3 % RPW]) + bx * 64 +
+               4 * (lane & 15)) * 4u;
+  auto ring_load = [&](const int kz, auto slotc) {
+    if constexpr (RING > 0) {
+      constexpr int SL = decltype(slotc)::value;
+      auto& ring = SL == 0 ? s_ring0 : s_ring1;
+      asm volatile("" ::: "memory");
+      const unsigned so = unsigned(kz * HWp) * 4u;
+#if defined(__HIP_DEVICE_COMPILE__)  // the host pass must not see the device-only builtin (it would drop the kernel's stub)
+      __builtin_amdgcn_raw_ptr_buffer_load_lds(rMc, &ring[wave][0][0][0], 16, rvoff, so, 0, 2);
+      __builtin_amdgcn_raw_ptr_buffer_load_lds(rMp, &ring[wave][1][0][0], 16, rvoff, so, 0, 2);
+#endif
+      asm volatile("" ::: "memory");
+    }
+  };
+  // block barrier; with the ring, one that waits for this wave's LDS accesses
+  // only (__syncthreads' fence would also wait for every direct-to-LDS load
+  // in flight, none of which another wave reads)
+  auto block_barrier = [&]() {
+    if constexpr (RING > 0) asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+    else __syncthreads();
   };
 
   // one kz plane of this wave's rows; a plane past the batch (live false)
   // stores nothing and adds zeros
   auto plane = [&](const PlaneIn& in, const int kz, const int kl, auto pzc, const bool live) {
     constexpr int PZ = decltype(pzc)::value;
+    float mc[RPW], mp[RPW];
+    if constexpr (RING > 0) {
+      // the slot of this plane's parity; the compiler waits for the loads into
+      // this slot (it tracks direct-to-LDS loads per LDS variable)
+      auto& ring = PZ == 0 ? s_ring0 : s_ring1;
+#pragma unroll
+      for (int r = 0; r < RPW; ++r) {
+        mc[r] = ring[wave][0][r][lane];
+        mp[r] = ring[wave][1][r][lane];
+      }
+    } else {
+#pragma unroll
+      for (int r = 0; r < RPW; ++r) {
+        mc[r] = in.mc[r];
+        mp[r] = in.mp[r];
+      }
+    }
     const float4 za = *reinterpret_cast<const float4*>(&s_zt[kl][0]);  // Ez[kz][0..2], Rz[kz][0]
     const float4 zb = *reinterpret_cast<const float4*>(&s_zt[kl][4]);  // Rz[kz][1..2], u_z, 0
     const float uxz = ux + zb.z;
@@ -404,7 +467,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RPW <= 4 ? 
         fh = fmaf(sfx[2] * py.z, pz.z, fh);
         fh = fmaf(sfx[3] * py.w, pz.w, fh);
       }
-      const float mv = fmaf(a, in.mc[r], fmaf(-0.999f, in.mp[r], g * fh));
+      const float mv = fmaf(a, mc[r], fmaf(-0.999f, mp[r], g * fh));
       const bool ok = rok[r] && xok && live;
       if (ok) __builtin_nontemporal_store(mv, &Mp[po + unsigned(rom[r]) + unsigned(kx)]);  // streamed, not reused in this step
       const float mn = ok ? mv : 0.f;
@@ -413,6 +476,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RPW <= 4 ? 
       tz[r][PZ * 3 + 1] = fmaf(zb.x, mn, tz[r][PZ * 3 + 1]);
       tz[r][PZ * 3 + 2] = fmaf(zb.y, mn, tz[r][PZ * 3 + 2]);
     }
+    if (RING > 0) ring_load(min(kz + RING, kz1 - 1), pzc);  // the slot just read takes the plane RING steps ahead
   };
 
   // x and y sums of a finished batch (planes kb .. kb+n-1):
@@ -499,6 +563,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RPW <= 4 ? 
   };
 
   PlaneIn cur, nxt;
+  ring_load(kz0, std::integral_constant<int, 0>());
+  ring_load(min(kz0 + 1, kz1 - 1), std::integral_constant<int, 1>());
   stage_load(kz0);
   load(cur, kz0);
   stage_store();
@@ -506,7 +572,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RPW <= 4 ? 
   for (int kb = kz0; kb < kz1; kb += KB) {
     const int n = min(KB, kz1 - kb);
     stage_load(kb + KB);  // next batch's factors (past the chunk: clamped, unused)
-    for (int kl = 0; kl < n; kl += 2) {
+    // n >= 1: a loop that runs at least once, so that the compiler's count of
+    // the loads issued after this batch's staging loads does not fall to zero
+    // on a path that skips the planes (it would then wait for every load in
+    // flight before storing the staged factors)
+    int kl = 0;
+    do {
       const int kz = kb + kl;
       // the next plane's loads go out before this plane's arithmetic (past
       // the chunk's end they re-read its last plane). Both planes of a pair
@@ -518,15 +589,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RPW <= 4 ? 
       plane(cur, kz, kl, std::integral_constant<int, 0>(), true);
       load(cur, min(kz + 2, kz1 - 1));
       plane(nxt, kz + 1, kl + 1, std::integral_constant<int, 1>(), kl + 1 < n);
-    }
+      kl += 2;
+    } while (kl < n);
     // the x sums read only this wave's rows of the mode tile: no block barrier
     __builtin_amdgcn_wave_barrier();
     xreduce(kb, n);
-    __syncthreads();  // y sums in LDS; the batch's factors no longer read
+    block_barrier();  // y sums in LDS; the batch's factors no longer read
     ysum(kb, n);
     stage_store();
-    __syncthreads();  // LDS free for the next batch, its factors in place
+    block_barrier();  // LDS free for the next batch, its factors in place
   }
+  // the ring's last loads (clamped planes) land before the block ends
+  if (RING > 0) asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
   // z sums of this kz chunk
   gfloat* tzp = ARD_G(P.Tz) + (long long)bz * 6 * HW;
 #pragma unroll
@@ -606,25 +680,24 @@ int launch_bsources(const BSourceDesc* d, int n, const int* d_step, float couran
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 
-int launch_modal(int rpw, bool pts, const ModalDesc* d, const long long* starts, int n, long long blocks, int par,
-                 hipStream_t s) {
+int launch_modal(int rpw, bool pts, int ring, const ModalDesc* d, const long long* starts, int n, long long blocks,
+                 int par, hipStream_t s) {
   if (blocks <= 0) return 0;
   const dim3 g{unsigned(blocks)}, b{256};
-#define ARD_MODAL_CASE(R)                                                                      \
-  case R:                                                                                      \
-    if (pts) hipLaunchKernelGGL((k_modal<R, true>), g, b, 0, s, d, starts, n, par);            \
-    else hipLaunchKernelGGL((k_modal<R, false>), g, b, 0, s, d, starts, n, par);               \
-    break;
-  switch (rpw) {
-    ARD_MODAL_CASE(2)
-    ARD_MODAL_CASE(4)
-    ARD_MODAL_CASE(6)
-    ARD_MODAL_CASE(8)
-    default:
-      return -1;
+  if (ring != 0 && rpw != 4) return -1;
+#define ARD_MODAL_CASE(R, G)                                                                   \
+  if (rpw == R && ring == G) {                                                                 \
+    if (pts) hipLaunchKernelGGL((k_modal<R, true, G>), g, b, 0, s, d, starts, n, par);         \
+    else hipLaunchKernelGGL((k_modal<R, false, G>), g, b, 0, s, d, starts, n, par);            \
+    return hipGetLastError() == hipSuccess ? 0 : -2;                                           \
   }
+  ARD_MODAL_CASE(2, 0)
+  ARD_MODAL_CASE(4, 0)
+  ARD_MODAL_CASE(4, 2)
+  ARD_MODAL_CASE(6, 0)
+  ARD_MODAL_CASE(8, 0)
 #undef ARD_MODAL_CASE
-  return hipGetLastError() == hipSuccess ? 0 : -2;
+  return -1;
 }
 
 int launch_brecord(const BRecDesc* d, int n, float* trace, int cap, int* d_step, hipStream_t s) {

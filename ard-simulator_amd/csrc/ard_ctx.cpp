@@ -341,6 +341,28 @@ int ard_boundary_info(ard_ctx* ctx, int bnd, int* info9) {
   return ARD_OK;
 }
 
+int ard_check_finite(ard_ctx* ctx, int* bad_partition) {
+  Ctx* c = reinterpret_cast<Ctx*>(ctx);
+  if (!c || !bad_partition) return fail(ARD_ERR_INVALID, "null argument");
+  if (!c->finalized) return fail(ARD_ERR_STATE, "not finalized");
+  hipSetDevice(c->device);
+  return check_finite(*c, bad_partition);
+}
+
+int ard_find_boundary(const int* box_a6, const int* box_b6, int* bnd_info9) {
+  if (!box_a6 || !box_b6 || !bnd_info9) return fail(ARD_ERR_INVALID, "null argument");
+  for (int k = 3; k < 6; ++k)
+    if (box_a6[k] <= 0 || box_b6[k] <= 0) return fail(ARD_ERR_INVALID, "box sides must be positive");
+  Topology t;
+  t.add_dct(box_a6[0], box_a6[1], box_a6[2], box_a6[3], box_a6[4], box_a6[5]);
+  t.add_dct(box_b6[0], box_b6[1], box_b6[2], box_b6[3], box_b6[4], box_b6[5]);
+  BoundaryGeom b;
+  if (!t.find_boundary(0, 1, 1.0f, &b)) return 0;
+  const int v[9] = {b.type, -1, -1, b.xs, b.xe, b.ys, b.ye, b.zs, b.ze};
+  std::memcpy(bnd_info9, v, sizeof v);
+  return 1;
+}
+
 int ard_recorder_partition(ard_ctx* ctx, int rec) {
   Ctx* c = reinterpret_cast<Ctx*>(ctx);
   if (!c) return fail(ARD_ERR_INVALID, "null context");
@@ -515,6 +537,18 @@ int ard_set_batched_tiling(ard_ctx* ctx, int rows_per_wave, int kz_per_block) {
   Ctx* c = reinterpret_cast<Ctx*>(ctx);
   if (!c) return fail(ARD_ERR_INVALID, "null context");
   return batched_set_tiling(*c, rows_per_wave, kz_per_block);
+}
+
+int ard_set_batched_option(ard_ctx* ctx, int option, int value) {
+  Ctx* c = reinterpret_cast<Ctx*>(ctx);
+  if (!c) return fail(ARD_ERR_INVALID, "null context");
+  return batched_set_option(*c, option, value);
+}
+
+int ard_batched_option(ard_ctx* ctx, int option, int* value) {
+  Ctx* c = reinterpret_cast<Ctx*>(ctx);
+  if (!c || !value) return fail(ARD_ERR_INVALID, "null argument");
+  return batched_option(*c, option, value);
 }
 
 int ard_set_transform_variant(int variant) {

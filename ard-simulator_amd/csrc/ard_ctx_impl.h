@@ -146,7 +146,15 @@ int batched_set_force(Ctx& c, int part, int x, int y, int z, float f);
 void destroy_batched(Ctx& c);
 int batched_config(Ctx& c, int* out4);
 int batched_set_tiling(Ctx& c, int rpw, int kzc);
+int batched_set_option(Ctx& c, int option, int value);
+int batched_option(Ctx& c, int option, int* value);
 int batched_modes(Ctx& c, int part, const float* M[2], int* wp);
+// The state arrays of a DCT partition in the batched form (modes, face-slab
+// force and pressure) as (pointer, floats), for the non-finite check.
+void batched_state_arrays(Ctx& c, int part, std::vector<std::pair<const float*, long long>>& out);
+// ard_check_finite: the lowest partition id whose state holds a NaN or an
+// infinity, or -1; synchronous.
+int check_finite(Ctx& c, int* bad_part);
 int fold_selftest(int device, int inverse, int b_layout, int L, int N, const float* X, float* Y, bool bf16);
 // FFT-style transforms (ard_fft.hip): the FFT fields of a dense pass
 // descriptor (its plan cached in the context; ARD_ERR_UNSUPPORTED for a

@@ -258,6 +258,18 @@ Simulation::Simulation(std::vector<std::shared_ptr<Partition>>& partitions,
       partitions.push_back(pml);
     }
     partitions_ = partitions;
+    // boundary handles in write order; those against a PML slab carry the
+    // partition absorption (ARD/simulation.cpp:104-260), the others 1
+    const int nb = ard_num_boundaries(ctx_);
+    for (int k = 0; k < nb; ++k) {
+      int b[9];
+      check(ard_boundary_info(ctx_, k, b), "ard_boundary_info");
+      const auto& pa = partitions_[size_t(b[1])];
+      const auto& pb = partitions_[size_t(b[2])];
+      const real_t absorp = pa->is_pml() || pb->is_pml() ? Partition::m_absorption : 1.0f;
+      boundaries_.push_back(
+          std::make_shared<Boundary>(Boundary::BoundaryType(b[0]), absorp, pa, pb, b[3], b[4], b[5], b[6], b[7], b[8]));
+    }
     const int steps = int(m_duration / m_dt);
     for (auto& s : sources) {
       std::vector<float> w = s->Waveform();
@@ -301,6 +313,12 @@ void Simulation::fetch_row(int t) {
   row_t_ = t;
 }
 
+int Simulation::CheckFinite() {
+  int bad = -1;
+  check(ard_check_finite(ctx_, &bad), "ard_check_finite");
+  return bad;
+}
+
 int Simulation::Update() {
   const int t = time_step_++;
   check(ard_step(ctx_, 1), "ard_step");
@@ -330,6 +348,36 @@ std::vector<uint32_t> Simulation::pixels() {
   std::vector<uint32_t> px(size_t(dims[0]) * size_t(dims[1]));
   check(ard_render_view(ctx_, view, -1, px.data(), dims), "ard_render_view");
   return px;
+}
+
+Boundary::Boundary(BoundaryType type, real_t absorp, std::shared_ptr<Partition> a, std::shared_ptr<Partition> b,
+                   int xs, int xe, int ys, int ye, int zs, int ze)
+    : type_(type), x_start_(xs), x_end_(xe), y_start_(ys), y_end_(ye), z_start_(zs), z_end_(ze), a_(std::move(a)),
+      b_(std::move(b)), absorption_(absorp) {
+  static int id_generator = 0;  // ARD/boundary.cpp:11-12
+  id_ = id_generator++;
+}
+
+std::shared_ptr<Boundary> Boundary::FindBoundary(std::shared_ptr<Partition> a, std::shared_ptr<Partition> b,
+                                                 real_t absorp) {
+  const int ba[6] = {a->x_start(), a->y_start(), a->z_start(), a->width(), a->height(), a->depth()};
+  const int bb[6] = {b->x_start(), b->y_start(), b->z_start(), b->width(), b->height(), b->depth()};
+  int g[9];
+  const int r = ard_find_boundary(ba, bb, g);
+  check(r, "ard_find_boundary");
+  if (r == 0) return nullptr;
+  return std::make_shared<Boundary>(BoundaryType(g[0]), absorp, a, b, g[3], g[4], g[5], g[6], g[7], g[8]);
+}
+
+void Boundary::ComputeForcingTerms() {
+  throw Error("Boundary::ComputeForcingTerms: the forcing of every boundary runs on the device inside "
+              "Simulation::Update");
+}
+
+void Boundary::Info() {
+  const char* t = type_ == X_BOUNDARY ? "X " : (type_ == Y_BOUNDARY ? "Y " : "Z ");
+  std::cout << "Boundary " << id_ << ": " << t << a_->id() << "," << b_->id() << " | " << x_start_ << ","
+            << y_start_ << "," << z_start_ << "->" << x_end_ << "," << y_end_ << "," << z_end_ << std::endl;
 }
 
 void Simulation::Info() {

@@ -175,6 +175,45 @@ class Recorder {
   friend class Simulation;
 };
 
+// ARD/boundary.h:9-45: the interface between two partitions, in cell units.
+// The Simulation constructor builds one per boundary of its topology, in
+// registration (write) order; every boundary's forcing
+// (Boundary::ComputeForcingTerms, ARD/boundary.cpp:21-173) runs on the device
+// inside Simulation::Update.
+class Boundary {
+ public:
+  enum BoundaryType { X_BOUNDARY, Y_BOUNDARY, Z_BOUNDARY };  // ARD/boundary.h:30-34
+
+  Boundary(BoundaryType type, real_t absorp, std::shared_ptr<Partition> a, std::shared_ptr<Partition> b, int xs,
+           int xe, int ys, int ye, int zs, int ze);
+  // ARD/boundary.cpp:289-334: the boundary between two partitions that share an
+  // x or y face (z extent from a), or nullptr; the geometry of ard_find_boundary
+  static std::shared_ptr<Boundary> FindBoundary(std::shared_ptr<Partition> a, std::shared_ptr<Partition> b,
+                                                real_t absorp = 1.0f);
+  // The forcing of a single boundary is not a separate call here: Update
+  // computes every boundary's on the device, in list order. Throws Error.
+  void ComputeForcingTerms();
+  void Info();  // ARD/boundary.cpp:336-351 format
+
+  BoundaryType type_;
+  int id() const { return id_; }
+  real_t absorption() const { return absorption_; }
+  std::shared_ptr<Partition> a() const { return a_; }
+  std::shared_ptr<Partition> b() const { return b_; }
+  int x_start() const { return x_start_; }
+  int x_end() const { return x_end_; }
+  int y_start() const { return y_start_; }
+  int y_end() const { return y_end_; }
+  int z_start() const { return z_start_; }
+  int z_end() const { return z_end_; }
+
+ private:
+  int x_start_, x_end_, y_start_, y_end_, z_start_, z_end_;
+  int id_;  // numbered in construction order (ARD/boundary.cpp:11-12)
+  std::shared_ptr<Partition> a_, b_;
+  real_t absorption_;
+};
+
 class Simulation {
  public:
   // statics of ARD/simulation.h:41-46, set before constructing partitions
@@ -204,6 +243,9 @@ class Simulation {
   Simulation& operator=(const Simulation&) = delete;
 
   int Update();  // ARD/simulation.cpp:300-317: returns the step index before the update
+  // the lowest partition id whose state holds a NaN or an infinity, or -1
+  // (ard_check_finite; no counterpart in the reference)
+  int CheckFinite();
   void Info();
   // The visualisation of ARD/simulation.cpp:320-420, drawn on the device when
   // asked for (pixels()) rather than in every Update: look_from_ 0 is the xy
@@ -218,11 +260,14 @@ class Simulation {
   std::vector<uint32_t> pixels();
   ard_ctx* context() { return ctx_; }
   int time_step() const { return time_step_; }
+  // the boundaries of the topology in write order (ARD/simulation.h:24-26 m_boundaries)
+  const std::vector<std::shared_ptr<Boundary>>& boundaries() const { return boundaries_; }
 
  private:
   ard_ctx* ctx_ = nullptr;
   std::vector<std::shared_ptr<Partition>> partitions_;
   std::vector<std::shared_ptr<Recorder>> recorders_;
+  std::vector<std::shared_ptr<Boundary>> boundaries_;
   int time_step_ = 0;
   // one step's device samples, shared by the recorders' RecordField calls
   int row_t_ = -1;

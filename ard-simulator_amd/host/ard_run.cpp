@@ -29,6 +29,9 @@
 //   --look-from V 0: xy plane at the first source's z (default), 1: yz plane at its x
 //   --copy-mib N  first run one device-to-device copy of N MiB: a kernel of known
 //                 traffic, the scale for hardware byte counters
+//   --check-every N  every N steps, check the whole state for NaN or infinity
+//                 (ard_check_finite); stop with status 3 at the first non-finite one
+//   --info        print every partition and boundary (Partition::Info, Boundary::Info)
 #include <hip/hip_runtime_api.h>
 
 #include <sys/stat.h>
@@ -48,6 +51,8 @@ using namespace ard::host;
 int main(int argc, char** argv) {
   bool cells = false, timing = false, output = true;
   long long copy_mib = 0;
+  int check_every = 0;
+  bool info = false;
   const char* frame = nullptr;
   int look_from = 0;
   std::vector<const char*> pos;
@@ -70,6 +75,8 @@ int main(int argc, char** argv) {
     else if (!std::strcmp(argv[i], "--c0") && i + 1 < argc) Simulation::m_c0 = float(std::atof(argv[++i]));
     else if (!std::strcmp(argv[i], "--duration") && i + 1 < argc) Simulation::m_duration = float(std::atof(argv[++i]));
     else if (!std::strcmp(argv[i], "--copy-mib") && i + 1 < argc) copy_mib = std::atoll(argv[++i]);
+    else if (!std::strcmp(argv[i], "--check-every") && i + 1 < argc) check_every = std::atoi(argv[++i]);
+    else if (!std::strcmp(argv[i], "--info")) info = true;
     else if (argv[i][0] == '-' && argv[i][1] == '-') {
       std::fprintf(stderr, "unknown option %s\n", argv[i]);
       return 2;
@@ -80,7 +87,7 @@ int main(int argc, char** argv) {
   if (pos.size() < 2) {
     std::fprintf(stderr,
                  "usage: %s [--cells] [--rec-local] [--plain] [--autotune] [--transform V] [--timing] [--no-output] [--no-cubes] "
-                 "[--c0 V] [--duration S] [--frame F] [--look-from V] [--copy-mib N] scene.txt sources.txt [recorders.txt] [steps] [h] [out_root]\n",
+                 "[--c0 V] [--duration S] [--frame F] [--look-from V] [--copy-mib N] [--check-every N] [--info] scene.txt sources.txt [recorders.txt] [steps] [h] [out_root]\n",
                  argv[0]);
     return 2;
   }
@@ -117,10 +124,21 @@ int main(int argc, char** argv) {
     auto t1 = std::chrono::steady_clock::now();
     std::cout << "Initialization finished. (" << std::chrono::duration<double>(t1 - t0).count() << " s)" << std::endl;
     sim.Info();
+    if (info) {
+      for (auto& p : partitions) p->Info();
+      for (auto& b : sim.boundaries()) b->Info();
+    }
     for (int s = 0; s < steps; ++s) {
       const int t = sim.Update();
       if (output)
         for (auto& r : recorders) r->RecordField(t);
+      if (check_every > 0 && (s + 1) % check_every == 0) {
+        const int bad = sim.CheckFinite();
+        if (bad >= 0) {
+          std::fprintf(stderr, "non-finite state in partition %d after step %d\n", bad, t);
+          return 3;
+        }
+      }
     }
     if (ard_synchronize(sim.context()) != ARD_OK) throw Error(ard_last_error());
     auto t2 = std::chrono::steady_clock::now();

@@ -85,6 +85,10 @@ int ard_add_pml_partition(ard_ctx* ctx, int neighbor, int type, int x0, int y0, 
  * order is write order: a cell forced by two boundaries keeps the later one's value. */
 int ard_add_boundary(ard_ctx* ctx, int type, int a, int b, int xs, int xe, int ys, int ye, int zs, int ze,
                      float absorption);
+/* Boundary::FindBoundary (ARD/boundary.cpp:289-334) on two boxes in cell units, box = {x0, y0,
+ * z0, w, h, d}: 1 and bnd_info9 = {type, -1, -1, xs,xe,ys,ye,zs,ze} when they share an x or y
+ * face (the 2.5D search: z from box a), 0 when they do not, negative on error. Needs no context. */
+int ard_find_boundary(const int* box_a6, const int* box_b6, int* bnd_info9);
 /* The topology part of Simulation::Simulation (ARD/simulation.cpp:19-266): boundaries between
  * the DCT partitions added so far (Boundary::FindBoundary, ARD/boundary.cpp:289-334), border
  * masks (Partition::AddBoundary, ARD/partition.cpp:96-143) and the PML slabs with their boundaries. */
@@ -112,6 +116,14 @@ int ard_num_recorders(ard_ctx* ctx);
 int ard_partition_info(ard_ctx* ctx, int part, int* info9);
 int ard_boundary_info(ard_ctx* ctx, int bnd, int* info9);
 int ard_recorder_partition(ard_ctx* ctx, int rec);
+
+/* Non-finite state check: reads every state array of the context (DCT force, pressure and
+ * modes, or in the batched form the modes and the face-slab force and pressure; PML pressure,
+ * phi and force) and sets *bad_partition to the lowest partition id holding a NaN or an
+ * infinity, -1 if none. Synchronous; one read of the state (about 0.2 ms for 80M cells), so a
+ * caller checks every so many steps. No counterpart in the reference, which has no failure
+ * detection (a diverging run fills its traces with NaN, ARD/recorder.cpp:40-57). */
+int ard_check_finite(ard_ctx* ctx, int* bad_partition);
 
 /* n calls of Simulation::Update (ARD/simulation.cpp:300-317) followed by the recorder point
  * sample of Recorder::RecordField (ARD/recorder.cpp:40-57), all on the device. Asynchronous
@@ -258,6 +270,20 @@ int ard_batched_config(ard_ctx* ctx, int* out4);
  * counterpart in the reference: it pins the tiling, for tests and callers that
  * skip the autotuner. */
 int ard_set_batched_tiling(ard_ctx* ctx, int rows_per_wave, int kz_per_block);
+/* Schedule options of the batched form after ard_finalize (chosen by the
+ * autotuner when ard_params.autotune is set). Results do not depend on them.
+ *   ARD_OPT_RING      0 or 2: planes of mode rows the modal pass keeps in
+ *                     flight per wave; 2 streams them into LDS by
+ *                     direct-to-LDS loads, 0 (default) keeps one plane in
+ *                     registers. With 4 rows per wave only (other tilings
+ *                     use 0).
+ * ARD_ERR_STATE for a context that is not a finalized batched one,
+ * ARD_ERR_INVALID for other options or values. No counterpart in the
+ * reference: they schedule the step of ARD/simulation.cpp:300-317 on one
+ * device. */
+#define ARD_OPT_RING 2
+int ard_set_batched_option(ard_ctx* ctx, int option, int value);
+int ard_batched_option(ard_ctx* ctx, int option, int* value);
 
 /* Discrete adjoint of the plain form (no counterpart in the reference, which
  * has no gradients; it transposes the step of ARD/simulation.cpp:300-317).

@@ -1,0 +1,112 @@
+"""GPU checks of the failure detection and schedule options of the library:
+ard_check_finite on a clean state and after a NaN force is injected from the
+host (the condition is forced from the host; kernels run normally on NaN
+data), and the batched form's schedule option (the LDS ring of the modal
+pass), which must leave the recorder traces bit-identical."""
+import numpy as np
+import pytest
+
+import ard_scenes as S
+import ard_sim as A
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("form", [A.FORM_PLAIN, A.FORM_BATCHED])
+def test_check_finite_reports_nan_state(form):
+    sc = S.two_rooms()
+    print("case check_finite form", form, flush=True)
+    sim = A.Simulation(sc, form=form)
+    sim.update(40)
+    assert sim.check_finite() == -1
+    # a NaN force in the second room's interior (partition 1): after one step its
+    # modes and pressure hold NaN, and the interface carries it to partition 0
+    sim.set_force(1, 12, 14, 15, float("nan"))
+    sim.update(1)
+    bad = sim.check_finite()
+    print("first partition with non-finite state:", bad)
+    assert bad in (0, 1)
+    sim.close()
+
+
+@pytest.mark.parametrize("form", [A.FORM_PLAIN, A.FORM_BATCHED])
+def test_check_finite_nan_in_pml(form):
+    sc = S.two_rooms()
+    sim = A.Simulation(sc, form=form)
+    parts = sim.partitions()
+    pml = [i for i, p in enumerate(parts) if p[0] == 1]
+    sim.update(5)
+    assert sim.check_finite() == -1
+    # the last PML slab: a NaN force enters its pressure in the next update
+    last = pml[-1]
+    sim.set_force(last, 1, 1, 1, float("inf"))
+    sim.update(1)
+    bad = sim.check_finite()
+    print("PML slab", last, "-> first non-finite partition", bad)
+    assert 0 <= bad <= last
+    sim.close()
+
+
+def test_batched_schedule_options_bit_identical():
+    sc = S.scene3()
+    steps = 300
+    traces = {}
+    for name, opts in (("default", {}), ("ring2", {A.OPT_RING: 2})):
+        print("case options", name, flush=True)
+        sim = A.Simulation(sc, form=A.FORM_BATCHED)
+        sim.set_batched_tiling(4, 48)
+        for k, v in opts.items():
+            sim.set_batched_option(k, v)
+            assert sim.batched_option(k) == v
+        sim.update(steps)
+        traces[name] = sim.recorders(0, steps)
+        sim.close()
+    for name, tr in traces.items():
+        assert np.array_equal(tr, traces["default"]), name
+
+
+def test_batched_option_values_checked():
+    sim = A.Simulation(S.two_rooms(), form=A.FORM_BATCHED)
+    with pytest.raises(A.ArdError):
+        sim.set_batched_option(A.OPT_RING, 3)
+    with pytest.raises(A.ArdError):
+        sim.set_batched_option(99, 1)
+    sim.close()
+    plain = A.Simulation(S.two_rooms(), form=A.FORM_PLAIN)
+    with pytest.raises(A.ArdError):
+        plain.set_batched_option(A.OPT_RING, 0)
+    plain.close()
+
+
+@pytest.mark.parametrize("form", [A.FORM_PLAIN, A.FORM_BATCHED])
+def test_ard_run_info_and_finite_checks(tmp_path, form):
+    """ard_run (the C++ mirror's main loop) with --info (Partition::Info and
+    Boundary::Info of every partition and boundary, the boundaries being the
+    mirror's Boundary handles) and --check-every (ard_check_finite between
+    steps): the boundary lines match the topology, and a clean run passes."""
+    import os
+    import re
+    import subprocess
+
+    sc = S.scene3(steps=60)
+    pkg = os.path.dirname(os.path.abspath(A.__file__))
+    S.write_cell_files(sc, str(tmp_path / "scene"))
+    cmd = [os.path.join(pkg, "ard_run"), "--cells", "--no-output", "--info", "--check-every", "20"]
+    if form == A.FORM_PLAIN:
+        cmd.append("--plain")
+    cmd += [str(tmp_path / "scene" / n) for n in ("scene.txt", "sources.txt", "recorders.txt")]
+    cmd += ["60", repr(sc.h)]
+    print("case ard_run --info --check-every form", form, flush=True)
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=120)
+    print(r.stdout)
+    assert r.returncode == 0
+    lines = re.findall(r"^Boundary (\d+): ([XYZ]) (\d+),(\d+) \| (\d+),(\d+),(\d+)->(\d+),(\d+),(\d+)$", r.stdout,
+                       re.M)
+    sim = A.Simulation(sc, device=-1, finalize=False)
+    bnds = sim.boundaries()
+    assert len(lines) == len(bnds)
+    for ln, b in zip(lines, bnds):
+        typ = "XYZ"[b[0]]
+        # Info prints start "xs,ys,zs" and end "xe,ye,ze"
+        assert (ln[1], int(ln[2]), int(ln[3])) == (typ, b[1], b[2])
+        assert tuple(int(v) for v in ln[4:]) == (b[3], b[5], b[7], b[4], b[6], b[8])

@@ -86,3 +86,29 @@ def test_hall_cell_fixture_matches_generator():
     assert srcs == [(s.x, s.y, s.z) for s in sc.sources]
     assert recs == [tuple(r) for r in sc.recorders]
     assert sum(b[3] * b[4] * b[5] for b in boxes) == 74_000_000
+
+
+@pytest.mark.parametrize("boxes", [
+    ((0, 0, 0, 30, 30, 30), (30, 0, 0, 30, 30, 30)),      # x neighbours, b to the right
+    ((30, 0, 0, 30, 30, 30), (0, 0, 0, 30, 30, 30)),      # x neighbours, b to the left
+    ((0, 0, 0, 40, 20, 16), (10, 20, 0, 50, 30, 16)),     # y neighbours, partial overlap
+    ((10, 20, 0, 50, 30, 16), (0, 0, 0, 40, 20, 16)),     # y neighbours, b below
+    ((0, 0, 0, 20, 20, 20), (40, 0, 0, 20, 20, 20)),      # apart
+    ((0, 0, 0, 20, 20, 20), (20, 20, 0, 20, 20, 20)),     # corner contact only
+])
+def test_find_boundary_matches_oracle_topology(boxes):
+    """ard_find_boundary (Boundary::FindBoundary, ARD/boundary.cpp:289-334, needing
+    no context) against the oracle's boundary between the same two DCT boxes: the
+    first boundary of a two-box oracle scene, whose partitions 0 and 1 are the boxes."""
+    a, b = boxes
+    o = O.OracleSim(0.1, 1.25e-4, 343.0)
+    o.add_dct(*a)
+    o.add_dct(*b)
+    o.finalize()
+    dct_pairs = [bd for bd in o.boundaries() if {bd[1], bd[2]} == {0, 1}]
+    got = A.find_boundary(a, b)
+    if not dct_pairs:
+        assert got is None
+    else:
+        bd = dct_pairs[0]
+        assert got == (bd[0],) + tuple(bd[3:9])
