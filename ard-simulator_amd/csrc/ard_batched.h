@@ -95,6 +95,8 @@ int launch_bsources(const BSourceDesc* d, int n, const int* d_step, float couran
 // ring: planes of mode rows in flight per wave through LDS (0: one, in registers; 2 or 3 with 4 rows per wave)
 int launch_modal(int rpw, bool pts, int ring, const ModalDesc* d, const long long* starts, int n, long long blocks,
                  int par, hipStream_t s);
-int launch_brecord(const BRecDesc* d, int n, float* trace, int cap, int* d_step, hipStream_t s);
+// records the step's samples, advances *d_step and writes the next step's source samples
+int launch_brecord(const BRecDesc* d, int n, float* trace, int cap, int* d_step, const BSourceDesc* src, int nsrc,
+                   float courant, hipStream_t s);
 
 }  // namespace ard

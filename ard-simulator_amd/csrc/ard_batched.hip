@@ -2,11 +2,11 @@
 // (ARD/simulation.cpp:300-317) for all partitions in one launch, and one step
 // captured in a hipGraph (one graph per step parity, replayed alternately).
 //
-// Per step: sources -> PML update -> fold face slabs -> 2D DCT-II of the face
-// slabs (2 grouped GEMM launches) -> modal pass (force spectrum expansion,
-// modal update, reduction onto face planes and recorders) -> unfold ->
-// 2D DCT-III to face-slab pressure (2 grouped GEMM launches) -> interface
-// forcing with last-writer masks -> recorder samples.
+// Per step: PML update -> fold face slabs -> 2D DCT-II of the face slabs (2
+// grouped GEMM launches) -> modal pass (force spectrum expansion, modal
+// update, reduction onto face planes and recorders) -> unfold -> 2D DCT-III
+// to face-slab pressure (2 grouped GEMM launches) -> interface forcing with
+// last-writer masks -> recorder samples and the next step's source samples.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -422,6 +422,14 @@ int build_stage_tiling(Ctx& c) {
   return ARD_OK;
 }
 
+// The source samples of the current step (*d_step) into their cells.
+int write_sources(Ctx& c) {
+  BatchedState& B = *c.bs;
+  if (launch_bsources(B.d_src, B.nsrc, c.d_step, c.courant, c.stream)) return set_error(ARD_ERR_HIP, "sources");
+  hipError_t e = hipStreamSynchronize(c.stream);
+  return e == hipSuccess ? ARD_OK : hip_fail(e, "hipStreamSynchronize");
+}
+
 int build_sources(Ctx& c) {
   BatchedState& B = *c.bs;
   int r;
@@ -462,7 +470,8 @@ int build_sources(Ctx& c) {
   }
   B.nsrc = int(sd.size());
   if (B.nsrc && (r = upload_vec(c, (void**)&B.d_src, sd.data(), sd.size() * sizeof(BSourceDesc)))) return r;
-  return ARD_OK;
+  // the current step's samples (each step's recorder kernel writes the next one's)
+  return write_sources(c);
 }
 
 int build_modal(Ctx& c) {
@@ -844,7 +853,8 @@ int enqueue_step(Ctx& c, int par, PhaseTimer* T) {
     if (T) T->end(s, ev);
   };
   auto ok = [&](hipError_t e) { r |= e != hipSuccess; };
-  phase(ARD_PH_SOURCES, st, [&] { return launch_bsources(B.d_src, B.nsrc, c.d_step, c.courant, st); });
+  // the step's source samples are in place: written by the previous step's
+  // recorder kernel (or at set-up)
   // captured: PML on the side stream, joined before the interface (which
   // reads the new PML pressure and writes PML forces); timed: all in order on
   // one stream, phase by phase over all partitions.
@@ -886,7 +896,8 @@ int enqueue_step(Ctx& c, int par, PhaseTimer* T) {
   phase(ARD_PH_INTERFACE, st, [&] { return launch_interface(B.d_bnd, B.nbnd, B.d_parts, par ^ 1, B.bnd_total, st); });
   phase(ARD_PH_RECORD, st, [&] {
     if (enqueue_cubes(c, par, st)) return 1;
-    return launch_brecord(B.d_rec, int(c.topo.recorders.size()), c.d_trace, c.trace_cap, c.d_step, st);
+    return launch_brecord(B.d_rec, int(c.topo.recorders.size()), c.d_trace, c.trace_cap, c.d_step, B.d_src, B.nsrc,
+                          c.courant, st);
   });
   return r ? set_error(ARD_ERR_HIP, "batched step launch") : ARD_OK;
 }
@@ -1126,7 +1137,12 @@ int batched_set_force(Ctx& c, int part, int x, int y, int z, float f) {
   const PartGeom& p = c.topo.parts[part];
   long long off;
   const int w = shell_of(p.w, p.h, p.d, x, y, z, &off);
-  if (w >= 0) return c.upload(B.parts[part].F[w] + off, &f, 4);
+  if (w >= 0) {
+    // a face-slab cell; if it is a source's, the source overwrites it before
+    // the next update, as in the reference (ARD/simulation.cpp:306-312)
+    const int r = c.upload(B.parts[part].F[w] + off, &f, 4);
+    return r ? r : write_sources(c);
+  }
   for (auto& s : B.srcs)
     if (s.part == part && s.x == x && s.y == y && s.z == z) {
       // a source cell: the source overwrites it before the next update anyway

@@ -124,21 +124,23 @@ __global__ __launch_bounds__(256) void k_unfold(const UnfoldDesc* __restrict__ d
 // cells receive the value in the face slab; interior cells in a value slot
 // read by k_modal.
 // ---------------------------------------------------------------------------
+__device__ __forceinline__ float bsource_value(const BSourceDesc& s, int t, float courant) {
+  if (s.kind == 0) {
+    const float x = 3.14159265358979f * ((2.f * courant * float(t)) / 6.f - 2.f);
+    return 1e9f * expf(-(x * x));
+  }
+  if (s.kind == 1) return t < s.n ? s.wave[t] : 0.f;
+  return s.value;
+}
+
+// the samples of the current step (*d_step): at set-up, and whenever the
+// sources or a source cell's force change between steps; within a step the
+// recorder kernel writes the next step's
 __global__ void k_bsources(const BSourceDesc* __restrict__ s, int n, const int* __restrict__ d_step,
                            float courant) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const int t = *d_step;
-  float v;
-  if (s[i].kind == 0) {
-    const float x = 3.14159265358979f * ((2.f * courant * float(t)) / 6.f - 2.f);
-    v = 1e9f * expf(-(x * x));
-  } else if (s[i].kind == 1) {
-    v = t < s[i].n ? s[i].wave[t] : 0.f;
-  } else {
-    v = s[i].value;
-  }
-  *s[i].dst = v;
+  *s[i].dst = bsource_value(s[i], *d_step, courant);
 }
 
 // ---------------------------------------------------------------------------
@@ -627,10 +629,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RPW <= 4 ? 
 }
 
 // ---------------------------------------------------------------------------
-// Recorder samples from the block partials, then the step counter.
+// Recorder samples from the block partials, then the step counter, then the
+// next step's source samples: they overwrite their cells after this step's
+// interface forcing and before the next update, where the reference sets
+// them (ARD/simulation.cpp:306-312), one launch fewer per step.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(1024) void k_brecord(const BRecDesc* __restrict__ r, int n, float* __restrict__ trace,
-                                                  int cap, int* __restrict__ d_step) {
+                                                  int cap, int* __restrict__ d_step, const BSourceDesc* __restrict__ src,
+                                                  int nsrc, float courant) {
   // one wave per recorder: its lanes sum the block partials in FP64, eight
   // loads in flight per lane
   const int t = *d_step;
@@ -655,6 +661,7 @@ __global__ __launch_bounds__(1024) void k_brecord(const BRecDesc* __restrict__ r
   }
   __syncthreads();
   if (threadIdx.x == 0) *d_step = t + 1;
+  for (int i = int(threadIdx.x); i < nsrc; i += 1024) *src[i].dst = bsource_value(src[i], t + 1, courant);
 }
 
 // ---------------------------------------------------------------------------
@@ -700,8 +707,9 @@ int launch_modal(int rpw, bool pts, int ring, const ModalDesc* d, const long lon
   return -1;
 }
 
-int launch_brecord(const BRecDesc* d, int n, float* trace, int cap, int* d_step, hipStream_t s) {
-  hipLaunchKernelGGL(k_brecord, dim3(1), dim3(1024), 0, s, d, n, trace, cap, d_step);
+int launch_brecord(const BRecDesc* d, int n, float* trace, int cap, int* d_step, const BSourceDesc* src, int nsrc,
+                   float courant, hipStream_t s) {
+  hipLaunchKernelGGL(k_brecord, dim3(1), dim3(1024), 0, s, d, n, trace, cap, d_step, src, nsrc, courant);
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 
