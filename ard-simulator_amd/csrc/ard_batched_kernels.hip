@@ -203,12 +203,13 @@ __device__ __forceinline__ float bload_nt(brsrc r, unsigned voff, unsigned soff)
 // ... lds: lane l's 16 bytes land at 16 l, so one instruction moves the wave's
 // 4 rows x 64 kx of one array), instead of one plane ahead in registers: two
 // planes in flight per wave at no register cost. The ring's two slots (even
-// and odd planes) are separate LDS arrays: the compiler tracks direct-to-LDS
-// loads per LDS variable and waits, before a read of a slot, for the loads
-// into that slot only. Block barriers inside the plane loop wait for this
-// wave's LDS accesses only (a fence would wait for the ring's loads too), and
-// the ring drains before the block ends (no load may write LDS after the block
-// has released it).
+// and odd planes) are separate LDS arrays, so that the compiler's own
+// tracking of direct-to-LDS loads (per LDS variable) never makes a read of one
+// slot wait for the loads into the other; the kernel waits for a slot's loads
+// itself, by count (see the plane step). Block barriers inside the plane loop
+// wait for this wave's LDS accesses only (a fence would wait for the ring's
+// loads too), and the ring drains before the block ends (no load may write
+// LDS after the block has released it).
 template <int RPW, bool PTS, int RING>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RPW <= 4 ? 3 : 1))) void k_modal(const ModalDesc* __restrict__ descs,
                                                const long long* __restrict__ starts, int ndesc, int par) {
@@ -393,15 +394,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RPW <= 4 ? 
   const unsigned rvoff =
       unsigned((rsel == 0 ? rom[0] : rsel == 1 ? rom[1 % RPW] : rsel == 2 ? rom[2 % RPW] : rom[3 % RPW]) + bx * 64 +
                4 * (lane & 15)) * 4u;
-  auto ring_load = [&](const int kz, auto slotc) {
+  auto ring_load = [&](const int kz, auto slotc, const bool idle) {
     if constexpr (RING > 0) {
       constexpr int SL = decltype(slotc)::value;
       auto& ring = SL == 0 ? s_ring0 : s_ring1;
       asm volatile("" ::: "memory");
-      const unsigned so = unsigned(kz * HWp) * 4u;
+      const unsigned so = idle ? 0u : unsigned(kz * HWp) * 4u;
+      const unsigned vo = idle ? 0u : rvoff;
 #if defined(__HIP_DEVICE_COMPILE__)  // the host pass must not see the device-only builtin (it would drop the kernel's stub)
-      __builtin_amdgcn_raw_ptr_buffer_load_lds(rMc, &ring[wave][0][0][0], 16, rvoff, so, 0, 2);
-      __builtin_amdgcn_raw_ptr_buffer_load_lds(rMp, &ring[wave][1][0][0], 16, rvoff, so, 0, 2);
+      __builtin_amdgcn_raw_ptr_buffer_load_lds(rMc, &ring[wave][0][0][0], 16, vo, so, 0, 2);
+      __builtin_amdgcn_raw_ptr_buffer_load_lds(rMp, &ring[wave][1][0][0], 16, vo, so, 0, 2);
 #endif
       asm volatile("" ::: "memory");
     }
@@ -420,9 +422,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RPW <= 4 ? 
     constexpr int PZ = decltype(pzc)::value;
     float mc[RPW], mp[RPW];
     if constexpr (RING > 0) {
-      // the slot of this plane's parity; the compiler waits for the loads into
-      // this slot (it tracks direct-to-LDS loads per LDS variable)
+      // the slot of this plane's parity. Its loads were issued at the end of
+      // the step two planes back; since then this wave issued at least 14
+      // vector-memory operations (two steps' y-spectrum loads, 6 each, and the
+      // other slot's 2 ring loads; 12 counted here), and such operations
+      // complete in issue order, so waiting until at most 12 are outstanding
+      // guarantees them. The compiler's own tracking of direct-to-LDS loads is
+      // not relied on (it drops its wait when they are issued under a branch).
       auto& ring = PZ == 0 ? s_ring0 : s_ring1;
+      asm volatile("s_waitcnt vmcnt(12)" ::: "memory");
 #pragma unroll
       for (int r = 0; r < RPW; ++r) {
         mc[r] = ring[wave][0][r][lane];
@@ -478,7 +486,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RPW <= 4 ? 
       tz[r][PZ * 3 + 1] = fmaf(zb.x, mn, tz[r][PZ * 3 + 1]);
       tz[r][PZ * 3 + 2] = fmaf(zb.y, mn, tz[r][PZ * 3 + 2]);
     }
-    if (RING > 0) ring_load(min(kz + RING, kz1 - 1), pzc);  // the slot just read takes the plane RING steps ahead
+    // the slot just read takes the plane RING steps ahead; past the chunk's
+    // end the step still issues its two loads (the same count every step, no
+    // branch for the compiler's wait counts to merge), all lanes from the
+    // array's first 16 bytes: one cached line instead of a plane re-read
+    if (RING > 0) ring_load(kz + RING, pzc, kz + RING >= kz1);
   };
 
   // x and y sums of a finished batch (planes kb .. kb+n-1):
@@ -565,8 +577,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RPW <= 4 ? 
   };
 
   PlaneIn cur, nxt;
-  ring_load(kz0, std::integral_constant<int, 0>());
-  ring_load(min(kz0 + 1, kz1 - 1), std::integral_constant<int, 1>());
+  ring_load(kz0, std::integral_constant<int, 0>(), false);
+  ring_load(kz0 + 1, std::integral_constant<int, 1>(), kz0 + 1 >= kz1);
   stage_load(kz0);
   load(cur, kz0);
   stage_store();
