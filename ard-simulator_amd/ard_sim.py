@@ -32,7 +32,7 @@ EXPORTED = [
     "ard_adjoint", "ard_batch_create", "ard_batch_step", "ard_batch_fetch_recorders", "ard_batch_context",
     "ard_batch_room_partition", "ard_batch_destroy", "ard_set_recorder_cubes", "ard_fetch_recorder_cubes",
     "ard_render_view", "ard_selftest_fold", "ard_dct3d", "ard_dct3d_noalloc", "ard_selftest_fft",
-    "ard_selftest_fold_bf16", "ard_check_finite", "ard_find_boundary",
+    "ard_selftest_fold_bf16", "ard_check_finite", "ard_find_boundary", "ard_selftest_fold_direct",
 ]
 PHASES = ["sources", "pml", "fold", "face_fwd1", "face_fwd2", "modal", "unfold", "face_inv1", "face_inv2",
           "interface", "record", "dct_fwd", "dct_inv"]
@@ -99,6 +99,7 @@ def lib():
         L.ard_selftest_fold.argtypes = [i, i, i, i, i, fp, fp]
         L.ard_selftest_fft.argtypes = [i, i, i, i, i, fp, fp]
         L.ard_selftest_fold_bf16.argtypes = [i, i, i, i, i, fp, fp]
+        L.ard_selftest_fold_direct.argtypes = [i, i, i, i, i, fp, fp]
         L.ard_dct3d.argtypes = [i, i, i, i, i, vp, vp, f, vp]
         L.ard_batched_config.argtypes = [vp, ip]
         L.ard_set_batched_tiling.argtypes = [vp, ctypes.c_int, ctypes.c_int]

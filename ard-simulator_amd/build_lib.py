@@ -14,7 +14,7 @@ OBJ = os.path.join(HERE, "_build")
 
 HIP_SOURCES = ["ard_kernels.hip", "ard_batched_kernels.hip", "ard_batched.hip", "ard_ctx.cpp", "ard_plain.cpp",
                "ard_adjoint_kernels.hip", "ard_adjoint.cpp", "ard_batch.cpp", "ard_output.hip",
-               "ard_view.hip", "ard_dct3d.hip", "ard_fft.hip", "ard_check.hip"]
+               "ard_view.hip", "ard_dct3d.hip", "ard_fft.hip", "ard_check.hip", "ard_fold_direct.hip"]
 CXX_SOURCES = ["ard_topology.cpp", "ard_masks.cpp"]
 HOST_SOURCES = ["host/ard_host.cpp"]
 

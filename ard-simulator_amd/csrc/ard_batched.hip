@@ -98,6 +98,9 @@ struct BatchedState {
     int2* fold_tiles[4] = {};      // their tile tables (fold_tile_table)
     int gemm_fold_tiles[4] = {};
     GemmDesc* gemm_bf16[4] = {};   // the folded passes in BF16 split precision (variant 2), fold_tiles' tiles
+    GemmDesc* gemm_direct[4] = {}; // the folded passes, operands loaded straight into registers (variant 5)
+    int2* direct_tiles[4] = {};
+    int gemm_direct_tiles[4] = {};
     GemmDesc* gemm_fft[4] = {};    // the same passes FFT-style (variant 4), when every length factors
     int gemm_fft_tiles[4] = {};
     ModalGroup modal;
@@ -108,7 +111,7 @@ struct BatchedState {
   std::vector<int> dct_ids;
   std::vector<FoldDesc> h_fold;
   std::vector<UnfoldDesc> h_unfold;
-  std::vector<GemmDesc> h_gemm[4], h_gemm_fold[4], h_gemm_bf16[4], h_gemm_fft[4];
+  std::vector<GemmDesc> h_gemm[4], h_gemm_fold[4], h_gemm_bf16[4], h_gemm_fft[4], h_gemm_direct[4];
   std::vector<ModalDesc> h_modal;
   size_t fft_lds[4] = {};
   bool has_fft = false;
@@ -288,16 +291,18 @@ int fold_tables(Ctx& c, int L, const float** fwd, const float** inv) {
   return ARD_OK;
 }
 
-// Tile table of a folded transform launch: every (descriptor, tile) pair.
+// Tile table of a folded transform launch: every (descriptor, tile) pair,
+// tiles of tile_m rows (64 for k_gemm_mfma_fold, 32 pair rows for
+// k_fold_direct) by the descriptor's tiles_n column tiles.
 // A panel (one n tile of a descriptor, all its m tiles) shares its B operand,
 // so a panel goes to one XCD (block b runs on XCD b % 8); panels are dealt
 // longest K first to the XCD with the least work so far, and each XCD's
 // blocks run longest first.
-int fold_tile_table(Ctx& c, const std::vector<GemmDesc>& v, int2** d) {
+int fold_tile_table(Ctx& c, const std::vector<GemmDesc>& v, int2** d, int tile_m) {
   struct Panel { int desc, nt, tm; };
   std::vector<Panel> panels;
   for (int i = 0; i < int(v.size()); ++i) {
-    const int tm = int(ceil_div(v[i].M, 64));
+    const int tm = int(ceil_div(v[i].M, tile_m));
     for (int nt = 0; nt < v[i].tiles_n; ++nt) panels.push_back(Panel{i, nt, tm});
   }
   std::stable_sort(panels.begin(), panels.end(), [&](const Panel& a, const Panel& b) { return v[a.desc].K > v[b.desc].K; });
@@ -328,10 +333,14 @@ int fold_tile_table(Ctx& c, const std::vector<GemmDesc>& v, int2** d) {
 
 // One face-transform launch k (0, 1 forward; 2, 3 inverse) of a group of
 // partitions with transform kernel v: 3 the even/odd folded matrix-core
-// passes, 4 the FFT-style passes, else the dense passes of launch_gemm (0, 1, 2).
+// passes (5 the same with operands loaded straight into registers), 4 the
+// FFT-style passes, else the dense passes of launch_gemm (0, 1, 2).
 int launch_face(const BatchedState& B, const BatchedState::Stage& S, int k, int v, hipStream_t st) {
   if (v == 3)
     return launch_gemm_fold(S.gemm_fold[k], S.fold_tiles[k], S.gemm_fold_tiles[k], k % 2 == 0, k < 2 ? 1 : 2, st);
+  if (v == 5)
+    return launch_fold_direct(S.gemm_direct[k], S.direct_tiles[k], S.gemm_direct_tiles[k], k % 2 == 0, k < 2 ? 1 : 2,
+                              st);
   if (v == 4)
     return launch_fft(S.gemm_fft[k], S.ngemm[k], S.gemm_fft_tiles[k], k % 2 == 0, k % 2 == 0, 0, B.fft_lds[k], st);
   if (v == 2)
@@ -370,7 +379,16 @@ int build_stage(Ctx& c) {
     if ((r = upload_gemms(c, g, &S.gemm[k], &S.gemm_tiles[k]))) return r;
     std::vector<GemmDesc> gf = B.h_gemm_fold[k];
     if ((r = upload_gemms(c, gf, &S.gemm_fold[k], &S.gemm_fold_tiles[k]))) return r;
-    if ((r = fold_tile_table(c, gf, &S.fold_tiles[k]))) return r;
+    if ((r = fold_tile_table(c, gf, &S.fold_tiles[k], 64))) return r;
+    {
+      const std::vector<GemmDesc>& gd = B.h_gemm_direct[k];
+      int t = 0;
+      for (const GemmDesc& x : gd) t += int(ceil_div(x.M, 32)) * x.tiles_n;
+      S.gemm_direct_tiles[k] = t;
+      if ((r = upload_vec(c, (void**)&S.gemm_direct[k], gd.data(), gd.size() * sizeof(GemmDesc))) ||
+          (r = fold_tile_table(c, gd, &S.direct_tiles[k], 32)))
+        return r;
+    }
     const std::vector<GemmDesc>& gb = B.h_gemm_bf16[k];
     if ((r = upload_vec(c, (void**)&S.gemm_bf16[k], gb.data(), gb.size() * sizeof(GemmDesc)))) return r;
     if (B.has_fft) {
@@ -762,6 +780,13 @@ int finalize_batched(Ctx& c) {
       g.tiles_n = int(ceil_div(g.N, 64));
     }
     B.h_gemm_fold[k] = gf;
+    // k_fold_direct's twins: M counts pair rows, tiles of 32 pair rows x 128 columns
+    std::vector<GemmDesc> gd = gf;
+    for (GemmDesc& g : gd) {
+      g.M = (g.L + 1) / 2;
+      g.tiles_n = int(ceil_div(g.N, 128));
+    }
+    B.h_gemm_direct[k] = gd;
     // and their BF16 split twins: the tables' high and low parts
     std::vector<GemmDesc> gb = gf;
     for (GemmDesc& g : gb) {
@@ -960,40 +985,73 @@ int autotune(Ctx& c) {
     hipEventElapsedTime(&ms, e0, e1);
     return ms / reps;
   };
-  int best_rpw = 4, best_kzc = 48, best_ring = 0, tried = 0;
-  float best = 1e30f;
+  // Every candidate is timed in two rounds (three in the transforms') and
+  // keeps its fastest: the first launches after set-up run on a GPU whose
+  // clocks are still rising, which would favour the candidates timed last.
+  struct Tiling { int rpw, kzc, ring; };
+  std::vector<Tiling> tilings;
+  for (int rpw : {2, 4, 6, 8})
+    for (int kzc : {16, 32, 48, 96})
+      for (int ring : {0, 2})
+        if (!ring || rpw == 4) tilings.push_back(Tiling{rpw, kzc, ring});
+  std::vector<float> tms(tilings.size(), 1e30f);
+  int tried = 0;
   int r = ARD_OK;
-  for (int rpw : {2, 4, 6, 8}) {
-    for (int kzc : {16, 32, 48, 96}) {
-      for (int ring : {0, 2}) {
-        if (ring && rpw != 4) continue;
-        B.ring = ring;
-        if ((r = apply_tiling(c, rpw, kzc))) break;
-        const BatchedState::Stage& S = B.stage;
-        const float ms = time_ms([&] {
-          const BatchedState::ModalGroup& g = S.modal;
-          launch_modal(g.rpw, g.pts, g.ring, g.d_desc[0], g.d_start, g.n, g.blocks, 0, c.stream);
-          launch_unfold(S.unfold, S.unfold_start, S.nunfold, S.unfold_total, c.stream);
-        }, 3);
-        tried++;
-        if (ms < best) { best = ms; best_rpw = rpw; best_kzc = kzc; best_ring = ring; }
-      }
-      if (r) break;
+  for (int round = 0; round < 2 && !r; ++round) {
+    for (size_t i = 0; i < tilings.size(); ++i) {
+      B.ring = tilings[i].ring;
+      if ((r = apply_tiling(c, tilings[i].rpw, tilings[i].kzc))) break;
+      const BatchedState::Stage& S = B.stage;
+      const float ms = time_ms([&] {
+        const BatchedState::ModalGroup& g = S.modal;
+        launch_modal(g.rpw, g.pts, g.ring, g.d_desc[0], g.d_start, g.n, g.blocks, 0, c.stream);
+        launch_unfold(S.unfold, S.unfold_start, S.nunfold, S.unfold_total, c.stream);
+      }, 3);
+      tms[i] = std::min(tms[i], ms);
     }
-    if (r) break;
   }
-  B.ring = best_ring;
+  tried += int(tilings.size());
+  const size_t bt = size_t(std::min_element(tms.begin(), tms.end()) - tms.begin());
+  const int best_rpw = tilings[bt].rpw, best_kzc = tilings[bt].kzc;
+  B.ring = tilings[bt].ring;
   // the FP32 transform kernels compete on speed; BF16 split precision, when
-  // requested, is kept (a precision choice, not a speed one)
-  float bestg = 1e30f;
-  for (int v : {0, 1, 3, 4}) {
-    if (B.gemm_variant == 2) break;
-    if (v == 4 && !B.has_fft) continue;
-    const float ms = time_ms([&] {
-      for (int k = 0; k < 4; ++k) launch_face(B, B.stage, k, v, c.stream);
-    }, 3);
-    tried++;
-    if (ms < bestg) { bestg = ms; B.gemm_variant = v; }
+  // requested, is kept (a precision choice, not a speed one). They are timed
+  // as the step graph runs them: the fold, then the forward passes with the
+  // PML slabs beside them on the second stream, then the inverse passes (the
+  // kernels share the CUs, and one that issues more memory instructions
+  // loses more to the PML than it gains alone). The PML launches write its
+  // next-step buffers from the zero state and the step-0 source samples, so
+  // its pressure and phi buffers are cleared again afterwards.
+  if (B.gemm_variant != 2) {
+    std::vector<int> cand;
+    for (int v : {0, 1, 3, 4, 5})
+      if (v != 4 || B.has_fft) cand.push_back(v);
+    const BatchedState::Stage& S = B.stage;
+    hipStream_t st = c.stream;
+    int e = 0;
+    auto faces = [&](int v) {
+      const bool fork = B.side && B.npml > 0;
+      if (fork) e |= hipEventRecord(B.ev_fork, st) != hipSuccess;
+      e |= launch_fold(S.fold, S.fold_start, S.nfold, S.fold_total, st);
+      if (fork) {
+        e |= hipStreamWaitEvent(B.side, B.ev_fork, 0) != hipSuccess;
+        e |= launch_pml(B.d_pml, B.npml, B.pml_blocks, 0, c.prm.pml_layers, c.prm.h, c.prm.dt, c.prm.c0, B.side);
+        e |= hipEventRecord(B.ev_join, B.side) != hipSuccess;
+      }
+      e |= launch_face(B, S, 0, v, st);
+      e |= launch_face(B, S, 1, v, st);
+      if (fork) e |= hipStreamWaitEvent(st, B.ev_join, 0) != hipSuccess;
+      e |= launch_face(B, S, 2, v, st);
+      e |= launch_face(B, S, 3, v, st);
+    };
+    std::vector<float> gms(cand.size(), 1e30f);
+    for (int round = 0; round < 3; ++round)
+      for (size_t i = 0; i < cand.size(); ++i) gms[i] = std::min(gms[i], time_ms([&] { faces(cand[i]); }, 3));
+    tried += int(cand.size());
+    B.gemm_variant = cand[size_t(std::min_element(gms.begin(), gms.end()) - gms.begin())];
+    for (size_t i = 0; i < c.pml.size() && !e; ++i)
+      if (c.pml[i].p[0]) e |= c.dzero(c.pml[i].p[0], size_t(c.topo.parts[i].cells()) * 8 * 4) != ARD_OK;
+    if (e && !r) r = set_error(ARD_ERR_HIP, "autotuner: transform timing");
   }
   if (!r) r = apply_tiling(c, best_rpw, best_kzc);
   B.graphs_valid = false;
@@ -1167,11 +1225,13 @@ void destroy_batched(Ctx& c) {
   c.bs = nullptr;
 }
 
-// Self-test of the folded transform kernel (ard_selftest_fold): one
-// transform of length L over N columns, built exactly as the batched form
-// builds its face passes (fold_tables, a folded GemmDesc, fold_tile_table), in
-// a temporary context of its own.
-int fold_selftest(int device, int inverse, int b_layout, int L, int N, const float* X, float* Y, bool bf16) {
+// Self-test of the folded transform kernels (ard_selftest_fold, _bf16,
+// _direct): one transform of length L over N columns, built exactly as the
+// batched form builds its face passes (fold_tables, a folded GemmDesc,
+// fold_tile_table), in a temporary context of its own. kind: 0 LDS-staged
+// FP32, 1 BF16 split, 2 direct-to-register FP32.
+int fold_selftest(int device, int inverse, int b_layout, int L, int N, const float* X, float* Y, int kind) {
+  const bool bf16 = kind == 1;
   hipError_t e = hipSetDevice(device);
   if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
   Ctx c;
@@ -1212,12 +1272,18 @@ int fold_selftest(int device, int inverse, int b_layout, int L, int N, const flo
     g.Alo = inverse ? t[3] : t[1];
   }
   g.tiles_n = int(ceil_div(N, 64));
+  const int tile_m = kind == 2 ? 32 : 64;
+  if (kind == 2) {  // pair rows, tiles of 32 x 128
+    g.M = (L + 1) / 2;
+    g.tiles_n = int(ceil_div(N, 128));
+  }
   std::vector<GemmDesc> v{g};
   GemmDesc* dd;
   int2* dt;
-  if ((r = upload_vec(c, (void**)&dd, v.data(), sizeof(GemmDesc))) || (r = fold_tile_table(c, v, &dt))) return r;
-  const int tiles = int(ceil_div(g.M, 64)) * g.tiles_n;
-  if ((bf16 ? launch_gemm_bf16_fold : launch_gemm_fold)(dd, dt, tiles, b_layout == 1, inverse ? 2 : 1, c.stream))
+  if ((r = upload_vec(c, (void**)&dd, v.data(), sizeof(GemmDesc))) || (r = fold_tile_table(c, v, &dt, tile_m))) return r;
+  const int tiles = int(ceil_div(g.M, tile_m)) * g.tiles_n;
+  auto launch = kind == 2 ? launch_fold_direct : kind == 1 ? launch_gemm_bf16_fold : launch_gemm_fold;
+  if (launch(dd, dt, tiles, b_layout == 1, inverse ? 2 : 1, c.stream))
     return set_error(ARD_ERR_HIP, "folded transform launch");
   if ((e = hipMemcpyAsync(Y, dY, n * 4, hipMemcpyDeviceToHost, c.stream)) != hipSuccess ||
       (e = hipStreamSynchronize(c.stream)) != hipSuccess)

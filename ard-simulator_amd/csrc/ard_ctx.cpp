@@ -552,7 +552,7 @@ int ard_batched_option(ard_ctx* ctx, int option, int* value) {
 }
 
 int ard_set_transform_variant(int variant) {
-  if (variant < 0 || variant > 4) return fail(ARD_ERR_INVALID, "unknown transform variant");
+  if (variant < 0 || variant > 5) return fail(ARD_ERR_INVALID, "unknown transform variant");
   g_gemm_variant = variant;
   return ARD_OK;
 }
@@ -608,13 +608,19 @@ int ard_selftest_fold(int device, int inverse, int b_layout, int L, int N, const
   // L >= 6: partition sides are at least 6 cells, and the kernel relies on it
   if (!X || !Y || L < 6 || N <= 0 || b_layout < 0 || b_layout > 1) return fail(ARD_ERR_INVALID, "bad arguments");
   if ((long long)L * N + 16 >= (1LL << 30)) return fail(ARD_ERR_INVALID, "too large");
-  return fold_selftest(device, inverse != 0, b_layout, L, N, X, Y, false);
+  return fold_selftest(device, inverse != 0, b_layout, L, N, X, Y, 0);
 }
 
 int ard_selftest_fold_bf16(int device, int inverse, int b_layout, int L, int N, const float* X, float* Y) {
   if (!X || !Y || L < 6 || N <= 0 || b_layout < 0 || b_layout > 1) return fail(ARD_ERR_INVALID, "bad arguments");
   if ((long long)L * N + 16 >= (1LL << 30)) return fail(ARD_ERR_INVALID, "too large");
-  return fold_selftest(device, inverse != 0, b_layout, L, N, X, Y, true);
+  return fold_selftest(device, inverse != 0, b_layout, L, N, X, Y, 1);
+}
+
+int ard_selftest_fold_direct(int device, int inverse, int b_layout, int L, int N, const float* X, float* Y) {
+  if (!X || !Y || L < 6 || N <= 0 || b_layout < 0 || b_layout > 1) return fail(ARD_ERR_INVALID, "bad arguments");
+  if ((long long)L * N + 16 >= (1LL << 30)) return fail(ARD_ERR_INVALID, "too large");
+  return fold_selftest(device, inverse != 0, b_layout, L, N, X, Y, 2);
 }
 
 }  // extern "C"

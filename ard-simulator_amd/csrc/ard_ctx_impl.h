@@ -155,7 +155,7 @@ void batched_state_arrays(Ctx& c, int part, std::vector<std::pair<const float*, 
 // ard_check_finite: the lowest partition id whose state holds a NaN or an
 // infinity, or -1; synchronous.
 int check_finite(Ctx& c, int* bad_part);
-int fold_selftest(int device, int inverse, int b_layout, int L, int N, const float* X, float* Y, bool bf16);
+int fold_selftest(int device, int inverse, int b_layout, int L, int N, const float* X, float* Y, int kind);
 // FFT-style transforms (ard_fft.hip): the FFT fields of a dense pass
 // descriptor (its plan cached in the context; ARD_ERR_UNSUPPORTED for a
 // length fft_factor refuses), its tile count, and the entry self-test

@@ -21,6 +21,12 @@ int launch_gemm_fold(const GemmDesc* d_descs, const int2* d_tiles, int total_til
 // the folded tables' BF16 high and low parts.
 int launch_gemm_bf16_fold(const GemmDesc* d_descs, const int2* d_tiles, int total_tiles, bool b_k_contig, int fold,
                           hipStream_t s);
+// The same FP32 folded transforms with each wave's operands loaded straight
+// into registers (no LDS, no barriers; ard_fold_direct.hip): the descriptors
+// of launch_gemm_fold with M = ceil(L/2) pair rows and tiles_n = ceil(N/128);
+// block tiles of 32 pair rows x 128 columns (fold_tile_table with tile_m 32).
+int launch_fold_direct(const GemmDesc* d_descs, const int2* d_tiles, int total_tiles, bool b_k_contig, int fold,
+                       hipStream_t s);
 // Blocks of one slab's update for PML thickness L (pml_layers); sets d.nb.
 // Slabs L cells thick along their normal, for L from 3 to 8, take the
 // LDS-tiled update; others (thicker layers, or slabs of another thickness

@@ -816,7 +816,7 @@ int launch_gemm(const GemmDesc* d_descs, int ndesc, int total_tiles, bool b_k_co
   if (total_tiles <= 0) return 0;
   dim3 grid(total_tiles), block(256);
   if (variant < 0) variant = g_gemm_variant;
-  if (variant == 3) variant = 1;  // folding applies to the batched form's face passes; dense passes use 1
+  if (variant == 3 || variant == 5) variant = 1;  // folding applies to the batched form's face passes; dense passes use 1
   if (variant == 2) {
     if (b_k_contig && ep == 0) hipLaunchKernelGGL((k_gemm_bf16x3<true, 0>), grid, block, 0, s, d_descs, ndesc);
     else if (!b_k_contig && ep == 0) hipLaunchKernelGGL((k_gemm_bf16x3<false, 0>), grid, block, 0, s, d_descs, ndesc);

@@ -208,9 +208,9 @@ int finalize_plain(Ctx& c) {
   if ((r = c.upload(c.d_gemm, c.h_gemm.data(), c.h_gemm.size() * sizeof(GemmDesc)))) return r;
   // transform kernel of the passes, fixed here: the FFT-style passes
   // (variant 4) when every axis length factors into radices up to 13, else
-  // the dense passes (variant 3, the folded face passes of the batched form,
-  // means matrix cores here)
-  c.plain_variant = g_gemm_variant == 3 ? 1 : g_gemm_variant;
+  // the dense passes (variants 3 and 5, the folded face passes of the batched
+  // form, mean matrix cores here)
+  c.plain_variant = g_gemm_variant == 3 || g_gemm_variant == 5 ? 1 : g_gemm_variant;
   if (c.plain_variant == 4 && !c.h_gemm.empty()) {
     std::vector<GemmDesc> f = c.h_gemm;
     bool ok = true;

@@ -36,6 +36,8 @@ import ard_sim as A  # noqa: E402
 
 METRIC = "Gcell-steps/s on 1 MI355X, batched vs plain GPU at recorder rel-L2 ≤1e-4"
 HBM_PEAK_GBS = 8000.0
+# transform kernels by variant number (ard_set_transform_variant)
+TRANSFORM_NAMES = ["vector-fma", "mfma-fp32", "mfma-bf16x3", "mfma-fp32-folded", "fft", "mfma-fp32-folded-direct"]
 FP32_PEAK_TFLOPS = 157.3
 # algorithmic bytes per unit of work (dense formulation, FP32)
 B_DCT, B_PML, B_FACE = 20, 44, 48
@@ -275,7 +277,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": "%s: batched face-shell form, hipGraph per step" % sc.name,
                    "tiling": {"rows_per_wave": tiling[0], "kz_per_block": tiling[1],
-                              "transform": ["vector-fma", "mfma-fp32", "mfma-bf16x3", "mfma-fp32-folded"][tiling[2]], "autotune_candidates": tiling[3]}, "dct_cells": n_dct,
+                              "transform": TRANSFORM_NAMES[tiling[2]], "autotune_candidates": tiling[3]}, "dct_cells": n_dct,
                    "pml_cells": n_pml, "face_elements": faces, "boxes": len(sc.boxes), "h": sc.h, "dt": sc.dt,
                    "sources": len(sc.sources), "recorders": len(sc.recorders)},
         "roofline": roof,

@@ -242,6 +242,10 @@ int ard_selftest_fold(int device, int inverse, int b_layout, int L, int N, const
  * BF16, three products on v_mfma_f32_32x32x16_bf16, FP32 accumulate). No
  * counterpart in the reference. */
 int ard_selftest_fold_bf16(int device, int inverse, int b_layout, int L, int N, const float* X, float* Y);
+/* The same self-test for the folded FP32 transform with each wave's operands
+ * loaded straight into registers (variant 5's batched face passes: no LDS
+ * staging, no barriers). No counterpart in the reference. */
+int ard_selftest_fold_direct(int device, int inverse, int b_layout, int L, int N, const float* X, float* Y);
 /* Self-test of the FFT-style transform kernel (variant 4; no counterpart in the
  * reference, whose FFTW plans it replaces, ARD/dct_volume.cpp:17,19): the same
  * transform as ard_selftest_fold, for L >= 2 whose prime factors are at most 13
@@ -251,11 +255,13 @@ int ard_selftest_fft(int device, int inverse, int b_layout, int L, int N, const 
 /* Selects the dense transform kernels used by contexts finalized afterwards
  * (0 vector FMA, 1 matrix cores FP32, 2 matrix cores BF16 split, 3 matrix
  * cores FP32 with the even/odd folding c(k, n-1-j) = (-1)^k c(k, j), half the
- * multiply-adds; 3 applies to the batched form's face passes, other passes
- * use 1). Default 1. The batched form's autotuner may choose among 0, 1 and 3
- * (same precision); it keeps 2. 4: FFT-style passes (Makhoul's reordering and
- * mixed-radix FFTs in LDS, O(L log L) per line) for every pass, in either form,
- * when every axis length's prime factors are at most 13 (else 1). */
+ * multiply-adds; 5 the same folded FP32 transforms with each wave's operands
+ * loaded straight into registers, no LDS staging; 3 and 5 apply to the
+ * batched form's face passes, other passes use 1). Default 1. The batched
+ * form's autotuner may choose among 0, 1, 3, 4 and 5 (same precision); it
+ * keeps 2. 4: FFT-style passes (Makhoul's reordering and mixed-radix FFTs in
+ * LDS, O(L log L) per line) for every pass, in either form, when every axis
+ * length's prime factors are at most 13 (else 1). */
 int ard_set_transform_variant(int variant);
 /* Tiling of the batched form after ard_finalize (chosen by the autotuner when
  * ard_params.autotune is set): out[4] = rows per wave, kz planes per block,

@@ -33,20 +33,23 @@ def test_transform_kernel_matches_numpy(variant, b_layout, shape):
 FOLD_L = [6, 7, 9, 11, 17, 33, 40, 65, 199, 200, 271]
 
 
+@pytest.mark.parametrize("entry", ["ard_selftest_fold", "ard_selftest_fold_direct"])
 @pytest.mark.parametrize("inverse", [False, True])
 @pytest.mark.parametrize("b_layout", [0, 1])
 @pytest.mark.parametrize("L", FOLD_L)
 @pytest.mark.parametrize("N", [1, 37, 130])
-def test_folded_transform_matches_numpy(inverse, b_layout, L, N):
-    """The even/odd folded matrix-core transform (variant 3, the kernel of the
-    hall's face passes) against scipy's REDFT10 / REDFT01 (FFTW's definitions)."""
+def test_folded_transform_matches_numpy(entry, inverse, b_layout, L, N):
+    """The even/odd folded matrix-core transforms against scipy's REDFT10 /
+    REDFT01 (FFTW's definitions): the LDS-staged kernel (variant 3) and the
+    one that loads each wave's operands straight into registers (variant 5),
+    on odd and even lengths, K tails and partial tiles."""
     import scipy.fft
 
     rng = np.random.default_rng(L * 131 + N * 7 + 2 * inverse + b_layout)
     x = rng.standard_normal((L, N)).astype(np.float32)
     ref = scipy.fft.dct(x.astype(np.float64), type=3 if inverse else 2, axis=0, norm=None)
     xin = x if b_layout == 0 else np.ascontiguousarray(x.T)
-    y = A.selftest_fold(xin, inverse=inverse, b_layout=b_layout)
+    y = A.selftest_fold(xin, inverse=inverse, b_layout=b_layout, entry=entry)
     if b_layout == 1:
         y = y.T
     err = np.abs(y - ref).max() / np.abs(ref).max()
