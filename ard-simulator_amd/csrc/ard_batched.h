@@ -85,8 +85,31 @@ struct ModalDesc {
 };
 
 struct BRecDesc {
-  const float* part;  // first block partial of the owner, stride ARD_MAX_REC_PER_PART
+  const float* part;  // this recorder's partial of the owner's first block
   int nblocks;
+  int stride;         // floats between consecutive blocks' partials
+};
+
+// Interior point sources of a partition past the modal pass's
+// ARD_MAX_SRC_PER_PART (k_xsrc), and recorders past its
+// ARD_MAX_REC_PER_PART (k_xrec): rows of 16 (kz, ky) mode rows per block.
+constexpr int kXRows = 16;
+struct XsrcDesc {
+  float* M[2];  // modes by step parity; the pass adjusts M[par ^ 1]
+  int W, H, D, Wp;
+  const float* wx2;
+  const float* wy2;
+  const float* wz2;
+  float gscale;
+  int n;
+  const SrcTab* src;
+};
+struct XrecDesc {
+  const float* M[2];  // modes by step parity; the pass reads the new ones, M[par ^ 1]
+  int W, H, D, Wp;
+  int n;
+  const RecTab* rec;
+  float* part;  // [blocks][n]
 };
 
 int launch_fold(const FoldDesc* d, const long long* starts, int n, long long total, hipStream_t s);
@@ -96,6 +119,9 @@ int launch_bsources(const BSourceDesc* d, int n, const int* d_step, float couran
 int launch_modal(int rpw, bool pts, int ring, const ModalDesc* d, const long long* starts, int n, long long blocks,
                  int par, hipStream_t s);
 // records the step's samples, advances *d_step and writes the next step's source samples
+// the extra point passes (before and after the modal pass)
+int launch_xsrc(const XsrcDesc* d, const long long* starts, int n, long long blocks, int par, hipStream_t s);
+int launch_xrec(const XrecDesc* d, const long long* starts, int n, long long blocks, int par, hipStream_t s);
 int launch_brecord(const BRecDesc* d, int n, float* trace, int cap, int* d_step, const BSourceDesc* src, int nsrc,
                    float courant, hipStream_t s);
 

@@ -58,6 +58,10 @@ struct BPart {
   float* R[3] = {};   // Rx, Ry, Rz
   float* yz[2] = {};  // y, z: [n][8] = {E0, E1, E2, R0, R1, R2, u, 0}
   float* rec_part = nullptr;
+  // recorders past the modal pass's ARD_MAX_REC_PER_PART: block partials of
+  // k_xrec, [xnb][recs.size() - ARD_MAX_REC_PER_PART]
+  float* xrec_part = nullptr;
+  int xnb = 0;
   int rpw = 4, nbx = 0, nby = 0, nbz = 0, kz_chunk = 0;
   int nrec = 0;
   std::vector<int> recs;  // recorder ids owned
@@ -104,6 +108,9 @@ struct BatchedState {
     GemmDesc* gemm_fft[4] = {};    // the same passes FFT-style (variant 4), when every length factors
     int gemm_fft_tiles[4] = {};
     ModalGroup modal;
+    // point sources and recorders past the modal pass's four per partition
+    XsrcDesc* xsrc = nullptr; long long* xsrc_start = nullptr; int nxsrc = 0; long long xsrc_blocks = 0;
+    XrecDesc* xrec = nullptr; long long* xrec_start = nullptr; int nxrec = 0; long long xrec_blocks = 0;
   };
   Stage stage;
   // host copies of the per-partition descriptors in descriptor order (three
@@ -496,6 +503,8 @@ int build_modal(Ctx& c) {
   BatchedState& B = *c.bs;
   int r;
   B.h_modal.clear();
+  std::vector<XsrcDesc> xs;
+  std::vector<XrecDesc> xr;
   for (int i = 0; i < int(c.topo.parts.size()); ++i) {
     const PartGeom& p = c.topo.parts[i];
     if (p.is_pml) continue;
@@ -515,9 +524,20 @@ int build_modal(Ctx& c) {
     std::vector<SrcTab> st;
     for (auto& s : B.srcs)
       if (s.part == i && !s.in_shell) st.push_back(SrcTab{s.tab, s.tab + p.w, s.tab + p.w + p.h, s.slot});
+    if (int(st.size()) > ARD_MAX_SRC_PER_PART) {  // the rest: k_xsrc
+      XsrcDesc x{};
+      x.M[0] = P.M[0]; x.M[1] = P.M[1];
+      x.W = p.w; x.H = p.h; x.D = p.d; x.Wp = P.wp;
+      x.wx2 = P.w2[0]; x.wy2 = P.w2[1]; x.wz2 = P.w2[2];
+      x.gscale = P.gscale;
+      x.n = int(st.size()) - ARD_MAX_SRC_PER_PART;
+      SrcTab* ds;
+      if ((r = upload_vec(c, (void**)&ds, st.data() + ARD_MAX_SRC_PER_PART, size_t(x.n) * sizeof(SrcTab)))) return r;
+      x.src = ds;
+      xs.push_back(x);
+      st.resize(ARD_MAX_SRC_PER_PART);
+    }
     d.nsrc = int(st.size());
-    if (d.nsrc > ARD_MAX_SRC_PER_PART)
-      return set_error(ARD_ERR_UNSUPPORTED, "more than 4 interior point forces in one partition");
     if (d.nsrc) {
       SrcTab* ds;
       if ((r = upload_vec(c, (void**)&ds, st.data(), st.size() * sizeof(SrcTab)))) return r;
@@ -534,6 +554,18 @@ int build_modal(Ctx& c) {
       if ((r = upload_vec(c, (void**)&dt, t.data(), t.size() * 4))) return r;
       rt.push_back(RecTab{dt, dt + p.w, dt + p.w + p.h});
     }
+    if (int(rt.size()) > ARD_MAX_REC_PER_PART) {  // the rest: k_xrec
+      XrecDesc x{};
+      x.M[0] = P.M[0]; x.M[1] = P.M[1];
+      x.W = p.w; x.H = p.h; x.D = p.d; x.Wp = P.wp;
+      x.n = int(rt.size()) - ARD_MAX_REC_PER_PART;
+      RecTab* dr;
+      if ((r = upload_vec(c, (void**)&dr, rt.data() + ARD_MAX_REC_PER_PART, size_t(x.n) * sizeof(RecTab)))) return r;
+      x.rec = dr;
+      x.part = P.xrec_part;
+      xr.push_back(x);
+      rt.resize(ARD_MAX_REC_PER_PART);
+    }
     d.nrec = int(rt.size());
     if (d.nrec) {
       RecTab* dr;
@@ -547,7 +579,7 @@ int build_modal(Ctx& c) {
       std::vector<float> ty(size_t(p.h) * 8, 0.f), tz(size_t(p.d) * 8, 0.f), col(size_t(std::max(p.h, p.d)));
       int q = 0;
       for (auto& s : B.srcs) {
-        if (s.part != i || s.in_shell) continue;
+        if (s.part != i || s.in_shell || q == ARD_MAX_SRC_PER_PART) continue;
         point_table(p.h, s.y, true, col.data());
         for (int k = 0; k < p.h; ++k) ty[size_t(k) * 8 + q] = col[size_t(k)];
         point_table(p.d, s.z, true, col.data());
@@ -556,6 +588,7 @@ int build_modal(Ctx& c) {
       }
       q = 0;
       for (int rid : P.recs) {
+        if (q == ARD_MAX_REC_PER_PART) break;
         const RecorderDef& rd = c.topo.recorders[rid];
         point_table(p.h, rd.ly, false, col.data());
         for (int k = 0; k < p.h; ++k) ty[size_t(k) * 8 + 4 + q] = col[size_t(k)];
@@ -570,6 +603,25 @@ int build_modal(Ctx& c) {
       d.ptz = dz;
     }
     B.h_modal.push_back(d);
+  }
+  // the extra point passes: one launch each over every partition that has them
+  BatchedState::Stage& S = B.stage;
+  S.nxsrc = int(xs.size());
+  S.nxrec = int(xr.size());
+  S.xsrc_blocks = S.xrec_blocks = 0;
+  if (S.nxsrc) {
+    std::vector<long long> st;
+    for (const XsrcDesc& x : xs) { st.push_back(S.xsrc_blocks); S.xsrc_blocks += ceil_div((long long)x.D * x.H, kXRows); }
+    if ((r = upload_vec(c, (void**)&S.xsrc, xs.data(), xs.size() * sizeof(XsrcDesc))) ||
+        (r = upload_vec(c, (void**)&S.xsrc_start, st.data(), st.size() * 8)))
+      return r;
+  }
+  if (S.nxrec) {
+    std::vector<long long> st;
+    for (const XrecDesc& x : xr) { st.push_back(S.xrec_blocks); S.xrec_blocks += ceil_div((long long)x.D * x.H, kXRows); }
+    if ((r = upload_vec(c, (void**)&S.xrec, xr.data(), xr.size() * sizeof(XrecDesc))) ||
+        (r = upload_vec(c, (void**)&S.xrec_start, st.data(), st.size() * 8)))
+      return r;
   }
   return build_stage_tiling(c);
 }
@@ -625,7 +677,11 @@ int apply_tiling(Ctx& c, int rpw, int kzc) {
   for (size_t q = 0; q < c.topo.recorders.size(); ++q) {
     const BPart& P = B.parts[c.topo.recorders[q].part];
     const int slot = int(std::find(P.recs.begin(), P.recs.end(), int(q)) - P.recs.begin());
-    rd.push_back(BRecDesc{P.rec_part + slot, P.nbx * P.nby * P.nbz});
+    if (slot < ARD_MAX_REC_PER_PART)
+      rd.push_back(BRecDesc{P.rec_part + slot, P.nbx * P.nby * P.nbz, ARD_MAX_REC_PER_PART});
+    else  // k_xrec's partials
+      rd.push_back(BRecDesc{P.xrec_part + (slot - ARD_MAX_REC_PER_PART), P.xnb,
+                            int(P.recs.size()) - ARD_MAX_REC_PER_PART});
   }
   if (!rd.empty() && (r = upload_vec(c, (void**)&B.d_rec, rd.data(), rd.size() * sizeof(BRecDesc)))) return r;
   B.graphs_valid = false;
@@ -655,8 +711,6 @@ int finalize_batched(Ctx& c) {
     if (p.is_pml) continue;
     B.dct_ids.push_back(i);
     BPart& P = B.parts[i];
-    if (int(P.recs.size()) > ARD_MAX_REC_PER_PART)
-      return set_error(ARD_ERR_UNSUPPORTED, "more than 4 recorders in one partition");
     const long long W = p.w, H = p.h, D = p.d, N = W * H * D;
     if (D * H * ceil_div(W, 32) * 32 >= (1LL << 30))  // 32-bit cell and byte offsets in the modal pass
       return set_error(ARD_ERR_UNSUPPORTED, "a DCT partition of 2^30 or more cells (split it)");
@@ -732,6 +786,11 @@ int finalize_batched(Ctx& c) {
       if ((r = upload_vec(c, (void**)&P.yz[ax - 1], t.data(), t.size() * 4))) return r;
     }
     if ((r = c.dalloc((void**)&P.rec_part, size_t(P.nbx) * P.nby * P.nbz * ARD_MAX_REC_PER_PART * 4))) return r;
+    if (int(P.recs.size()) > ARD_MAX_REC_PER_PART) {
+      P.xnb = int(ceil_div(D * H, kXRows));
+      const size_t nx = P.recs.size() - ARD_MAX_REC_PER_PART;
+      if ((r = c.dalloc((void**)&P.xrec_part, size_t(P.xnb) * nx * 4))) return r;
+    }
     for (int w = 0; w < 3; ++w) folds.push_back(FoldDesc{P.F[w], P.S[w], P.plane[w], vec_of(P.plane[w], {P.F[w], P.S[w]})});
     const float *fw, *iw, *fh, *ih, *fd, *id;
     if ((r = c.cos_tables(p.w, &fw, &iw)) || (r = c.cos_tables(p.h, &fh, &ih)) || (r = c.cos_tables(p.d, &fd, &id)))
@@ -907,8 +966,15 @@ int enqueue_step(Ctx& c, int par, PhaseTimer* T) {
   face(0);
   face(1);
   {
+    // with the extra point passes (partitions of more than four interior
+    // point sources or recorders) before and after it
     const BatchedState::ModalGroup& g = S.modal;
-    phase(ARD_PH_MODAL, st, [&] { return launch_modal(g.rpw, g.pts, g.ring, g.d_desc[par], g.d_start, g.n, g.blocks, par, st); });
+    phase(ARD_PH_MODAL, st, [&] {
+      int e = launch_xsrc(S.xsrc, S.xsrc_start, S.nxsrc, S.xsrc_blocks, par, st);
+      e |= launch_modal(g.rpw, g.pts, g.ring, g.d_desc[par], g.d_start, g.n, g.blocks, par, st);
+      e |= launch_xrec(S.xrec, S.xrec_start, S.nxrec, S.xrec_blocks, par, st);
+      return e;
+    });
   }
   phase(ARD_PH_UNFOLD, st, [&] { return launch_unfold(S.unfold, S.unfold_start, S.nunfold, S.unfold_total, st); });
   face(2);

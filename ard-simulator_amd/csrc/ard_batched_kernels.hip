@@ -641,6 +641,99 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RPW <= 4 ? 
 }
 
 // ---------------------------------------------------------------------------
+// Point sources and recorders of a partition past the modal pass's four of
+// each (ARD/simulation.cpp:37-48 and ARD/recorder.cpp:26-38 set no limit).
+//
+// Sources (k_xsrc, before the modal pass): the update
+//   M+ = a M - .999 M- + g Fh
+// is linear in M-, so subtracting g dF / .999 from M- (which the pass then
+// overwrites with M+) adds g dF to M+, and so to every face and recorder sum
+// the pass forms from M+; dF = sum_s f_s fx_s(kx) fy_s(ky) fz_s(kz) is the
+// extra sources' force spectrum, with their step samples from the value
+// slots, and g is formed as the pass forms it.
+// Recorders (k_xrec, after it): each one's sample is the sum over the new
+// modes of M+ fx fy fz (the inverse transforms evaluated at its cell);
+// per-block partials in FP32, summed over blocks in FP64 by k_brecord.
+// A block walks kXRows mode rows (kz, ky), four per wave, lanes along kx.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_xsrc(const XsrcDesc* __restrict__ descs, const long long* __restrict__ starts,
+                                              int ndesc, int par) {
+  const int di = find_desc_ll(starts, ndesc, blockIdx.x);
+  const XsrcDesc& X = descs[di];
+  const int lb = int(blockIdx.x - starts[di]);
+  const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int W = X.W, H = X.H, n = X.n;
+  gfloat* __restrict__ Mp = ARD_G(X.M[par ^ 1]);
+  const float rg = 1.f / 0.999f;
+  for (int rr = 0; rr < kXRows / 4; ++rr) {
+    const int row = lb * kXRows + wave * (kXRows / 4) + rr;  // kz * H + ky
+    if (row >= X.D * H) break;
+    const int kz = row / H, ky = row - kz * H;
+    const float uz = ARD_CG(X.wz2)[kz], uy = ARD_CG(X.wy2)[ky];
+    for (int kx0 = 0; kx0 < W; kx0 += 64) {
+      const int kx = kx0 + lane, kxc = min(kx, W - 1);
+      float f = 0.f;
+      for (int s = 0; s < n; ++s) {
+        const SrcTab& t = X.src[s];
+        f = fmaf(*t.val * ARD_CG(t.fy)[ky] * ARD_CG(t.fz)[kz], ARD_CG(t.fx)[kxc], f);
+      }
+      // u summed in the modal pass's order: (ux + uz) + uy
+      const float g = X.gscale * sinc2_poly((ARD_CG(X.wx2)[kxc] + uz) + uy);
+      if (kx < W) {
+        const long long o = (long long)row * X.Wp + kx;
+        Mp[o] = fmaf(-g * f, rg, Mp[o]);
+      }
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void k_xrec(const XrecDesc* __restrict__ descs, const long long* __restrict__ starts,
+                                              int ndesc, int par) {
+  constexpr int G = 8;  // recorders per round over the block's rows
+  __shared__ float s_part[4][G];
+  const int di = find_desc_ll(starts, ndesc, blockIdx.x);
+  const XrecDesc& X = descs[di];
+  const int lb = int(blockIdx.x - starts[di]);
+  const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int W = X.W, H = X.H, n = X.n;
+  const gfloat* __restrict__ Mn = ARD_CG(X.M[par ^ 1]);
+  for (int r0 = 0; r0 < n; r0 += G) {
+    float acc[G];
+#pragma unroll
+    for (int q = 0; q < G; ++q) acc[q] = 0.f;
+    for (int rr = 0; rr < kXRows / 4; ++rr) {
+      const int row = lb * kXRows + wave * (kXRows / 4) + rr;
+      if (row >= X.D * H) break;
+      const int kz = row / H, ky = row - kz * H;
+      float wyz[G];
+#pragma unroll
+      for (int q = 0; q < G; ++q) {
+        const RecTab& t = X.rec[min(r0 + q, n - 1)];
+        wyz[q] = r0 + q < n ? ARD_CG(t.fy)[ky] * ARD_CG(t.fz)[kz] : 0.f;
+      }
+      for (int kx0 = 0; kx0 < W; kx0 += 64) {
+        const int kx = kx0 + lane, kxc = min(kx, W - 1);
+        const float m = kx < W ? Mn[(long long)row * X.Wp + kx] : 0.f;
+#pragma unroll
+        for (int q = 0; q < G; ++q) acc[q] = fmaf(wyz[q] * ARD_CG(X.rec[min(r0 + q, n - 1)].fx)[kxc], m, acc[q]);
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < G; ++q) {
+      float v = acc[q];
+#pragma unroll
+      for (int m = 1; m < 64; m <<= 1) v += __shfl_xor(v, m);
+      if (lane == 0) s_part[wave][q] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < G && r0 + int(threadIdx.x) < n)
+      X.part[(long long)lb * n + r0 + threadIdx.x] =
+          s_part[0][threadIdx.x] + s_part[1][threadIdx.x] + s_part[2][threadIdx.x] + s_part[3][threadIdx.x];
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Recorder samples from the block partials, then the step counter, then the
 // next step's source samples: they overwrite their cells after this step's
 // interface forcing and before the next update, where the reference sets
@@ -662,7 +755,7 @@ __global__ __launch_bounds__(1024) void k_brecord(const BRecDesc* __restrict__ r
 #pragma unroll
       for (int j = 0; j < 8; ++j) {
         const int b = b0 + 64 * j + lane;
-        v[j] = b < nb ? part[(long long)b * ARD_MAX_REC_PER_PART] : 0.f;
+        v[j] = b < nb ? part[(long long)b * r[i].stride] : 0.f;
       }
 #pragma unroll
       for (int j = 0; j < 8; ++j) s += double(v[j]);
@@ -717,6 +810,18 @@ int launch_modal(int rpw, bool pts, int ring, const ModalDesc* d, const long lon
   ARD_MODAL_CASE(8, 0)
 #undef ARD_MODAL_CASE
   return -1;
+}
+
+int launch_xsrc(const XsrcDesc* d, const long long* starts, int n, long long blocks, int par, hipStream_t s) {
+  if (blocks <= 0) return 0;
+  hipLaunchKernelGGL(k_xsrc, dim3(unsigned(blocks)), dim3(256), 0, s, d, starts, n, par);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+int launch_xrec(const XrecDesc* d, const long long* starts, int n, long long blocks, int par, hipStream_t s) {
+  if (blocks <= 0) return 0;
+  hipLaunchKernelGGL(k_xrec, dim3(unsigned(blocks)), dim3(256), 0, s, d, starts, n, par);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 
 int launch_brecord(const BRecDesc* d, int n, float* trace, int cap, int* d_step, const BSourceDesc* src, int nsrc,

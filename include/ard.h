@@ -101,11 +101,12 @@ int ard_add_source(ard_ctx* ctx, int x, int y, int z, const float* wave, int n);
 int ard_add_recorder(ard_ctx* ctx, int x, int y, int z);
 
 /* Allocates device state and tables. After it, topology calls fail with ARD_ERR_STATE.
- * Limits of the batched form (the reference has none; the plain form has none either):
- * at most 4 point sources in a DCT partition's interior (more than 3 cells from its faces;
- * sources in the face slabs are not counted), at most 4 recorders per DCT partition, and
- * fewer than 2^30 cells per DCT partition. Beyond them: ARD_ERR_UNSUPPORTED (split the
- * partition or use the plain form). */
+ * Limit of the batched form (the reference has none; the plain form has none either):
+ * fewer than 2^30 cells per DCT partition; beyond it ARD_ERR_UNSUPPORTED (split the
+ * partition or use the plain form). Point sources and recorders per partition are not
+ * limited: the modal pass takes up to 4 of each in a DCT partition's interior, and a
+ * partition with more adds one pass over its modes for the extra sources (before the
+ * modal pass) and one for the extra recorders (after it). */
 int ard_finalize(ard_ctx* ctx);
 
 /* Topology queries. part_info[9] = is_pml, pml_type, parent, x0,y0,z0, w,h,d.
