@@ -312,10 +312,11 @@ class Simulation:
         return {PHASES[i]: (float(ms[i]), int(cnt[i])) for i in range(n) if cnt[i]}
 
     def adjoint(self, dl_drec, checkpoint_stride=0):
-        """Gradients of a recorder loss by the discrete adjoint (plain form).
+        """Gradients of a recorder loss by the discrete adjoint.
 
         dl_drec: [nsteps][nrec] array of dL/dy. Re-runs nsteps steps from the
-        zero state with checkpoints, then the reverse sweep. Returns
+        zero state with checkpoints, then the reverse sweep (a batched context
+        runs both on a plain-form twin of itself). Returns
         (dL/dwave [nsources][nsteps] float32, dL/dscale [6] float32)."""
         g = np.ascontiguousarray(dl_drec, dtype=np.float32)
         nsteps = g.shape[0]

@@ -262,6 +262,41 @@ int set_step(Ctx& c, int n) {
   return e == hipSuccess ? ARD_OK : hip_fail(e, "step counter");
 }
 
+// A batched context's adjoint runs on a plain-form twin: both forms compute
+// the same discrete step (the same DCT-II of the same force field, modal
+// update, PML and interface forcing; only the order of floating-point
+// additions differs), and the transposed kernels work on the plain form's
+// dense fields. The twin is built once, from the context's resolved topology,
+// parameters and PML damping scales (all fixed at finalize), and keeps its own
+// device state; the batched context's state is not touched.
+int plain_twin(Ctx& c, Ctx** out) {
+  if (!c.twin) {
+    auto* t = new Ctx();
+    t->device = c.device;
+    t->prm = c.prm;
+    t->prm.form = ARD_FORM_PLAIN;
+    t->prm.autotune = 0;
+    for (int i = 0; i < 6; ++i) t->pml_scale[i] = c.pml_scale[i];
+    t->topo = c.topo;
+    t->topology_built = true;
+    hipError_t e = hipStreamCreateWithFlags(&t->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+      t->stream = nullptr;
+      delete t;
+      return hip_fail(e, "hipStreamCreate");
+    }
+    const int r = finalize_plain(*t);
+    if (r != ARD_OK) {
+      delete t;
+      return r;
+    }
+    t->finalized = true;
+    c.twin = t;
+  }
+  *out = c.twin;
+  return ARD_OK;
+}
+
 }  // namespace
 
 }  // namespace ard
@@ -274,10 +309,16 @@ extern "C" int ard_adjoint(ard_ctx* ctx, int nsteps, const float* dl_drec, float
   if (!c) return set_error(ARD_ERR_INVALID, "null context");
   if (!c->finalized) return set_error(ARD_ERR_STATE, "not finalized");
   if (c->device < 0) return set_error(ARD_ERR_STATE, "host-only context");
-  if (c->prm.form != ARD_FORM_PLAIN) return set_error(ARD_ERR_UNSUPPORTED, "the adjoint runs on the plain form");
   if (nsteps <= 0 || !dl_drec) return set_error(ARD_ERR_INVALID, "nsteps and dl_drec are required");
   hipSetDevice(c->device);
   int r;
+  if (c->prm.form != ARD_FORM_PLAIN) {
+    Ctx* t = nullptr;
+    if ((r = plain_twin(*c, &t))) return r;
+    r = ard_adjoint(reinterpret_cast<ard_ctx*>(t), nsteps, dl_drec, dl_dwave, dl_dscale, checkpoint_stride);
+    hipSetDevice(c->device);
+    return r;
+  }
   if (!c->adj && (r = setup_adjoint(*c))) return r;
   AdjState& S = *c->adj;
   const int nrec = int(c->topo.recorders.size());

@@ -87,6 +87,8 @@ Ctx::~Ctx() {
   if (graph_exec[1]) hipGraphExecDestroy(graph_exec[1]);
   destroy_batched(*this);
   destroy_adjoint(*this);
+  delete twin;
+  if (device >= 0) hipSetDevice(device);
   destroy_output(*this);
   destroy_view(*this);
   for (void* p : allocs) hipFree(p);

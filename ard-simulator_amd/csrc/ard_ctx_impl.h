@@ -114,6 +114,8 @@ struct Ctx {
   BatchedState* bs = nullptr;
   // adjoint of the plain form (ard_adjoint.cpp), set up on first use
   AdjState* adj = nullptr;
+  // a batched context's plain-form twin, where its adjoint runs (ard_adjoint.cpp)
+  Ctx* twin = nullptr;
   hipGraphExec_t graph_exec[2] = {nullptr, nullptr};
 
   ~Ctx();

@@ -292,7 +292,7 @@ int ard_set_batched_tiling(ard_ctx* ctx, int rows_per_wave, int kz_per_block);
 int ard_set_batched_option(ard_ctx* ctx, int option, int value);
 int ard_batched_option(ard_ctx* ctx, int option, int* value);
 
-/* Discrete adjoint of the plain form (no counterpart in the reference, which
+/* Discrete adjoint of the step (no counterpart in the reference, which
  * has no gradients; it transposes the step of ARD/simulation.cpp:300-317).
  * Runs `nsteps` steps from the zero state with a checkpoint every
  * checkpoint_stride steps (<= 0: ceil(sqrt(nsteps))), then the reverse sweep,
@@ -302,9 +302,13 @@ int ard_batched_option(ard_ctx* ctx, int option, int* value);
  *                   source in registration order (zero rows for sources outside
  *                   every DCT partition); may be NULL
  *   dl_dscale[6]    dL/d(damping scale of each PML face, ard_set_pml_scale); may be NULL
- * Plain form only (ARD_ERR_UNSUPPORTED otherwise). The context is left in its
- * forward state after nsteps (time step nsteps, the traces of that run).
- * Allocates its checkpoints for the call. */
+ * A plain-form context is left in its forward state after nsteps (time step
+ * nsteps, the traces of that run). A batched context runs the adjoint on a
+ * plain-form twin built on its first call from the context's topology,
+ * parameters and PML damping scales (the same discrete step; gradients agree
+ * with its own forward run to FP32 rounding); its own state is not touched.
+ * Forces set with ard_set_force are not part of the run (it starts from the
+ * zero state). Allocates its checkpoints for the call. */
 int ard_adjoint(ard_ctx* ctx, int nsteps, const float* dl_drec, float* dl_dwave, float* dl_dscale,
                 int checkpoint_stride);
 

@@ -120,17 +120,65 @@ def test_checkpoint_stride_does_not_change_gradients():
     sim.close()
 
 
-def test_torch_autograd_wrapper():
+@pytest.mark.parametrize("form", [A.FORM_PLAIN, A.FORM_BATCHED])
+def test_torch_autograd_wrapper(form):
     steps = 200
     sc = quarter_room(steps)
     w = torch.tensor(sc.sources[0].wave[None, :].astype(np.float64), requires_grad=True)
     th = torch.ones(6, dtype=torch.float64, requires_grad=True)
-    tr = ard_autograd.recorder_traces(sc, w, th, steps)
+    tr = ard_autograd.recorder_traces(sc, w, th, steps, form=form)
     assert tr.shape == (steps, len(sc.recorders))
     (tr.double() ** 2).sum().backward()
     sim, t2 = traces(sc, sc.sources[0].wave, np.ones(6), steps)
     dw, ds = sim.adjoint(2 * t2)
     sim.close()
-    # equal up to the order of atomic additions in the reverse sweep
+    # equal up to the order of atomic additions in the reverse sweep (and,
+    # for the batched forward, the FP32 rounding of its traces in the loss)
     assert np.linalg.norm(w.grad.numpy() - dw) <= 1e-5 * np.linalg.norm(dw)
     assert np.linalg.norm(th.grad.numpy() - ds) <= 1e-5 * np.linalg.norm(ds)
+
+
+def batched_traces(sc, wave, theta, steps):
+    s2 = S.Scene(**{**sc.__dict__})
+    s2.sources = [S.Source(sc.sources[0].x, sc.sources[0].y, sc.sources[0].z, wave=np.asarray(wave, np.float32))]
+    sim = A.Simulation(s2, form=A.FORM_BATCHED, pml_scale=np.asarray(theta, np.float32))
+    sim.update(steps)
+    return sim, sim.recorders(0, steps).astype(np.float64)
+
+
+def test_batched_context_adjoint():
+    """A batched context's adjoint (run on its plain-form twin): the plain
+    context's gradients for the same loss, the batched context's own state
+    untouched, and the damping-scale gradients against central finite
+    differences of the batched form's own forward loss."""
+    steps = 400
+    sc = quarter_room(steps)
+    w0 = sc.sources[0].wave.astype(np.float64)
+    s, target = batched_traces(sc, w0 * 1.05, np.full(6, 1 / 1.2), steps)
+    s.close()
+    simb, trb = batched_traces(sc, w0, np.ones(6), steps)
+    g = 2 * (trb - target)
+    dwb, dsb = simb.adjoint(g)
+    assert simb.time_step == steps
+    assert np.array_equal(simb.recorders(0, steps).astype(np.float64), trb)
+    simb.close()
+    simp, trp = traces(sc, w0, np.ones(6), steps)
+    dwp, dsp = simp.adjoint(g)
+    simp.close()
+    print("batched vs plain forward rel-L2 %.2e" % (np.linalg.norm(trb - trp) / np.linalg.norm(trp)))
+    assert np.linalg.norm(dwb - dwp) <= 1e-5 * np.linalg.norm(dwp)
+    assert np.linalg.norm(dsb - dsp) <= 1e-5 * np.linalg.norm(dsp)
+
+    def bloss(th):
+        sim, tr = batched_traces(sc, w0, th, steps)
+        sim.close()
+        return float(np.sum((tr - target) ** 2))
+
+    for f in (0, 3, 5):
+        eps = 1e-2
+        tp, tm = np.ones(6), np.ones(6)
+        tp[f] += eps
+        tm[f] -= eps
+        fd = (bloss(tp) - bloss(tm)) / (2 * eps)
+        print("face %d batched adjoint %.6e fd %.6e" % (f, dsb[f], fd))
+        assert abs(dsb[f] - fd) <= TOL * abs(fd)
