@@ -123,6 +123,7 @@ struct BatchedState {
   size_t fft_lds[4] = {};
   bool has_fft = false;
   PmlDesc* d_pml = nullptr; long long pml_blocks = 0; int npml = 0;
+  int pml_shape = 0;  // tile shape of the tiled PML update (k_pml_tiled), chosen at finalize
   BndDesc* d_bnd = nullptr; long long bnd_total = 0; int nbnd = 0;
   PartAccess* d_parts = nullptr;
   BSourceDesc* d_src = nullptr; int nsrc = 0;
@@ -445,6 +446,22 @@ int build_stage_tiling(Ctx& c) {
   if ((r = upload_vec(c, (void**)&g.d_start, starts.data(), starts.size() * 8))) return r;
   S.modal = g;
   return ARD_OK;
+}
+
+// The grouped PML launch's descriptor table for tile shape `shape` (the
+// blocks of each slab depend on it): each slab's first block and the total.
+int build_pml_table(Ctx& c, int shape) {
+  BatchedState& B = *c.bs;
+  std::vector<PmlDesc> pd = c.h_pml;
+  long long b0 = 0;
+  for (PmlDesc& d : pd) {
+    d.block0 = b0;
+    b0 += pml_slab_blocks(d, c.prm.pml_layers, shape);
+  }
+  B.pml_blocks = b0;
+  B.pml_shape = shape;
+  B.graphs_valid = false;
+  return pd.empty() ? ARD_OK : c.upload(B.d_pml, pd.data(), pd.size() * sizeof(PmlDesc));
 }
 
 // The source samples of the current step (*d_step) into their cells.
@@ -870,17 +887,9 @@ int finalize_batched(Ctx& c) {
   if (B.gemm_variant == 4 && !B.has_fft) B.gemm_variant = 1;
   if ((r = build_stage(c))) return r;
   // PML slabs: one grouped launch
-  {
-    std::vector<PmlDesc> pd = c.h_pml;
-    long long b0 = 0;
-    for (size_t k = 0; k < pd.size(); ++k) {
-      pd[k].block0 = b0;
-      b0 += c.pml_blocks[k];
-    }
-    B.npml = int(pd.size());
-    B.pml_blocks = b0;
-    if (B.npml && (r = upload_vec(c, (void**)&B.d_pml, pd.data(), pd.size() * sizeof(PmlDesc)))) return r;
-  }
+  B.npml = int(c.h_pml.size());
+  if (B.npml && (r = c.dalloc((void**)&B.d_pml, c.h_pml.size() * sizeof(PmlDesc)))) return r;
+  if ((r = build_pml_table(c, 0))) return r;
   // partition access for the interface kernel
   std::vector<PartAccess> pa(np);
   for (int i = 0; i < np; ++i) {
@@ -955,7 +964,8 @@ int enqueue_step(Ctx& c, int par, PhaseTimer* T) {
   {
     hipStream_t sp = fork ? B.side : st;
     phase(ARD_PH_PML, sp, [&] {
-      return launch_pml(B.d_pml, B.npml, B.pml_blocks, par, c.prm.pml_layers, c.prm.h, c.prm.dt, c.prm.c0, sp);
+      return launch_pml(B.d_pml, B.npml, B.pml_blocks, par, c.prm.pml_layers, c.prm.h, c.prm.dt, c.prm.c0, sp,
+                        B.pml_shape);
     });
   }
   if (fork) ok(hipEventRecord(B.ev_join, B.side));
@@ -1101,7 +1111,8 @@ int autotune(Ctx& c) {
       e |= launch_fold(S.fold, S.fold_start, S.nfold, S.fold_total, st);
       if (fork) {
         e |= hipStreamWaitEvent(B.side, B.ev_fork, 0) != hipSuccess;
-        e |= launch_pml(B.d_pml, B.npml, B.pml_blocks, 0, c.prm.pml_layers, c.prm.h, c.prm.dt, c.prm.c0, B.side);
+        e |= launch_pml(B.d_pml, B.npml, B.pml_blocks, 0, c.prm.pml_layers, c.prm.h, c.prm.dt, c.prm.c0, B.side,
+                        B.pml_shape);
         e |= hipEventRecord(B.ev_join, B.side) != hipSuccess;
       }
       e |= launch_face(B, S, 0, v, st);
@@ -1115,6 +1126,17 @@ int autotune(Ctx& c) {
       for (size_t i = 0; i < cand.size(); ++i) gms[i] = std::min(gms[i], time_ms([&] { faces(cand[i]); }, 3));
     tried += int(cand.size());
     B.gemm_variant = cand[size_t(std::min_element(gms.begin(), gms.end()) - gms.begin())];
+    // then the PML tile shapes, timed the same way with the chosen transforms
+    if (B.npml && B.side) {
+      float pms[2] = {1e30f, 1e30f};
+      for (int round = 0; round < 3 && !e; ++round)
+        for (int sh = 0; sh < 2 && !e; ++sh) {
+          e |= build_pml_table(c, sh) != ARD_OK;
+          if (!e) pms[sh] = std::min(pms[sh], time_ms([&] { faces(B.gemm_variant); }, 3));
+        }
+      tried += 2;
+      if (!e) e |= build_pml_table(c, pms[1] < pms[0] ? 1 : 0) != ARD_OK;
+    }
     for (size_t i = 0; i < c.pml.size() && !e; ++i)
       if (c.pml[i].p[0]) e |= c.dzero(c.pml[i].p[0], size_t(c.topo.parts[i].cells()) * 8 * 4) != ARD_OK;
     if (e && !r) r = set_error(ARD_ERR_HIP, "autotuner: transform timing");
@@ -1137,12 +1159,17 @@ int batched_set_option(Ctx& c, int option, int value) {
     B.graphs_valid = false;
     return build_stage_tiling(c);
   }
+  if (option == ARD_OPT_PML_SHAPE) {
+    if (value != 0 && value != 1) return set_error(ARD_ERR_INVALID, "PML tile shape must be 0 or 1");
+    return build_pml_table(c, value);
+  }
   return set_error(ARD_ERR_INVALID, "unknown batched option");
 }
 
 int batched_option(Ctx& c, int option, int* value) {
   if (!c.bs) return set_error(ARD_ERR_STATE, "not a finalized batched context");
   if (option == ARD_OPT_RING) *value = c.bs->rpw == 4 ? c.bs->ring : 0;
+  else if (option == ARD_OPT_PML_SHAPE) *value = c.bs->pml_shape;
   else return set_error(ARD_ERR_INVALID, "unknown batched option");
   return ARD_OK;
 }

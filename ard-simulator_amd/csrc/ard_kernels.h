@@ -27,13 +27,15 @@ int launch_gemm_bf16_fold(const GemmDesc* d_descs, const int2* d_tiles, int tota
 // block tiles of 32 pair rows x 128 columns (fold_tile_table with tile_m 32).
 int launch_fold_direct(const GemmDesc* d_descs, const int2* d_tiles, int total_tiles, bool b_k_contig, int fold,
                        hipStream_t s);
-// Blocks of one slab's update for PML thickness L (pml_layers); sets d.nb.
-// Slabs L cells thick along their normal, for L from 3 to 8, take the
-// LDS-tiled update; others (thicker layers, or slabs of another thickness
-// added through ard_add_pml_partition) one thread per cell, in the same launch.
-long long pml_slab_blocks(PmlDesc& d, int L);
+// Blocks of one slab's update for PML thickness L (pml_layers) and tile
+// shape (0 or 1, see k_pml_tiled); sets d.nb. Slabs L cells thick along their
+// normal, for L from 3 to 8, take the LDS-tiled update; others (thicker
+// layers, or slabs of another thickness added through ard_add_pml_partition)
+// one thread per cell, in the same launch. A launch takes the shape its
+// descriptors' blocks were counted for.
+long long pml_slab_blocks(PmlDesc& d, int L, int shape = 0);
 int launch_pml(const PmlDesc* d_descs, int ndesc, long long total_blocks, int par, int L, float dh, float dt,
-               float c0, hipStream_t s);
+               float c0, hipStream_t s, int shape = 0);
 int launch_interface(const BndDesc* d_descs, int ndesc, const PartAccess* d_parts, int par, long long total,
                      hipStream_t s);
 int launch_sources(const SourceDesc* d_src, int n, const int* d_step, float courant, hipStream_t s);

@@ -956,19 +956,29 @@ __global__ __launch_bounds__(256) void k_pml(const PmlDesc* __restrict__ descs, 
 // ---------------------------------------------------------------------------
 // LDS-tiled PML slab update, same arithmetic as k_pml. A block of 256 threads
 // owns a tile that spans the slab's whole thickness L along its normal axis
-// and 32 x 8 (normal z or y) or 16 x 16 (normal x) cells across it, 256 L
-// cells, L per thread in memory order. p is staged with a 3-cell halo across
+// and 256 cells across it, 256 L cells, L per thread in memory order. Tile
+// shape SH across the normal (first x second of the other axes in x, y, z
+// order): SH 0 32 x 8 (normal z or y) or 16 x 16 (normal x); SH 1 the other
+// way round (the autotuner's candidates: less halo against longer rows). p is staged with a 3-cell halo across
 // the normal, each phi_a with a 2-cell halo along a when a lies across the
 // normal, all from buffer loads that read 0 outside the slab; along the
 // normal the tile covers the slab, so neighbours there outside [0, L) are the
 // zero sentinel and are skipped. Each p and phi value is loaded from global
 // memory about twice per tile instead of 21 and 5 times per cell.
 // ---------------------------------------------------------------------------
-template <int L, int AX>
+template <int SH, int AX>
+struct PmlShape {  // tile extents across the normal: first and second other axis
+  static constexpr bool square = (SH == 0) == (AX == 0);
+  static constexpr int T0 = square ? 16 : 32, T1 = square ? 16 : 8;
+};
+
+template <int L, int AX, int SH>
 struct PmlTile {
-  static constexpr int TX = AX == 0 ? L : 32;
-  static constexpr int TY = AX == 1 ? L : (AX == 0 ? 16 : 8);
-  static constexpr int TZ = AX == 2 ? L : (AX == 0 ? 16 : 8);
+  using S = PmlShape<SH, AX>;
+  static constexpr int TX = AX == 0 ? L : S::T0;
+  static constexpr int TY = AX == 1 ? L : (AX == 0 ? S::T0 : S::T1);
+  static constexpr int TZ = AX == 2 ? L : S::T1;
+  static_assert(S::T0 * S::T1 == 256, "one cell per thread in each normal layer");
   // p halo per axis (none along the normal) and its staged box
   static constexpr int HX = AX == 0 ? 0 : 3, HY = AX == 1 ? 0 : 3, HZ = AX == 2 ? 0 : 3;
   static constexpr int PX = TX + 2 * HX, PY = TY + 2 * HY, PZ = TZ + 2 * HZ;
@@ -979,9 +989,9 @@ struct PmlTile {
   static constexpr int total = NP + N0 + N1 + N2;
 };
 
-template <int L>
+template <int L, int SH>
 constexpr int pml_tile_lds() {
-  constexpr int a = PmlTile<L, 0>::total, b = PmlTile<L, 1>::total, c = PmlTile<L, 2>::total;
+  constexpr int a = PmlTile<L, 0, SH>::total, b = PmlTile<L, 1, SH>::total, c = PmlTile<L, 2, SH>::total;
   return a > b ? (a > c ? a : c) : (b > c ? b : c);
 }
 
@@ -1029,10 +1039,10 @@ __device__ __forceinline__ float pml_normal(const float* __restrict__ s, int c, 
   return unsigned(ln + d) < unsigned(L) ? v : 0.f;
 }
 
-template <int L, int AX>
+template <int L, int AX, int SH>
 __device__ __forceinline__ void pml_tile(const PmlDesc& P, int t, float* __restrict__ sm, int par, float r180,
                                          float r12, float dt, float rdt, float c0) {
-  using T = PmlTile<L, AX>;
+  using T = PmlTile<L, AX, SH>;
   using BP = PmlBox<T::PX, T::PY, T::PZ>;
   using B0 = PmlBox<T::TX + 2 * T::GX, T::TY, T::TZ>;
   using B1 = PmlBox<T::TX, T::TY + 2 * T::GY, T::TZ>;
@@ -1058,7 +1068,7 @@ __device__ __forceinline__ void pml_tile(const PmlDesc& P, int t, float* __restr
       const int e = int(threadIdx.x) + 256 * r;
       lx[r] = e % T::TX;
       ly[r] = (e / T::TX) % T::TY;
-      lz[r] = AX == 2 ? r : e / (T::TX * T::TY);  // a z-normal tile is 32 x 8 = 256 cells per plane
+      lz[r] = AX == 2 ? r : e / (T::TX * T::TY);  // a z-normal tile has 256 cells per plane
       const int i = x0 + lx[r], j = y0 + ly[r], k = z0 + lz[r];
       coff[r] = ((i < W) & (j < H) & (k < D)) ? unsigned((k * H + j) * W + i) * 4u : kOut;
       po_r[r] = bload(rpn, coff[r], 0u);
@@ -1152,10 +1162,10 @@ __device__ __forceinline__ void pml_tile(const PmlDesc& P, int t, float* __restr
   }
 }
 
-template <int L>
+template <int L, int SH>
 __global__ __launch_bounds__(256) void k_pml_tiled(const PmlDesc* __restrict__ descs, int ndesc, int par,
                                                    float r180, float r12, float dt, float rdt, float c0) {
-  __shared__ float sm[pml_tile_lds<L>()];
+  __shared__ float sm[pml_tile_lds<L, SH>()];
   const long long b = xcd_block(int(blockIdx.x), int(gridDim.x));
   int lo = 0, hi = ndesc - 1;
   while (lo < hi) {
@@ -1166,12 +1176,12 @@ __global__ __launch_bounds__(256) void k_pml_tiled(const PmlDesc* __restrict__ d
   const PmlDesc& P = descs[lo];
   const int t = int(b - P.block0);
   if (P.nb == 0) pml_cells(P, t, par, r180, r12, dt, rdt, c0);  // a slab not L cells thick
-  else if (P.axis == 0) pml_tile<L, 0>(P, t, sm, par, r180, r12, dt, rdt, c0);
-  else if (P.axis == 1) pml_tile<L, 1>(P, t, sm, par, r180, r12, dt, rdt, c0);
-  else pml_tile<L, 2>(P, t, sm, par, r180, r12, dt, rdt, c0);
+  else if (P.axis == 0) pml_tile<L, 0, SH>(P, t, sm, par, r180, r12, dt, rdt, c0);
+  else if (P.axis == 1) pml_tile<L, 1, SH>(P, t, sm, par, r180, r12, dt, rdt, c0);
+  else pml_tile<L, 2, SH>(P, t, sm, par, r180, r12, dt, rdt, c0);
 }
 
-long long pml_slab_blocks(PmlDesc& d, int L) {
+long long pml_slab_blocks(PmlDesc& d, int L, int shape) {
   const long long n = (long long)d.W * d.H * d.D;
   const int ext[3] = {d.W, d.H, d.D};
   if (L < 3 || L > 8 || ext[d.axis] != L) {  // one thread per cell
@@ -1180,25 +1190,33 @@ long long pml_slab_blocks(PmlDesc& d, int L) {
   }
   // tile extents across the normal: first and second axis other than it
   const int a0 = d.axis == 0 ? 1 : 0, a1 = d.axis == 2 ? 1 : 2;
-  const int t0 = d.axis == 0 ? 16 : 32, t1 = d.axis == 0 ? 16 : 8;
+  const bool square = (shape == 0) == (d.axis == 0);  // PmlShape
+  const int t0 = square ? 16 : 32, t1 = square ? 16 : 8;
   d.nb = (ext[a0] + t0 - 1) / t0;
   return (long long)d.nb * ((ext[a1] + t1 - 1) / t1);
 }
 
 int launch_pml(const PmlDesc* d_descs, int ndesc, long long total_blocks, int par, int L, float dh, float dt,
-               float c0, hipStream_t s) {
+               float c0, hipStream_t s, int shape) {
   if (total_blocks <= 0) return 0;
   const float r180 = 1.f / (180.f * dh * dh), r12 = 1.f / (12.f * dh), rdt = 1.f / dt;
   const dim3 grid(static_cast<unsigned>(total_blocks)), block(256);
+#define ARD_PML_CASE(LL)                                                                                             \
+  case LL:                                                                                                           \
+    if (shape == 1) hipLaunchKernelGGL((k_pml_tiled<LL, 1>), grid, block, 0, s, d_descs, ndesc, par, r180, r12, dt, \
+                                       rdt, c0);                                                                     \
+    else hipLaunchKernelGGL((k_pml_tiled<LL, 0>), grid, block, 0, s, d_descs, ndesc, par, r180, r12, dt, rdt, c0);   \
+    break;
   switch (L) {
-    case 3: hipLaunchKernelGGL(k_pml_tiled<3>, grid, block, 0, s, d_descs, ndesc, par, r180, r12, dt, rdt, c0); break;
-    case 4: hipLaunchKernelGGL(k_pml_tiled<4>, grid, block, 0, s, d_descs, ndesc, par, r180, r12, dt, rdt, c0); break;
-    case 5: hipLaunchKernelGGL(k_pml_tiled<5>, grid, block, 0, s, d_descs, ndesc, par, r180, r12, dt, rdt, c0); break;
-    case 6: hipLaunchKernelGGL(k_pml_tiled<6>, grid, block, 0, s, d_descs, ndesc, par, r180, r12, dt, rdt, c0); break;
-    case 7: hipLaunchKernelGGL(k_pml_tiled<7>, grid, block, 0, s, d_descs, ndesc, par, r180, r12, dt, rdt, c0); break;
-    case 8: hipLaunchKernelGGL(k_pml_tiled<8>, grid, block, 0, s, d_descs, ndesc, par, r180, r12, dt, rdt, c0); break;
+    ARD_PML_CASE(3)
+    ARD_PML_CASE(4)
+    ARD_PML_CASE(5)
+    ARD_PML_CASE(6)
+    ARD_PML_CASE(7)
+    ARD_PML_CASE(8)
     default: hipLaunchKernelGGL(k_pml, grid, block, 0, s, d_descs, ndesc, par, r180, r12, dt, rdt, c0); break;
   }
+#undef ARD_PML_CASE
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 

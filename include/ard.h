@@ -284,11 +284,16 @@ int ard_set_batched_tiling(ard_ctx* ctx, int rows_per_wave, int kz_per_block);
  *                     direct-to-LDS loads, 0 (default) keeps one plane in
  *                     registers. With 4 rows per wave only (other tilings
  *                     use 0).
+ *   ARD_OPT_PML_SHAPE 0 or 1: tile shape of the tiled PML update across a
+ *                     slab's normal, 32 x 8 cells for y- and z-normal slabs
+ *                     and 16 x 16 for x-normal ones (0, default), or the
+ *                     other way round (1).
  * ARD_ERR_STATE for a context that is not a finalized batched one,
  * ARD_ERR_INVALID for other options or values. No counterpart in the
  * reference: they schedule the step of ARD/simulation.cpp:300-317 on one
  * device. */
 #define ARD_OPT_RING 2
+#define ARD_OPT_PML_SHAPE 3
 int ard_set_batched_option(ard_ctx* ctx, int option, int value);
 int ard_batched_option(ard_ctx* ctx, int option, int* value);
 

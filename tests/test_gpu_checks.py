@@ -51,7 +51,8 @@ def test_batched_schedule_options_bit_identical():
     sc = S.scene3()
     steps = 300
     traces = {}
-    for name, opts in (("default", {}), ("ring2", {A.OPT_RING: 2})):
+    for name, opts in (("default", {}), ("ring2", {A.OPT_RING: 2}), ("pml_shape1", {A.OPT_PML_SHAPE: 1}),
+                       ("both", {A.OPT_RING: 2, A.OPT_PML_SHAPE: 1})):
         print("case options", name, flush=True)
         sim = A.Simulation(sc, form=A.FORM_BATCHED)
         sim.set_batched_tiling(4, 48)
@@ -65,10 +66,33 @@ def test_batched_schedule_options_bit_identical():
         assert np.array_equal(tr, traces["default"]), name
 
 
+@pytest.mark.parametrize("layers", [3, 5, 8])
+def test_pml_tile_shapes_bit_identical_fields(layers):
+    """The tiled PML update in both tile shapes: the same arithmetic per cell,
+    so every partition's field (PML slabs included) is bit-identical."""
+    sc = S.two_rooms()
+    sc.boxes = [(0, 0, 0, 16, 12, 16), (16, 0, 0, 12, 20, 16), (0, 12, 0, 16, 8, 16)]
+    sc.sources = [S.Source(5, 1, 4), S.Source(21, 10, 8)]
+    sc.recorders = [(22, 10, 8)]
+    sc.recorder_mode = S.REC_LOCAL
+    sc.pml_layers = layers
+    fields = {}
+    for sh in (0, 1):
+        sim = A.Simulation(sc, form=A.FORM_BATCHED)
+        sim.set_batched_option(A.OPT_PML_SHAPE, sh)
+        sim.update(40)
+        fields[sh] = [sim.pressure_field(i) for i in range(len(sim.partitions()))]
+        sim.close()
+    for a, b in zip(fields[0], fields[1]):
+        assert np.array_equal(a, b)
+
+
 def test_batched_option_values_checked():
     sim = A.Simulation(S.two_rooms(), form=A.FORM_BATCHED)
     with pytest.raises(A.ArdError):
         sim.set_batched_option(A.OPT_RING, 3)
+    with pytest.raises(A.ArdError):
+        sim.set_batched_option(A.OPT_PML_SHAPE, 2)
     with pytest.raises(A.ArdError):
         sim.set_batched_option(99, 1)
     sim.close()
