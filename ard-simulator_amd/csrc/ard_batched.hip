@@ -1095,9 +1095,7 @@ int autotune(Ctx& c) {
   // as the step graph runs them: the fold, then the forward passes with the
   // PML slabs beside them on the second stream, then the inverse passes (the
   // kernels share the CUs, and one that issues more memory instructions
-  // loses more to the PML than it gains alone). The PML launches write its
-  // next-step buffers from the zero state and the step-0 source samples, so
-  // its pressure and phi buffers are cleared again afterwards.
+  // loses more to the PML than it gains alone).
   if (B.gemm_variant != 2) {
     std::vector<int> cand;
     for (int v : {0, 1, 3, 4, 5})
@@ -1137,9 +1135,23 @@ int autotune(Ctx& c) {
       tried += 2;
       if (!e) e |= build_pml_table(c, pms[1] < pms[0] ? 1 : 0) != ARD_OK;
     }
-    for (size_t i = 0; i < c.pml.size() && !e; ++i)
-      if (c.pml[i].p[0]) e |= c.dzero(c.pml[i].p[0], size_t(c.topo.parts[i].cells()) * 8 * 4) != ARD_OK;
     if (e && !r) r = set_error(ARD_ERR_HIP, "autotuner: transform timing");
+  }
+  // The timed launches ran on the step-0 state: zero fields plus the step-0
+  // source samples. The modal pass wrote M+ = -.999 M- + g F over the
+  // previous modes M[1] (a residue of the interior point sources' samples,
+  // since the -.999 keeps pairs of launches from cancelling), and the PML
+  // update its next pressure and phi; both go back to zero (the PML force it
+  // cleared held no samples: sources lie in DCT partitions), as does every
+  // buffer the step writes before reading.
+  if (!r) {
+    for (int i : B.dct_ids) {
+      const BPart& P = B.parts[size_t(i)];
+      const PartGeom& p = c.topo.parts[size_t(i)];
+      if ((r = c.dzero(P.M[1], size_t(p.d) * p.h * P.wp * 4))) break;
+    }
+    for (size_t i = 0; i < c.pml.size() && !r; ++i)
+      if (c.pml[i].p[0]) r = c.dzero(c.pml[i].p[0], size_t(c.topo.parts[i].cells()) * 8 * 4);
   }
   if (!r) r = apply_tiling(c, best_rpw, best_kzc);
   B.graphs_valid = false;

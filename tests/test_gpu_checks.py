@@ -87,6 +87,42 @@ def test_pml_tile_shapes_bit_identical_fields(layers):
         assert np.array_equal(a, b)
 
 
+def test_autotuned_context_starts_from_the_zero_state():
+    """The autotuner times its candidates on the step-0 state; afterwards the
+    context holds that state again: an autotuned context and one pinned to
+    the same choices give bit-identical traces, also with an interior
+    waveform source whose step-0 sample is large (the modal pass's timed
+    launches leave a residue of it in the previous modes) and PML slabs (the
+    timed PML launches write their next buffers)."""
+    sc = S.two_rooms()
+    sc.boxes = [(0, 0, 0, 24, 20, 24), (24, 0, 0, 16, 28, 24), (0, 20, 0, 24, 8, 24)]
+    wave = S.band_limited_noise(300, seed=4)
+    wave[0] = 5.0 * np.abs(wave).max()
+    sc.sources = [S.Source(10, 9, 11, wave=wave), S.Source(30, 12, 9)]
+    sc.recorders = [(12, 10, 12), (30, 14, 10)]
+    sc.recorder_mode = S.REC_LOCAL
+    n = 120
+    tuned = A.Simulation(sc, form=A.FORM_BATCHED, autotune=True)
+    rpw, kzc, variant, _ = tuned.batched_config()
+    ring, shape = tuned.batched_option(A.OPT_RING), tuned.batched_option(A.OPT_PML_SHAPE)
+    tuned.update(n)
+    got = tuned.recorders()
+    tuned.close()
+    A.set_transform_variant(variant)
+    try:
+        pinned = A.Simulation(sc, form=A.FORM_BATCHED, autotune=False)
+    finally:
+        A.set_transform_variant(1)
+    pinned.set_batched_tiling(rpw, kzc)
+    pinned.set_batched_option(A.OPT_RING, ring)
+    pinned.set_batched_option(A.OPT_PML_SHAPE, shape)
+    pinned.update(n)
+    ref = pinned.recorders()
+    pinned.close()
+    print("autotuned", (rpw, kzc, variant, ring, shape), "max |diff|", np.abs(got - ref).max())
+    assert np.abs(ref).max() > 0 and np.array_equal(got, ref)
+
+
 def test_batched_option_values_checked():
     sim = A.Simulation(S.two_rooms(), form=A.FORM_BATCHED)
     with pytest.raises(A.ArdError):
