@@ -666,6 +666,33 @@ int build_masks(Ctx& c, std::vector<BndDesc>& descs, const std::vector<long long
 
 }  // namespace
 
+// The even/odd folded twin of a dense per-axis pass of length L = g.M (the
+// descriptor k_gemm_mfma_fold takes): the folded tables of L, K halved,
+// padded extents, rows as the batched form's face passes lay them out.
+int fold_twin(Ctx& c, GemmDesc& g, bool inverse) {
+  const int L = g.M;
+  const float *ff, *fi;
+  const int r = fold_tables(c, L, &ff, &fi);
+  if (r) return r;
+  g.L = L;
+  g.K = (L + 1) / 2;
+  g.Kp = fold_kp(L);
+  g.Ap = inverse ? fold_rows_inv(L) : fold_rows_fwd(L);
+  if (!inverse) {
+    g.A = ff;
+    g.Mh = fold_even_rows(L);
+    g.M = g.Mh + L / 2;
+  } else {
+    g.A = fi;
+    g.M = int(64 * ceil_div((L + 1) / 2, 32));
+  }
+  g.tiles_n = int(ceil_div(g.N, 64));
+  return ARD_OK;
+}
+
+// The tile table of a launch of folded passes (64-row tiles).
+int fold_pass_tiles(Ctx& c, const std::vector<GemmDesc>& v, int2** d) { return fold_tile_table(c, v, d, 64); }
+
 
 // Sets the modal-pass tiling of every DCT partition (rows per wave, kz planes
 // per block) and rebuilds the descriptors that depend on it.
@@ -1387,8 +1414,12 @@ int fold_selftest(int device, int inverse, int b_layout, int L, int N, const flo
   int2* dt;
   if ((r = upload_vec(c, (void**)&dd, v.data(), sizeof(GemmDesc))) || (r = fold_tile_table(c, v, &dt, tile_m))) return r;
   const int tiles = int(ceil_div(g.M, tile_m)) * g.tiles_n;
-  auto launch = kind == 2 ? launch_fold_direct : kind == 1 ? launch_gemm_bf16_fold : launch_gemm_fold;
-  if (launch(dd, dt, tiles, b_layout == 1, inverse ? 2 : 1, c.stream))
+  const bool kc = b_layout == 1;
+  const int fold = inverse ? 2 : 1;
+  const int lr = kind == 2   ? launch_fold_direct(dd, dt, tiles, kc, fold, c.stream)
+                 : kind == 1 ? launch_gemm_bf16_fold(dd, dt, tiles, kc, fold, c.stream)
+                             : launch_gemm_fold(dd, dt, tiles, kc, fold, c.stream);
+  if (lr)
     return set_error(ARD_ERR_HIP, "folded transform launch");
   if ((e = hipMemcpyAsync(Y, dY, n * 4, hipMemcpyDeviceToHost, c.stream)) != hipSuccess ||
       (e = hipStreamSynchronize(c.stream)) != hipSuccess)

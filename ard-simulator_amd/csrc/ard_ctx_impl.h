@@ -31,6 +31,7 @@ struct DctPlain {
   float* M[2] = {nullptr, nullptr};
   float* a = nullptr;
   float* g = nullptr;
+  float* t[2] = {nullptr, nullptr};  // pass intermediates of its own (folded passes, all partitions in one launch)
   int gemm0 = 0;  // first of its 12 GEMM problems (2 parities x 6 passes)
   int tiles[6] = {0, 0, 0, 0, 0, 0};
   int fft_tiles[6] = {0, 0, 0, 0, 0, 0};  // the same passes as FFT-style transforms (variant 4)
@@ -92,8 +93,22 @@ struct Ctx {
   float* tmp[2] = {nullptr, nullptr};
   std::vector<GemmDesc> h_gemm;
   GemmDesc* d_gemm = nullptr;
-  int plain_variant = 1;       // transform kernel of the plain form, fixed at finalize (1, 0, 2 or 4)
+  int plain_variant = 1;       // transform kernel of the plain form, fixed at finalize (1, 0, 2, 3 or 4)
   GemmDesc* d_fft = nullptr;   // FFT twins of h_gemm (variant 4)
+  // variant 3: pass j of every partition in one launch (even/odd folded),
+  // per step parity: descriptors, tile table, blocks
+  struct PassGroup {
+    GemmDesc* d = nullptr;
+    int2* tiles = nullptr;
+    int blocks = 0;
+  };
+  PassGroup pass_group[2][6];
+  // every PML slab in one launch, every boundary in one launch (last-writer
+  // masks keep the boundary list order's result)
+  PmlDesc* d_pml_grp = nullptr;
+  long long pml_grp_blocks = 0;
+  BndDesc* d_bnd_grp = nullptr;
+  long long bnd_grp_total = 0;
   const float* plain_graph_trace = nullptr;  // trace buffer the plain step graphs were captured with
   size_t fft_lds = 0;          // dynamic LDS of the plain form's FFT passes
   std::vector<PmlDesc> h_pml;      // block0 = 0 each (one launch per slab)
@@ -158,6 +173,10 @@ void batched_state_arrays(Ctx& c, int part, std::vector<std::pair<const float*, 
 // infinity, or -1; synchronous.
 int check_finite(Ctx& c, int* bad_part);
 int fold_selftest(int device, int inverse, int b_layout, int L, int N, const float* X, float* Y, int kind);
+// The even/odd folded twin of a dense per-axis pass (k_gemm_mfma_fold) and
+// the tile table of a launch of such passes (ard_batched.hip).
+int fold_twin(Ctx& c, GemmDesc& g, bool inverse);
+int fold_pass_tiles(Ctx& c, const std::vector<GemmDesc>& v, int2** d);
 // FFT-style transforms (ard_fft.hip): the FFT fields of a dense pass
 // descriptor (its plan cached in the context; ARD_ERR_UNSUPPORTED for a
 // length fft_factor refuses), its tile count, and the entry self-test

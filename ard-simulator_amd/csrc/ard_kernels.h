@@ -14,8 +14,9 @@ int launch_gemm(const GemmDesc* d_descs, int ndesc, int total_tiles, bool b_k_co
                 hipStream_t s, int variant = -1);  // variant -1: g_gemm_variant
 // Even/odd folded transform on the matrix cores (FP32): fold 1 forward, 2 inverse.
 // d_tiles[b] = (descriptor, tile within it) of block b (fold_tile_table).
+// ep 1: the modal epilogue of GemmDesc (forward, B contiguous along n only).
 int launch_gemm_fold(const GemmDesc* d_descs, const int2* d_tiles, int total_tiles, bool b_k_contig, int fold,
-                     hipStream_t s);
+                     hipStream_t s, int ep = 0);
 // The same folded transforms in BF16 split precision (three BF16 products,
 // FP32 accumulate) on v_mfma_f32_32x32x16_bf16; the descriptors' A / Alo are
 // the folded tables' BF16 high and low parts.

@@ -414,7 +414,7 @@ __global__ __launch_bounds__(256) void k_gemm_bf16x3(const GemmDesc* __restrict_
 // accumulate E, wm = 32 O), so M counts 64 rows per 32 j; E and O meet
 // through LDS and the E waves write x[j], the O waves x[L-1-j].
 // ---------------------------------------------------------------------------
-template <bool B_K_CONTIG, int FOLD>
+template <bool B_K_CONTIG, int FOLD, int EP>
 __global__ __launch_bounds__(256) void k_gemm_mfma_fold(const GemmDesc* __restrict__ descs,
                                                         const int2* __restrict__ tiles) {
   // rows of 66: the staging stores (four k rows per quad of lanes) fall in
@@ -601,7 +601,14 @@ __global__ __launch_bounds__(256) void k_gemm_mfma_fold(const GemmDesc* __restri
     }
     if (FOLD == 1) {
       const int k = i < g.Mh ? 2 * i : 2 * (i - g.Mh) + 1;
-      if (k < L) C[(long long)k * g.cM + cn] = g.alpha * acc[r];
+      if (k < L) {
+        const long long o = (long long)k * g.cM + cn;
+        float v = g.alpha * acc[r];
+        // modal epilogue (the plain form's last forward pass): C holds the
+        // previous modes and receives the next, as in k_gemm_mfma
+        if (EP == 1) v = ARD_CG(g.ep_a)[o] * ARD_CG(g.ep_cur)[o] - g.ep_b * C[o] + ARD_CG(g.ep_g)[o] * v;
+        C[o] = v;
+      }
     } else {
       const float e = bsel == 0 ? acc[r] : oth[r], o = bsel == 0 ? oth[r] : acc[r];
       if (bsel == 0) C[(long long)i * g.cM + cn] = g.alpha * (e + o);
@@ -796,13 +803,16 @@ int launch_gemm_bf16_fold(const GemmDesc* d_descs, const int2* d_tiles, int tota
 }
 
 int launch_gemm_fold(const GemmDesc* d_descs, const int2* d_tiles, int total_tiles, bool b_k_contig, int fold,
-                     hipStream_t s) {
+                     hipStream_t s, int ep) {
   if (total_tiles <= 0) return 0;
   dim3 grid(total_tiles), block(256);
-  if (fold == 1 && b_k_contig) hipLaunchKernelGGL((k_gemm_mfma_fold<true, 1>), grid, block, 0, s, d_descs, d_tiles);
-  else if (fold == 1) hipLaunchKernelGGL((k_gemm_mfma_fold<false, 1>), grid, block, 0, s, d_descs, d_tiles);
-  else if (fold == 2 && b_k_contig) hipLaunchKernelGGL((k_gemm_mfma_fold<true, 2>), grid, block, 0, s, d_descs, d_tiles);
-  else if (fold == 2) hipLaunchKernelGGL((k_gemm_mfma_fold<false, 2>), grid, block, 0, s, d_descs, d_tiles);
+  if (ep == 1 && fold == 1 && !b_k_contig)
+    hipLaunchKernelGGL((k_gemm_mfma_fold<false, 1, 1>), grid, block, 0, s, d_descs, d_tiles);
+  else if (ep != 0) return -1;
+  else if (fold == 1 && b_k_contig) hipLaunchKernelGGL((k_gemm_mfma_fold<true, 1, 0>), grid, block, 0, s, d_descs, d_tiles);
+  else if (fold == 1) hipLaunchKernelGGL((k_gemm_mfma_fold<false, 1, 0>), grid, block, 0, s, d_descs, d_tiles);
+  else if (fold == 2 && b_k_contig) hipLaunchKernelGGL((k_gemm_mfma_fold<true, 2, 0>), grid, block, 0, s, d_descs, d_tiles);
+  else if (fold == 2) hipLaunchKernelGGL((k_gemm_mfma_fold<false, 2, 0>), grid, block, 0, s, d_descs, d_tiles);
   else return -1;
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }

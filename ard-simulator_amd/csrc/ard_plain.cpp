@@ -122,6 +122,17 @@ int setup_common(Ctx& c) {
   if (!c.h_pml.empty()) {
     if ((r = c.dalloc((void**)&c.d_pml, c.h_pml.size() * sizeof(PmlDesc)))) return r;
     if ((r = c.upload(c.d_pml, c.h_pml.data(), c.h_pml.size() * sizeof(PmlDesc)))) return r;
+    // and every slab in one launch (the plain form's step)
+    std::vector<PmlDesc> g = c.h_pml;
+    long long b0 = 0;
+    for (size_t k = 0; k < g.size(); ++k) {
+      g[k].block0 = b0;
+      b0 += c.pml_blocks[k];
+    }
+    c.pml_grp_blocks = b0;
+    if ((r = c.dalloc((void**)&c.d_pml_grp, g.size() * sizeof(PmlDesc))) ||
+        (r = c.upload(c.d_pml_grp, g.data(), g.size() * sizeof(PmlDesc))))
+      return r;
   }
   return ensure_trace(c, 1024);
 }
@@ -134,7 +145,14 @@ int finalize_plain(Ctx& c) {
   size_t maxn = 0;
   for (int i = 0; i < np; ++i)
     if (!c.topo.parts[i].is_pml) maxn = std::max(maxn, size_t(c.topo.parts[i].cells()));
-  if ((r = c.dalloc((void**)&c.tmp[0], maxn * 4)) || (r = c.dalloc((void**)&c.tmp[1], maxn * 4))) return r;
+  // pass intermediates: shared by the partitions, which run one after the
+  // other, or (folded passes, variant 3) each partition's own, since pass j
+  // of every partition is one launch. 16 floats of slack: the folded passes'
+  // 16-byte operand loads may end up to 3 floats past an operand array.
+  const bool grouped = g_gemm_variant == 3 || g_gemm_variant == 5;
+  if (!grouped &&
+      ((r = c.dalloc((void**)&c.tmp[0], (maxn + 16) * 4)) || (r = c.dalloc((void**)&c.tmp[1], (maxn + 16) * 4))))
+    return r;
   c.h_gemm.clear();
   for (int i = 0; i < np; ++i) {
     const PartGeom& p = c.topo.parts[i];
@@ -145,6 +163,14 @@ int finalize_plain(Ctx& c) {
     if ((r = c.dalloc((void**)&blk, n * 6 * 4))) return r;
     if ((r = c.dzero(blk, n * 4 * 4))) return r;
     D.F = blk; D.p = blk + n; D.M[0] = blk + 2 * n; D.M[1] = blk + 3 * n; D.a = blk + 4 * n; D.g = blk + 5 * n;
+    D.t[0] = c.tmp[0];
+    D.t[1] = c.tmp[1];
+    if (grouped) {
+      float* tb = nullptr;
+      if ((r = c.dalloc((void**)&tb, (2 * n + 32) * 4))) return r;
+      D.t[0] = tb;
+      D.t[1] = tb + n + 16;
+    }
     std::vector<float> a, g;
     modal_tables(p.w, p.h, p.d, c.prm.h, c.prm.dt, c.prm.c0, a, g);
     if ((r = c.upload(D.a, a.data(), n * 4)) || (r = c.upload(D.g, g.data(), n * 4))) return r;
@@ -186,16 +212,17 @@ int finalize_plain(Ctx& c) {
         g.alpha = 1.f;
         return g;
       };
-      GemmDesc f0 = xpass(fw, D.F, c.tmp[0]);
-      GemmDesc f1 = ypass(fh, c.tmp[0], c.tmp[1]);
-      GemmDesc f2 = zpass(fd, c.tmp[1], nxt);  // modal update into the previous-modes buffer
+      GemmDesc f0 = xpass(fw, D.F, D.t[0]);
+      GemmDesc f1 = ypass(fh, D.t[0], D.t[1]);
+      GemmDesc f2 = zpass(fd, D.t[1], nxt);  // modal update into the previous-modes buffer
       f2.ep_cur = cur; f2.ep_a = D.a; f2.ep_g = D.g; f2.ep_b = 0.999f;
-      GemmDesc i0 = xpass(iw, nxt, c.tmp[0]);
-      GemmDesc i1 = ypass(ih, c.tmp[0], c.tmp[1]);
-      GemmDesc i2 = zpass(id, c.tmp[1], D.p);
+      GemmDesc i0 = xpass(iw, nxt, D.t[0]);
+      GemmDesc i1 = ypass(ih, D.t[0], D.t[1]);
+      GemmDesc i2 = zpass(id, D.t[1], D.p);
       for (GemmDesc* g : {&f0, &f1, &f2, &i0, &i1, &i2}) {
         g->tiles_n = int(ceil_div(g->N, 64));
         g->tile0 = 0;
+        g->b_elems = unsigned(n);  // every operand array holds the partition's n cells
         c.h_gemm.push_back(*g);
       }
       if (par == 0) {
@@ -208,9 +235,27 @@ int finalize_plain(Ctx& c) {
   if ((r = c.upload(c.d_gemm, c.h_gemm.data(), c.h_gemm.size() * sizeof(GemmDesc)))) return r;
   // transform kernel of the passes, fixed here: the FFT-style passes
   // (variant 4) when every axis length factors into radices up to 13, else
-  // the dense passes (variants 3 and 5, the folded face passes of the batched
-  // form, mean matrix cores here)
-  c.plain_variant = g_gemm_variant == 3 || g_gemm_variant == 5 ? 1 : g_gemm_variant;
+  // the dense passes; variants 3 and 5 (the batched form's folded face
+  // passes) mean the even/odd folded matrix-core passes (k_gemm_mfma_fold,
+  // the modal update fused into the last forward pass as in variant 1)
+  c.plain_variant = grouped ? 3 : g_gemm_variant;
+  if (c.plain_variant == 3 && !c.h_gemm.empty()) {
+    for (int par = 0; par < 2; ++par)
+      for (int j = 0; j < 6; ++j) {
+        std::vector<GemmDesc> f;
+        for (const DctPlain& D : c.dct)
+          if (D.F) f.push_back(c.h_gemm[size_t(D.gemm0 + par * 6 + j)]);
+        Ctx::PassGroup& G = c.pass_group[par][j];
+        G.blocks = 0;
+        for (GemmDesc& g : f) {
+          if ((r = fold_twin(c, g, j >= 3))) return r;
+          G.blocks += int(ceil_div(g.M, 64)) * g.tiles_n;
+        }
+        if ((r = fold_pass_tiles(c, f, &G.tiles)) || (r = c.dalloc((void**)&G.d, f.size() * sizeof(GemmDesc))) ||
+            (r = c.upload(G.d, f.data(), f.size() * sizeof(GemmDesc))))
+          return r;
+      }
+  }
   if (c.plain_variant == 4 && !c.h_gemm.empty()) {
     std::vector<GemmDesc> f = c.h_gemm;
     bool ok = true;
@@ -249,6 +294,25 @@ int finalize_plain(Ctx& c) {
   if (!c.h_bnd.empty()) {
     if ((r = c.dalloc((void**)&c.d_bnd, c.h_bnd.size() * sizeof(BndDesc)))) return r;
     if ((r = c.upload(c.d_bnd, c.h_bnd.data(), c.h_bnd.size() * sizeof(BndDesc)))) return r;
+    // every boundary in one launch: the elements of each start on a block,
+    // and each force cell is written by its last writer in list order only
+    std::vector<BndDesc> g = c.h_bnd;
+    std::vector<uint8_t> mask;
+    std::vector<long long> mstart;
+    final_writer_masks(c.topo, g, c.bnd_elems, mask, mstart);
+    uint8_t* dm = nullptr;
+    if ((r = c.dalloc((void**)&dm, std::max<size_t>(mask.size(), 1))) || (r = c.upload(dm, mask.data(), mask.size())))
+      return r;
+    long long e0 = 0;
+    for (size_t b = 0; b < g.size(); ++b) {
+      g[b].mask = dm + mstart[b];
+      g[b].elem0 = e0;
+      e0 += ceil_div(c.bnd_elems[b], 256) * 256;
+    }
+    c.bnd_grp_total = e0;
+    if ((r = c.dalloc((void**)&c.d_bnd_grp, g.size() * sizeof(BndDesc))) ||
+        (r = c.upload(c.d_bnd_grp, g.data(), g.size() * sizeof(BndDesc))))
+      return r;
   }
 
   // sources: ARD/partition.cpp:187-197 writes the sample into the owner's force volume
@@ -304,37 +368,55 @@ static int enqueue_plain(Ctx& c, int par, PhaseTimer* T) {
   int ev = begin(ARD_PH_SOURCES);
   if (launch_sources(c.d_src, c.nsrc, c.d_step, c.courant, c.stream)) return set_error(ARD_ERR_HIP, "sources");
   end(ev);
-  for (size_t i = 0; i < c.dct.size(); ++i) {
-    const DctPlain& D = c.dct[i];
-    if (!D.F) continue;
-    const int off = D.gemm0 + par * 6;
-    const bool fft = c.plain_variant == 4;
-    // pass j: x (B and C contiguous along the axis), then y, then z (with
-    // the modal update in the forward z pass)
-    auto pass = [&](int j) {
-      const bool kc = j % 3 == 0;
-      const int ep = j == 2 ? 1 : 0;
-      return fft ? launch_fft(c.d_fft + off + j, 1, D.fft_tiles[j], kc, kc, ep, c.fft_lds, c.stream)
-                 : launch_gemm(c.d_gemm + off + j, 1, D.tiles[j], kc, kc, ep, c.stream, c.plain_variant);
-    };
+  if (c.plain_variant == 3) {
+    // pass j of every partition in one launch (the partitions' passes are
+    // independent; each has its own intermediates)
     int r = 0;
-    ev = begin(ARD_PH_DCT_FWD);
-    for (int j = 0; j < 3; ++j) r |= pass(j);
-    end(ev);
-    ev = begin(ARD_PH_DCT_INV);
-    for (int j = 3; j < 6; ++j) r |= pass(j);
-    end(ev);
+    for (int j = 0; j < 6; ++j) {
+      if (j == 0) ev = begin(ARD_PH_DCT_FWD);
+      if (j == 3) ev = begin(ARD_PH_DCT_INV);
+      const Ctx::PassGroup& G = c.pass_group[par][j];
+      r |= launch_gemm_fold(G.d, G.tiles, G.blocks, j % 3 == 0, j < 3 ? 1 : 2, c.stream, j == 2 ? 1 : 0);
+      if (j == 2 || j == 5) end(ev);
+    }
     if (r) return set_error(ARD_ERR_HIP, "DCT pass launch");
+  } else {
+    for (size_t i = 0; i < c.dct.size(); ++i) {
+      const DctPlain& D = c.dct[i];
+      if (!D.F) continue;
+      const int off = D.gemm0 + par * 6;
+      const bool fft = c.plain_variant == 4;
+      // pass j: x (B and C contiguous along the axis), then y, then z (with
+      // the modal update in the forward z pass)
+      auto pass = [&](int j) {
+        const bool kc = j % 3 == 0;
+        const int ep = j == 2 ? 1 : 0;
+        return fft ? launch_fft(c.d_fft + off + j, 1, D.fft_tiles[j], kc, kc, ep, c.fft_lds, c.stream)
+                   : launch_gemm(c.d_gemm + off + j, 1, D.tiles[j], kc, kc, ep, c.stream, c.plain_variant);
+      };
+      int r = 0;
+      ev = begin(ARD_PH_DCT_FWD);
+      for (int j = 0; j < 3; ++j) r |= pass(j);
+      end(ev);
+      ev = begin(ARD_PH_DCT_INV);
+      for (int j = 3; j < 6; ++j) r |= pass(j);
+      end(ev);
+      if (r) return set_error(ARD_ERR_HIP, "DCT pass launch");
+    }
   }
-  for (size_t i = 0; i < c.h_pml.size(); ++i) {
+  // every PML slab in one launch, then every boundary in one launch (its
+  // last-writer masks give the list order's result: the reference's loop
+  // writes each force cell last from its final writer, ARD/simulation.cpp:313-317)
+  if (!c.h_pml.empty()) {
     ev = begin(ARD_PH_PML);
-    if (launch_pml(c.d_pml + i, 1, c.pml_blocks[i], par, c.prm.pml_layers, c.prm.h, c.prm.dt, c.prm.c0, c.stream))
+    if (launch_pml(c.d_pml_grp, int(c.h_pml.size()), c.pml_grp_blocks, par, c.prm.pml_layers, c.prm.h, c.prm.dt,
+                   c.prm.c0, c.stream))
       return set_error(ARD_ERR_HIP, "PML launch");
     end(ev);
   }
-  for (size_t b = 0; b < c.h_bnd.size(); ++b) {
+  if (!c.h_bnd.empty()) {
     ev = begin(ARD_PH_INTERFACE);
-    if (launch_interface(c.d_bnd + b, 1, c.d_parts, par ^ 1, c.bnd_elems[b], c.stream))
+    if (launch_interface(c.d_bnd_grp, int(c.h_bnd.size()), c.d_parts, par ^ 1, c.bnd_grp_total, c.stream))
       return set_error(ARD_ERR_HIP, "interface launch");
     end(ev);
   }

@@ -55,6 +55,27 @@ def test_scene3_trace_matches_golden(form):
     assert e32 <= TOL and e64 <= TOL
 
 
+@pytest.mark.parametrize("scene", ["two_rooms", "scene3"])
+def test_plain_folded_passes_match_goldens(scene):
+    """The plain form's per-axis passes even/odd folded (transform variant 3:
+    k_gemm_mfma_fold on the dense volumes, the modal update fused into the
+    last forward pass) against the FP32 and FP64 goldens."""
+    sc = getattr(S, scene)()
+    A.set_transform_variant(3)
+    try:
+        sim = A.Simulation(sc, form=A.FORM_PLAIN)
+    finally:
+        A.set_transform_variant(1)
+    assert sim.batched_config()[2] == 3
+    sim.update(sc.steps)
+    tr = sim.recorders()
+    g32 = np.load(os.path.join(GOLD, "%s_fp32.npy" % scene))
+    g64 = np.load(os.path.join(GOLD, "%s_fp64.npy" % scene))
+    e32, e64 = rel_l2(tr, g32), rel_l2(tr, g64)
+    print("%s plain folded rel-L2 vs fp32 %.3e vs fp64 %.3e" % (scene, e32, e64))
+    assert e32 <= TOL and e64 <= TOL
+
+
 @pytest.mark.parametrize("variant", [0, 2, 3])
 def test_scene3_batched_transform_variants(variant):
     """The batched form with each transform kernel against the goldens
@@ -214,8 +235,8 @@ def odd_scene():
     return sc
 
 
-@pytest.mark.parametrize("form,variant", [(A.FORM_PLAIN, 1), (A.FORM_BATCHED, 1), (A.FORM_BATCHED, 3),
-                                          (A.FORM_BATCHED, 0)])
+@pytest.mark.parametrize("form,variant", [(A.FORM_PLAIN, 1), (A.FORM_PLAIN, 3), (A.FORM_BATCHED, 1),
+                                          (A.FORM_BATCHED, 3), (A.FORM_BATCHED, 0)])
 def test_odd_sides_all_fields_match_oracle(form, variant):
     sc = odd_scene()
     print("case odd_scene fields form", form, "variant", variant, flush=True)
