@@ -34,6 +34,8 @@
 //   --info        print every partition and boundary (Partition::Info, Boundary::Info)
 //   --ring N      batched form: the modal pass's LDS ring (ard_set_batched_option
 //                 ARD_OPT_RING, 0 or 2), for runs that skip the autotuner
+//   --pml-shape N batched form: the tiled PML update's tile shape
+//                 (ARD_OPT_PML_SHAPE, 0 or 1), likewise
 #include <hip/hip_runtime_api.h>
 
 #include <sys/stat.h>
@@ -53,7 +55,7 @@ using namespace ard::host;
 int main(int argc, char** argv) {
   bool cells = false, timing = false, output = true;
   long long copy_mib = 0;
-  int check_every = 0, ring = -1;
+  int check_every = 0, ring = -1, pml_shape = -1;
   bool info = false;
   const char* frame = nullptr;
   int look_from = 0;
@@ -80,6 +82,7 @@ int main(int argc, char** argv) {
     else if (!std::strcmp(argv[i], "--check-every") && i + 1 < argc) check_every = std::atoi(argv[++i]);
     else if (!std::strcmp(argv[i], "--info")) info = true;
     else if (!std::strcmp(argv[i], "--ring") && i + 1 < argc) ring = std::atoi(argv[++i]);
+    else if (!std::strcmp(argv[i], "--pml-shape") && i + 1 < argc) pml_shape = std::atoi(argv[++i]);
     else if (argv[i][0] == '-' && argv[i][1] == '-') {
       std::fprintf(stderr, "unknown option %s\n", argv[i]);
       return 2;
@@ -90,7 +93,7 @@ int main(int argc, char** argv) {
   if (pos.size() < 2) {
     std::fprintf(stderr,
                  "usage: %s [--cells] [--rec-local] [--plain] [--autotune] [--transform V] [--timing] [--no-output] [--no-cubes] "
-                 "[--c0 V] [--duration S] [--frame F] [--look-from V] [--copy-mib N] [--check-every N] [--info] [--ring N] scene.txt sources.txt [recorders.txt] [steps] [h] [out_root]\n",
+                 "[--c0 V] [--duration S] [--frame F] [--look-from V] [--copy-mib N] [--check-every N] [--info] [--ring N] [--pml-shape N] scene.txt sources.txt [recorders.txt] [steps] [h] [out_root]\n",
                  argv[0]);
     return 2;
   }
@@ -125,6 +128,8 @@ int main(int argc, char** argv) {
     const int steps = pos.size() > 3 ? std::atoi(pos[3]) : int(Simulation::m_duration / Simulation::m_dt);
     if (timing && ard_set_kernel_timing(sim.context(), 1) != ARD_OK) throw Error(ard_last_error());
     if (ring >= 0 && ard_set_batched_option(sim.context(), ARD_OPT_RING, ring) != ARD_OK) throw Error(ard_last_error());
+    if (pml_shape >= 0 && ard_set_batched_option(sim.context(), ARD_OPT_PML_SHAPE, pml_shape) != ARD_OK)
+      throw Error(ard_last_error());
     auto t1 = std::chrono::steady_clock::now();
     std::cout << "Initialization finished. (" << std::chrono::duration<double>(t1 - t0).count() << " s)" << std::endl;
     sim.Info();

@@ -90,3 +90,23 @@ def test_many_points_under_modal_tilings():
         err = np.linalg.norm(tr - base) / np.linalg.norm(base)
         print("tiling %d/%d rel-L2 vs first %.2e" % (rpw, kzc, err))
         assert err < 1e-5
+
+
+def test_many_points_graph_replays_match_direct_launches():
+    """The extra point passes inside the captured step graphs (one-step and
+    8-step graphs, from even and odd steps) against direct launches:
+    identical recorder samples and fields."""
+    sc = many_points_scene()
+    g = A.Simulation(sc, form=A.FORM_BATCHED)
+    d = A.Simulation(sc, form=A.FORM_BATCHED)
+    d.set_kernel_timing(True)
+    t = 0
+    for chunk, n in enumerate((7, 8, 17, 3, 16, 9)):
+        g.update(n)
+        d.update(n)
+        t += n
+        assert np.array_equal(g.recorders(0, t), d.recorders(0, t)), chunk
+    for i in range(len(g.partitions())):
+        assert np.array_equal(g.pressure_field(i), d.pressure_field(i)), i
+    g.close()
+    d.close()
