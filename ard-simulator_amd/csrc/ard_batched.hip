@@ -863,25 +863,10 @@ int finalize_batched(Ctx& c) {
   for (int k = 0; k < 4; ++k) {
     // the folded twin of each pass: forward passes (k = 0, 1) fold the input,
     // inverse passes (k = 2, 3) produce the output pair
+    // (inverse: 64 tile rows, 32 of A_E and 32 of A_O, per 32 outputs j)
     std::vector<GemmDesc> gf = gm[k];
-    for (GemmDesc& g : gf) {
-      const int L = g.M;
-      const float *ff, *fi;
-      if ((r = fold_tables(c, L, &ff, &fi))) return r;
-      g.L = L;
-      g.K = (L + 1) / 2;
-      g.Kp = fold_kp(L);
-      g.Ap = k < 2 ? fold_rows_fwd(L) : fold_rows_inv(L);
-      if (k < 2) {
-        g.A = ff;
-        g.Mh = fold_even_rows(L);
-        g.M = g.Mh + L / 2;
-      } else {  // 64 tile rows (32 of A_E, 32 of A_O) per 32 outputs j
-        g.A = fi;
-        g.M = int(64 * ceil_div((L + 1) / 2, 32));
-      }
-      g.tiles_n = int(ceil_div(g.N, 64));
-    }
+    for (GemmDesc& g : gf)
+      if ((r = fold_twin(c, g, k >= 2))) return r;
     B.h_gemm_fold[k] = gf;
     // k_fold_direct's twins: M counts pair rows, tiles of 32 pair rows x 128 columns
     std::vector<GemmDesc> gd = gf;
