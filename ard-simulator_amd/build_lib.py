@@ -63,6 +63,15 @@ def build(verbose=False, jobs=None):
     run(["g++", "-O2", "-std=c++17", "-D__HIP_PLATFORM_AMD__", "-I" + os.path.join(ROCM, "include"),
          "-o", os.path.join(HERE, "ard_run"), os.path.join(HERE, "host", "ard_run.cpp"),
          "-L" + HERE, "-lard", "-L" + os.path.join(ROCM, "lib"), "-lamdhip64", "-Wl,-rpath,$ORIGIN"])
+    # measurement probes behind figures in the design notes (probes/*.hip):
+    # matrix-core throughput and the modal pass's streaming floor
+    probes = []
+    for name in ("mfma_peak", "stream_probe"):
+        probes.append([HIPCC, "-x", "hip", os.path.join(HERE, "probes", name + ".hip"), "-o",
+                       os.path.join(HERE, "probes", name)] + COMMON + DEVICE)
+    with concurrent.futures.ThreadPoolExecutor(max_workers=jobs) as ex:
+        for _ in ex.map(run, probes):
+            pass
     if verbose:
         print("built", OUT)
     return OUT
