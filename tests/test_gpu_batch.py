@@ -9,14 +9,25 @@ import ard_sim as A
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("form", [A.FORM_BATCHED, A.FORM_PLAIN])
-def test_batch_rooms_match_rooms_alone(form):
+@pytest.mark.parametrize("form,variant", [(A.FORM_BATCHED, 1), (A.FORM_PLAIN, 1), (A.FORM_PLAIN, 3)])
+def test_batch_rooms_match_rooms_alone(form, variant):
+    """Rooms in one batched or plain context against each room alone (the
+    plain form with its dense passes, and with its folded passes grouped
+    across the partitions of all rooms)."""
     steps = 300
     rooms = S.split_rooms(S.rooms64(n_rooms=4, steps=steps))
-    b = A.Batch(rooms, form=form)
+    A.set_transform_variant(variant)
+    try:
+        b = A.Batch(rooms, form=form)
+    finally:
+        A.set_transform_variant(1)
     b.update(steps)
     for r, sc in enumerate(rooms):
-        sim = A.Simulation(sc, form=form)
+        A.set_transform_variant(variant)
+        try:
+            sim = A.Simulation(sc, form=form)
+        finally:
+            A.set_transform_variant(1)
         sim.update(steps)
         alone = sim.recorders(0, steps)
         sim.close()
