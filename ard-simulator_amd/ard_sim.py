@@ -21,6 +21,7 @@ FORM_PLAIN, FORM_BATCHED = 0, 1
 REC_REFERENCE, REC_LOCAL = 0, 1
 OPT_RING = 2  # ard_set_batched_option
 OPT_PML_SHAPE = 3
+OPT_PML_SPLIT = 4
 
 EXPORTED = [
     "ard_ctx_create", "ard_ctx_destroy", "ard_last_error", "ard_set_pml_scale", "ard_add_dct_partition",
@@ -345,11 +346,11 @@ class Simulation:
         return out.value
 
     def set_batched_option(self, option, value):
-        """Sets a schedule option of a finalized batched context (OPT_RING, OPT_PML_SHAPE)."""
+        """Sets a schedule option of a finalized batched context (OPT_RING, OPT_PML_SHAPE, OPT_PML_SPLIT)."""
         _check(self.L.ard_set_batched_option(self.ctx, int(option), int(value)))
 
     def batched_option(self, option):
-        """A schedule option of a finalized batched context (OPT_RING, OPT_PML_SHAPE)."""
+        """A schedule option of a finalized batched context (OPT_RING, OPT_PML_SHAPE, OPT_PML_SPLIT)."""
         out = ctypes.c_int(0)
         _check(self.L.ard_batched_option(self.ctx, int(option), ctypes.byref(out)))
         return out.value

@@ -9,17 +9,10 @@
 namespace ard {
 
 // sum / difference of the face pair planes (x, n-1-x), x = 0..2
-// Fold and unfold walk their planes in items of `vec` consecutive cells (4 when
+// The unfold walks its planes in items of `vec` consecutive cells (4 when
 // the plane length and every pointer allow 16-byte accesses, else 1); each
 // descriptor's items start on a 256-item boundary of the grid (block-uniform
 // descriptor lookup).
-struct FoldDesc {
-  const float* src;  // [6][plane] raw face slab, plane q = x for q < 3, n-6+q for q >= 3
-  float* dst;        // [6][plane]: q < 3 sum, q >= 3 difference
-  long long plane;
-  int vec;
-};
-
 struct UnfoldDesc {
   const float* part;  // [nchunk][6][plane] partial reductions: q < 3 even modes, q >= 3 odd
   float* dst;         // [6][plane] in face-pair order
@@ -69,7 +62,10 @@ struct ModalDesc {
   const float* Rz;
   const float* yt;  // [H][8] {Ey[ky][0..2], Ry[ky][0..2], u_y, 0}
   const float* zt;  // [D][8] likewise for z
-  const float* Gx;  // [D][H][6] folded face spectra (q < 3 even, q >= 3 odd)
+  // face spectra of the six raw slabs of each face pair (slab q at x = q for
+  // q < 3, at x = W - 6 + q for q >= 3; likewise y, z), folded by the pass:
+  // q and 5 - q summed for even wavenumbers along the normal, subtracted for odd
+  const float* Gx;  // [D][H][6]
   const float* Gy;  // [6][D][W]
   const float* Gz;  // [6][H][W]
   float* Tx;        // [nbx][6][D][H] partial reductions
@@ -112,7 +108,6 @@ struct XrecDesc {
   float* part;  // [blocks][n]
 };
 
-int launch_fold(const FoldDesc* d, const long long* starts, int n, long long total, hipStream_t s);
 int launch_unfold(const UnfoldDesc* d, const long long* starts, int n, long long total, hipStream_t s);
 int launch_bsources(const BSourceDesc* d, int n, const int* d_step, float courant, hipStream_t s);
 // ring: planes of mode rows in flight per wave through LDS (0: one, in registers; 2 or 3 with 4 rows per wave)

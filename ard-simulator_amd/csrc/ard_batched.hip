@@ -2,9 +2,10 @@
 // (ARD/simulation.cpp:300-317) for all partitions in one launch, and one step
 // captured in a hipGraph (one graph per step parity, replayed alternately).
 //
-// Per step: PML update -> fold face slabs -> 2D DCT-II of the face slabs (2
-// grouped GEMM launches) -> modal pass (force spectrum expansion, modal
-// update, reduction onto face planes and recorders) -> unfold -> 2D DCT-III
+// Per step: PML update (beside) 2D DCT-II of the raw force face slabs (2
+// grouped GEMM launches) -> modal pass (face spectra folded by the normal's
+// parity, force spectrum expansion, modal update, reduction onto face planes
+// and recorders) -> unfold -> 2D DCT-III
 // to face-slab pressure (2 grouped GEMM launches) -> interface forcing with
 // last-writer masks -> recorder samples and the next step's source samples.
 #include <hip/hip_runtime.h>
@@ -45,7 +46,7 @@ struct BPart {
   // sizes of the three face-slab families: [6][D][H], [6][D][W], [6][H][W]
   long long plane[3] = {0, 0, 0};
   float* F[3] = {};   // raw face-slab force (written by the interface kernel and sources)
-  float* S[3] = {};   // folded force; after the modal pass, unfolded mode planes
+  float* S[3] = {};   // after the modal pass, the mode planes' face-pair sums
   float* U[3] = {};   // GEMM intermediate
   float* G[3] = {};   // folded force spectra
   float* P[3] = {};   // face-slab pressure
@@ -91,10 +92,9 @@ struct BatchedState {
     int n = 0;
     long long blocks = 0;
   };
-  // Launch tables of the DCT partitions' phases: fold, the four face passes
+  // Launch tables of the DCT partitions' phases: the four face passes
   // in every transform kernel, the modal pass and the unfold.
   struct Stage {
-    FoldDesc* fold = nullptr; long long* fold_start = nullptr; int nfold = 0; long long fold_total = 0;
     UnfoldDesc* unfold = nullptr; long long* unfold_start = nullptr; int nunfold = 0; long long unfold_total = 0;
     GemmDesc* gemm[4] = {};        // FT1, FT2, IT1, IT2 (dense kernels, variants 0 and 1)
     int ngemm[4] = {}, gemm_tiles[4] = {};
@@ -114,16 +114,19 @@ struct BatchedState {
   };
   Stage stage;
   // host copies of the per-partition descriptors in descriptor order (three
-  // per DCT partition for fold, unfold and each face pass; one modal)
+  // per DCT partition for unfold and each face pass; one modal)
   std::vector<int> dct_ids;
-  std::vector<FoldDesc> h_fold;
   std::vector<UnfoldDesc> h_unfold;
   std::vector<GemmDesc> h_gemm[4], h_gemm_fold[4], h_gemm_bf16[4], h_gemm_fft[4], h_gemm_direct[4];
   std::vector<ModalDesc> h_modal;
   size_t fft_lds[4] = {};
   bool has_fft = false;
-  PmlDesc* d_pml = nullptr; long long pml_blocks = 0; int npml = 0;
-  int pml_shape = 0;  // tile shape of the tiled PML update (k_pml_tiled), chosen at finalize
+  // the PML slabs in two launch groups (ARD_OPT_PML_SPLIT): group 0 beside
+  // the forward face transforms, group 1 beside the unfold and the inverse
+  // ones; npml counts both
+  PmlDesc* d_pml[2] = {}; long long pml_blocks[2] = {}; int pml_n[2] = {}; int npml = 0;
+  int pml_shape = 0;   // tile shape of the tiled PML update (k_pml_tiled), chosen at finalize
+  int pml_split = 0;   // percent of the PML's tile blocks in group 0
   BndDesc* d_bnd = nullptr; long long bnd_total = 0; int nbnd = 0;
   PartAccess* d_parts = nullptr;
   BSourceDesc* d_src = nullptr; int nsrc = 0;
@@ -140,9 +143,9 @@ struct BatchedState {
   int rpw = 4, kz_chunk = 48, gemm_variant = 1, tuned = 0;
   int ring = 0;  // modal pass: planes of mode rows in flight per wave through LDS (4 rows per wave)
   // second stream and events: in the step graph the PML slabs run beside the
-  // fold and forward face transforms, which they share no data with
+  // forward face transforms, which they share no data with
   hipStream_t side = nullptr;
-  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_fork2 = nullptr, ev_join = nullptr;
   // kMultiSteps steps captured in one graph, per starting parity: consecutive
   // graph launches leave the GPU idle a few microseconds between them
   static constexpr int kMultiSteps = 8;
@@ -151,6 +154,7 @@ struct BatchedState {
     for (auto& g : graph_multi)
       if (g) hipGraphExecDestroy(g);
     if (ev_fork) hipEventDestroy(ev_fork);
+    if (ev_fork2) hipEventDestroy(ev_fork2);
     if (ev_join) hipEventDestroy(ev_join);
     if (side) hipStreamDestroy(side);
   }
@@ -367,8 +371,8 @@ int upload_gemms(Ctx& c, std::vector<GemmDesc>& v, GemmDesc** d, int* tiles) {
   return upload_vec(c, (void**)d, v.data(), v.size() * sizeof(GemmDesc));
 }
 
-// Uploads the stage's launch tables that depend on the geometry only: fold
-// and the four face passes in every transform kernel.
+// Uploads the stage's launch tables that depend on the geometry only: the
+// four face passes in every transform kernel.
 int build_stage(Ctx& c) {
   BatchedState& B = *c.bs;
   BatchedState::Stage& S = B.stage;
@@ -376,11 +380,6 @@ int build_stage(Ctx& c) {
   if (B.dct_ids.empty()) return ARD_OK;  // nothing to launch
   std::vector<long long> st;
   long long t = 0;
-  for (auto& x : B.h_fold) { st.push_back(t); t += ceil_div(3 * x.plane / x.vec, 256) * 256; }
-  S.nfold = int(B.h_fold.size());
-  S.fold_total = t;
-  if ((r = upload_vec(c, (void**)&S.fold, B.h_fold.data(), B.h_fold.size() * sizeof(FoldDesc)))) return r;
-  if ((r = upload_vec(c, (void**)&S.fold_start, st.data(), st.size() * 8))) return r;
   for (int k = 0; k < 4; ++k) {
     std::vector<GemmDesc> g = B.h_gemm[k];
     S.ngemm[k] = int(g.size());
@@ -450,19 +449,45 @@ int build_stage_tiling(Ctx& c) {
 
 // The grouped PML launch's descriptor table for tile shape `shape` (the
 // blocks of each slab depend on it): each slab's first block and the total.
-int build_pml_table(Ctx& c, int shape) {
+int build_pml_table(Ctx& c, int shape, int split) {
   BatchedState& B = *c.bs;
+  const int L = c.prm.pml_layers;
   std::vector<PmlDesc> pd = c.h_pml;
-  long long b0 = 0;
-  for (PmlDesc& d : pd) {
-    d.block0 = b0;
-    b0 += pml_slab_blocks(d, c.prm.pml_layers, shape);
+  std::vector<long long> nb(pd.size());
+  long long total = 0;
+  for (size_t i = 0; i < pd.size(); ++i) total += nb[i] = pml_slab_blocks(pd[i], L, shape);
+  // group 0 takes the leading slabs until it holds split % of the blocks
+  std::vector<PmlDesc> g[2];
+  long long acc = 0;
+  for (size_t i = 0; i < pd.size(); ++i) {
+    g[acc * 100 < split * total ? 0 : 1].push_back(pd[i]);
+    acc += nb[i];
   }
-  B.pml_blocks = b0;
+  for (int k = 0; k < 2; ++k) {
+    long long b0 = 0;
+    for (PmlDesc& d : g[k]) {
+      d.block0 = b0;
+      b0 += pml_slab_blocks(d, L, shape);
+    }
+    B.pml_blocks[k] = b0;
+    B.pml_n[k] = int(g[k].size());
+    if (!g[k].empty()) {
+      const int r = c.upload(B.d_pml[k], g[k].data(), g[k].size() * sizeof(PmlDesc));
+      if (r) return r;
+    }
+  }
   B.pml_shape = shape;
+  B.pml_split = split;
   B.graphs_valid = false;
-  return pd.empty() ? ARD_OK : c.upload(B.d_pml, pd.data(), pd.size() * sizeof(PmlDesc));
+  return ARD_OK;
 }
+
+namespace {
+int launch_pml_group(Ctx& c, const BatchedState& B, int k, int par, hipStream_t s) {
+  return launch_pml(B.d_pml[k], B.pml_n[k], B.pml_blocks[k], par, c.prm.pml_layers, c.prm.h, c.prm.dt, c.prm.c0, s,
+                    B.pml_shape);
+}
+}  // namespace
 
 // The source samples of the current step (*d_step) into their cells.
 int write_sources(Ctx& c) {
@@ -742,13 +767,13 @@ int finalize_batched(Ctx& c) {
   {
     hipError_t e = hipStreamCreateWithFlags(&B.side, hipStreamNonBlocking);  // default priority
     if (e == hipSuccess) e = hipEventCreateWithFlags(&B.ev_fork, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&B.ev_fork2, hipEventDisableTiming);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&B.ev_join, hipEventDisableTiming);
     if (e != hipSuccess) return hip_fail(e, "side stream");
   }
   const int np = int(c.topo.parts.size());
   B.parts.assign(np, BPart());
   for (size_t i = 0; i < c.topo.recorders.size(); ++i) B.parts[c.topo.recorders[i].part].recs.push_back(int(i));
-  std::vector<FoldDesc>& folds = B.h_fold;
   std::vector<GemmDesc>* gm = B.h_gemm;
   for (int i = 0; i < np; ++i) {
     const PartGeom& p = c.topo.parts[i];
@@ -835,16 +860,17 @@ int finalize_batched(Ctx& c) {
       const size_t nx = P.recs.size() - ARD_MAX_REC_PER_PART;
       if ((r = c.dalloc((void**)&P.xrec_part, size_t(P.xnb) * nx * 4))) return r;
     }
-    for (int w = 0; w < 3; ++w) folds.push_back(FoldDesc{P.F[w], P.S[w], P.plane[w], vec_of(P.plane[w], {P.F[w], P.S[w]})});
     const float *fw, *iw, *fh, *ih, *fd, *id;
     if ((r = c.cos_tables(p.w, &fw, &iw)) || (r = c.cos_tables(p.h, &fh, &ih)) || (r = c.cos_tables(p.d, &fd, &id)))
       return r;
-    // x faces [6][D][H]: along y (contiguous) then z; y faces [6][D][W]: x then z; z faces [6][H][W]: x then y
-    gm[0].push_back(gemm_k_contig(fh, p.h, P.S[0], P.U[0], int(6 * D), H));
-    gm[0].push_back(gemm_k_contig(fw, p.w, P.S[1], P.U[1], int(6 * D), W));
-    gm[0].push_back(gemm_k_contig(fw, p.w, P.S[2], P.U[2], int(6 * H), W));
+    // x faces [6][D][H]: along y (contiguous) then z; y faces [6][D][W]: x then z; z faces [6][H][W]: x then y.
+    // The forward passes transform the six raw force slabs of each face pair
+    // (the modal pass folds their spectra: the transforms are linear)
+    gm[0].push_back(gemm_k_contig(fh, p.h, P.F[0], P.U[0], int(6 * D), H));
+    gm[0].push_back(gemm_k_contig(fw, p.w, P.F[1], P.U[1], int(6 * D), W));
+    gm[0].push_back(gemm_k_contig(fw, p.w, P.F[2], P.U[2], int(6 * H), W));
     {
-      // x-face spectra are stored [D][H][6] (the six folded slabs of one
+      // x-face spectra are stored [D][H][6] (the six slabs of one
       // (kz, ky) adjacent), so the modal pass reads a row group with one
       // scalar load
       GemmDesc g = gemm_n_contig(fd, p.d, P.U[0], P.G[0], p.h);
@@ -900,8 +926,9 @@ int finalize_batched(Ctx& c) {
   if ((r = build_stage(c))) return r;
   // PML slabs: one grouped launch
   B.npml = int(c.h_pml.size());
-  if (B.npml && (r = c.dalloc((void**)&B.d_pml, c.h_pml.size() * sizeof(PmlDesc)))) return r;
-  if ((r = build_pml_table(c, 0))) return r;
+  for (int k = 0; k < 2 && B.npml; ++k)
+    if ((r = c.dalloc((void**)&B.d_pml[k], c.h_pml.size() * sizeof(PmlDesc)))) return r;
+  if ((r = build_pml_table(c, 0, B.pml_split))) return r;
   // partition access for the interface kernel
   std::vector<PartAccess> pa(np);
   for (int i = 0; i < np; ++i) {
@@ -965,27 +992,24 @@ int enqueue_step(Ctx& c, int par, PhaseTimer* T) {
   // one stream, phase by phase over all partitions.
   const bool fork = !T && B.side;
   const BatchedState::Stage& S = B.stage;
-  if (fork) ok(hipEventRecord(B.ev_fork, st));
-  // the fold is enqueued first, so that its blocks are dispatched before the
-  // PML slabs' fill the CUs; the PML still depends on the sources only.
-  // Forking after the fold or after the first forward transform was slower
-  // (the fold is stretched less, but the PML then meets the modal pass or
-  // loses its overlap).
-  phase(ARD_PH_FOLD, st, [&] { return launch_fold(S.fold, S.fold_start, S.nfold, S.fold_total, st); });
-  if (fork) ok(hipStreamWaitEvent(B.side, B.ev_fork, 0));
-  {
-    hipStream_t sp = fork ? B.side : st;
-    phase(ARD_PH_PML, sp, [&] {
-      return launch_pml(B.d_pml, B.npml, B.pml_blocks, par, c.prm.pml_layers, c.prm.h, c.prm.dt, c.prm.c0, sp,
-                        B.pml_shape);
-    });
-  }
-  if (fork) ok(hipEventRecord(B.ev_join, B.side));
   auto face = [&](int k) {
     static const int ph[4] = {ARD_PH_FACE_FWD1, ARD_PH_FACE_FWD2, ARD_PH_FACE_INV1, ARD_PH_FACE_INV2};
     phase(ph[k], st, [&] { return launch_face(B, S, k, B.gemm_variant, st); });
   };
+  // PML group 0 beside the forward transforms (forked at the step's start:
+  // it depends on the sources only). The first forward pass is enqueued
+  // first, so that its blocks are dispatched before the PML slabs' fill the
+  // CUs
+  const bool early = fork && B.pml_n[0] > 0;
+  if (early) ok(hipEventRecord(B.ev_fork, st));
   face(0);
+  if (early) {
+    ok(hipStreamWaitEvent(B.side, B.ev_fork, 0));
+    r |= launch_pml_group(c, B, 0, par, B.side);
+  } else if (!fork) {
+    // timed: both groups here, in one phase
+    phase(ARD_PH_PML, st, [&] { return launch_pml_group(c, B, 0, par, st) | launch_pml_group(c, B, 1, par, st); });
+  }
   face(1);
   {
     // with the extra point passes (partitions of more than four interior
@@ -998,10 +1022,22 @@ int enqueue_step(Ctx& c, int par, PhaseTimer* T) {
       return e;
     });
   }
+  if (fork) {
+    // PML group 1 beside the unfold and the inverse transforms (by default
+    // the whole PML: on the hall 0.622 ms per step, against 0.630 with the
+    // whole PML beside the forward transforms and 0.641-0.645 with it split
+    // between the two, 25-75 % beside the forward ones: there it outlasts
+    // them and slows the memory-bound modal pass). The fork also keeps the
+    // side stream in the capture when group 1 is empty.
+    ok(hipEventRecord(B.ev_fork2, st));
+    ok(hipStreamWaitEvent(B.side, B.ev_fork2, 0));
+    if (B.pml_n[1]) r |= launch_pml_group(c, B, 1, par, B.side);
+    ok(hipEventRecord(B.ev_join, B.side));
+  }
   phase(ARD_PH_UNFOLD, st, [&] { return launch_unfold(S.unfold, S.unfold_start, S.nunfold, S.unfold_total, st); });
   face(2);
   face(3);
-  // the PML stencil overlaps the fold and the face transforms and is joined
+  // the PML stencil overlaps the forward face transforms and is joined
   // here, at its first consumer (long finished by then, so the join costs the
   // step no wait)
   if (fork) ok(hipStreamWaitEvent(st, B.ev_join, 0));
@@ -1104,8 +1140,8 @@ int autotune(Ctx& c) {
   B.ring = tilings[bt].ring;
   // the FP32 transform kernels compete on speed; BF16 split precision, when
   // requested, is kept (a precision choice, not a speed one). They are timed
-  // as the step graph runs them: the fold, then the forward passes with the
-  // PML slabs beside them on the second stream, then the inverse passes (the
+  // as the step graph runs them, with the PML slabs beside them on the
+  // second stream where the step places them (the
   // kernels share the CUs, and one that issues more memory instructions
   // loses more to the PML than it gains alone).
   if (B.gemm_variant != 2) {
@@ -1115,21 +1151,27 @@ int autotune(Ctx& c) {
     const BatchedState::Stage& S = B.stage;
     hipStream_t st = c.stream;
     int e = 0;
+    // the step's transforms with the PML beside them as the step runs it
     auto faces = [&](int v) {
       const bool fork = B.side && B.npml > 0;
       if (fork) e |= hipEventRecord(B.ev_fork, st) != hipSuccess;
-      e |= launch_fold(S.fold, S.fold_start, S.nfold, S.fold_total, st);
+      e |= launch_face(B, S, 0, v, st);
       if (fork) {
         e |= hipStreamWaitEvent(B.side, B.ev_fork, 0) != hipSuccess;
-        e |= launch_pml(B.d_pml, B.npml, B.pml_blocks, 0, c.prm.pml_layers, c.prm.h, c.prm.dt, c.prm.c0, B.side,
-                        B.pml_shape);
+        e |= launch_pml_group(c, B, 0, 0, B.side);
+      }
+      e |= launch_face(B, S, 1, v, st);
+      if (fork) {
+        if (B.pml_n[1]) {
+          e |= hipEventRecord(B.ev_fork2, st) != hipSuccess;
+          e |= hipStreamWaitEvent(B.side, B.ev_fork2, 0) != hipSuccess;
+          e |= launch_pml_group(c, B, 1, 0, B.side);
+        }
         e |= hipEventRecord(B.ev_join, B.side) != hipSuccess;
       }
-      e |= launch_face(B, S, 0, v, st);
-      e |= launch_face(B, S, 1, v, st);
-      if (fork) e |= hipStreamWaitEvent(st, B.ev_join, 0) != hipSuccess;
       e |= launch_face(B, S, 2, v, st);
       e |= launch_face(B, S, 3, v, st);
+      if (fork) e |= hipStreamWaitEvent(st, B.ev_join, 0) != hipSuccess;
     };
     std::vector<float> gms(cand.size(), 1e30f);
     for (int round = 0; round < 3; ++round)
@@ -1141,11 +1183,11 @@ int autotune(Ctx& c) {
       float pms[2] = {1e30f, 1e30f};
       for (int round = 0; round < 3 && !e; ++round)
         for (int sh = 0; sh < 2 && !e; ++sh) {
-          e |= build_pml_table(c, sh) != ARD_OK;
+          e |= build_pml_table(c, sh, B.pml_split) != ARD_OK;
           if (!e) pms[sh] = std::min(pms[sh], time_ms([&] { faces(B.gemm_variant); }, 3));
         }
       tried += 2;
-      if (!e) e |= build_pml_table(c, pms[1] < pms[0] ? 1 : 0) != ARD_OK;
+      if (!e) e |= build_pml_table(c, pms[1] < pms[0] ? 1 : 0, B.pml_split) != ARD_OK;
     }
     if (e && !r) r = set_error(ARD_ERR_HIP, "autotuner: transform timing");
   }
@@ -1185,7 +1227,11 @@ int batched_set_option(Ctx& c, int option, int value) {
   }
   if (option == ARD_OPT_PML_SHAPE) {
     if (value != 0 && value != 1) return set_error(ARD_ERR_INVALID, "PML tile shape must be 0 or 1");
-    return build_pml_table(c, value);
+    return build_pml_table(c, value, B.pml_split);
+  }
+  if (option == ARD_OPT_PML_SPLIT) {
+    if (value < 0 || value > 100) return set_error(ARD_ERR_INVALID, "PML split must be a percentage in [0, 100]");
+    return build_pml_table(c, B.pml_shape, value);
   }
   return set_error(ARD_ERR_INVALID, "unknown batched option");
 }
@@ -1194,6 +1240,7 @@ int batched_option(Ctx& c, int option, int* value) {
   if (!c.bs) return set_error(ARD_ERR_STATE, "not a finalized batched context");
   if (option == ARD_OPT_RING) *value = c.bs->rpw == 4 ? c.bs->ring : 0;
   else if (option == ARD_OPT_PML_SHAPE) *value = c.bs->pml_shape;
+  else if (option == ARD_OPT_PML_SPLIT) *value = c.bs->pml_split;
   else return set_error(ARD_ERR_INVALID, "unknown batched option");
   return ARD_OK;
 }

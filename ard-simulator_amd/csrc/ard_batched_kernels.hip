@@ -39,34 +39,6 @@ __device__ __forceinline__ int find_desc_ll(const long long* __restrict__ starts
 typedef __attribute__((address_space(1))) v4f gv4f;
 
 // ---------------------------------------------------------------------------
-// Fold: face pair (x, n-1-x) -> sum / difference planes, x = 0..2.
-// ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_fold(const FoldDesc* __restrict__ d, const long long* __restrict__ starts,
-                                              int n) {
-  const long long b0 = (long long)blockIdx.x * 256;
-  const int i = find_desc_ll(starts, n, b0);  // block-uniform
-  const FoldDesc& F = d[i];
-  const long long it = b0 - starts[i] + threadIdx.x;
-  const long long pl = F.plane;
-  if (it * F.vec >= 3 * pl) return;
-  const long long cell = it * F.vec;
-  const int q = int(cell / pl);
-  const long long o = cell - q * pl;
-  const gfloat* src = ARD_CG(F.src);
-  gfloat* dst = ARD_G(F.dst);
-  if (F.vec == 4) {
-    const v4f a = *reinterpret_cast<const gv4f*>(src + q * pl + o);
-    const v4f b = *reinterpret_cast<const gv4f*>(src + (5 - q) * pl + o);
-    *reinterpret_cast<gv4f*>(dst + q * pl + o) = v4f{a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w};
-    *reinterpret_cast<gv4f*>(dst + (3 + q) * pl + o) = v4f{a.x - b.x, a.y - b.y, a.z - b.z, a.w - b.w};
-  } else {
-    const float a = src[q * pl + o], b = src[(5 - q) * pl + o];
-    dst[q * pl + o] = a + b;
-    dst[(3 + q) * pl + o] = a - b;
-  }
-}
-
-// ---------------------------------------------------------------------------
 // Unfold: sum the per-chunk partial reductions and turn the (even, odd) pair
 // back into the face pair: x -> even + odd, n-1-x -> even - odd.
 // ---------------------------------------------------------------------------
@@ -300,14 +272,23 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RPW <= 4 ? 
 #pragma unroll
     for (int j = 0; j < 3; ++j) ey[r][j] = yt[j];
   }
-  // z-face force spectrum (both kz parities) and z sums of this thread's rows
+  // z-face force spectrum of both kz parities, folded from the six slabs'
+  // spectra (slab q at z = q, its mirror 5 - q at z = D - 1 - q, and
+  // c_D(kz, D - 1 - z) = (-1)^kz c_D(kz, z)): q < 3 the sums (even kz), q >= 3
+  // the differences (odd kz); and the z sums of this thread's rows
   float gz[RPW][6], tz[RPW][6];
 #pragma unroll
   for (int r = 0; r < RPW; ++r) {
+    float raw[6];
 #pragma unroll
     for (int q = 0; q < 6; ++q) {
-      gz[r][q] = ARD_CG(P.Gz)[q * HW + rof[r] + kxc];
+      raw[q] = ARD_CG(P.Gz)[q * HW + rof[r] + kxc];
       tz[r][q] = 0.f;
+    }
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      gz[r][q] = raw[q] + raw[5 - q];
+      gz[r][3 + q] = raw[q] - raw[5 - q];
     }
   }
   // point forces: x factor (times the source value) per lane
@@ -325,7 +306,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RPW <= 4 ? 
   // them before then, so no wait for their loads.
   constexpr int NGX = KB * TY * 8, NZT = KB * 8, NPC = PTS ? KB * 8 : 0, NPY = PTS ? TY * 8 : 0;
   constexpr int NSTG = (NGX + NZT + NPC + NPY + 255) / 256;
-  float stg[NSTG];
+  // the x-face spectra fill the first NSX slots of every thread: two loads
+  // each (a slab's spectrum and its mirror's), folded when stored
+  constexpr int NSX = NGX / 256;
+  static_assert(NGX % 256 == 0 && NSX >= 1, "x-face staging must fill whole slots");
+  float stg[NSTG], stg2[NSX];
   const gfloat* const gxt = pinned(P.Gx);
   const gfloat* const ztt = pinned(P.zt);
   const gfloat* const pzt = pinned(PTS ? P.ptz : P.zt);
@@ -337,15 +322,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RPW <= 4 ? 
 #pragma unroll
     for (int t = 0; t < NSTG; ++t) {
       const int e = tid + 256 * t;
-      const gfloat* src = ztt;
-      int idx = 0;  // past the staged tables: any valid address, value unused
-      if (e < NGX) {
-        // slots 0..2 even spectra, 4..6 odd; slots 3 and 7 are never read
+      if (t < NSX) {
+        // slots 0..2 the even-kx spectra (sum of slab i = sl and its mirror
+        // 5 - i), 4..6 the odd ones (difference); slots 3 and 7 are never read
         const int sl = e & 7, row = (e >> 3) % TY, kz = min(kb + e / (8 * TY), kz1 - 1);
         const int ky = min(kyb + row, H - 1);
-        src = gxt;
-        idx = (kz * H + ky) * 6 + (sl < 3 ? sl : sl < 7 ? sl - 1 : 5);
-      } else if (e < NGX + NZT) {
+        const int i = min(sl & 3, 2), base = (kz * H + ky) * 6;
+        stg[t] = gxt[base + i];
+        stg2[t < NSX ? t : 0] = gxt[base + 5 - i];
+        continue;
+      }
+      const gfloat* src = ztt;
+      int idx = 0;  // past the staged tables: any valid address, value unused
+      if (e < NGX + NZT) {
         const int f = e - NGX;
         idx = min(kb + (f >> 3), kz1 - 1) * 8 + (f & 7);
       } else if (PTS && e < NGX + NZT + NPC) {
@@ -364,7 +353,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RPW <= 4 ? 
 #pragma unroll
     for (int t = 0; t < NSTG; ++t) {
       const int e = tid + 256 * t;
-      if (e < NGX) (&s_gx[0][0][0])[e] = stg[t];
+      if (t < NSX) (&s_gx[0][0][0])[e] = stg[t] + ((tid & 4) ? -stg2[t < NSX ? t : 0] : stg2[t < NSX ? t : 0]);
       else if (e < NGX + NZT) (&s_zt[0][0])[e - NGX] = stg[t];
       else if (PTS && e < NGX + NZT + NPC) (&s_ptz[0][0])[e - NGX - NZT] = stg[t];
       else if (PTS && e < NGX + NZT + NPC + NPY) (&s_pty[0][0])[e - NGX - NZT - NPC] = stg[t];
@@ -443,6 +432,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RPW <= 4 ? 
         mp[r] = in.mp[r];
       }
     }
+    // y-face spectra of both ky parities from the six slabs' (as for z)
+    float gy[6];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      gy[q] = in.gy[q] + in.gy[5 - q];
+      gy[3 + q] = in.gy[q] - in.gy[5 - q];
+    }
     const float4 za = *reinterpret_cast<const float4*>(&s_zt[kl][0]);  // Ez[kz][0..2], Rz[kz][0]
     const float4 zb = *reinterpret_cast<const float4*>(&s_zt[kl][4]);  // Rz[kz][1..2], u_z, 0
     const float uxz = ux + zb.z;
@@ -463,9 +459,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RPW <= 4 ? 
       float fh = ex[0] * ga.x;
       fh = fmaf(ex[1], ga.y, fh);
       fh = fmaf(ex[2], ga.z, fh);
-      fh = fmaf(ey[r][0], in.gy[PY * 3 + 0], fh);
-      fh = fmaf(ey[r][1], in.gy[PY * 3 + 1], fh);
-      fh = fmaf(ey[r][2], in.gy[PY * 3 + 2], fh);
+      fh = fmaf(ey[r][0], gy[PY * 3 + 0], fh);
+      fh = fmaf(ey[r][1], gy[PY * 3 + 1], fh);
+      fh = fmaf(ey[r][2], gy[PY * 3 + 2], fh);
       fh = fmaf(za.x, gz[r][PZ * 3 + 0], fh);
       fh = fmaf(za.y, gz[r][PZ * 3 + 1], fh);
       fh = fmaf(za.z, gz[r][PZ * 3 + 2], fh);
@@ -773,12 +769,6 @@ __global__ __launch_bounds__(1024) void k_brecord(const BRecDesc* __restrict__ r
 // launchers
 // ---------------------------------------------------------------------------
 static int grid_of(long long total) { return int((total + 255) / 256); }
-
-int launch_fold(const FoldDesc* d, const long long* starts, int n, long long total, hipStream_t s) {
-  if (total <= 0) return 0;
-  hipLaunchKernelGGL(k_fold, dim3(grid_of(total)), dim3(256), 0, s, d, starts, n);
-  return hipGetLastError() == hipSuccess ? 0 : -2;
-}
 
 int launch_unfold(const UnfoldDesc* d, const long long* starts, int n, long long total, hipStream_t s) {
   if (total <= 0) return 0;

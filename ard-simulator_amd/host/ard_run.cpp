@@ -36,6 +36,8 @@
 //                 ARD_OPT_RING, 0 or 2), for runs that skip the autotuner
 //   --pml-shape N batched form: the tiled PML update's tile shape
 //                 (ARD_OPT_PML_SHAPE, 0 or 1), likewise
+//   --pml-split N batched form: percent of the PML's tile blocks beside
+//                 the forward face transforms (ARD_OPT_PML_SPLIT), likewise
 #include <hip/hip_runtime_api.h>
 
 #include <sys/stat.h>
@@ -55,7 +57,7 @@ using namespace ard::host;
 int main(int argc, char** argv) {
   bool cells = false, timing = false, output = true;
   long long copy_mib = 0;
-  int check_every = 0, ring = -1, pml_shape = -1;
+  int check_every = 0, ring = -1, pml_shape = -1, pml_split = -1;
   bool info = false;
   const char* frame = nullptr;
   int look_from = 0;
@@ -83,6 +85,7 @@ int main(int argc, char** argv) {
     else if (!std::strcmp(argv[i], "--info")) info = true;
     else if (!std::strcmp(argv[i], "--ring") && i + 1 < argc) ring = std::atoi(argv[++i]);
     else if (!std::strcmp(argv[i], "--pml-shape") && i + 1 < argc) pml_shape = std::atoi(argv[++i]);
+    else if (!std::strcmp(argv[i], "--pml-split") && i + 1 < argc) pml_split = std::atoi(argv[++i]);
     else if (argv[i][0] == '-' && argv[i][1] == '-') {
       std::fprintf(stderr, "unknown option %s\n", argv[i]);
       return 2;
@@ -93,7 +96,7 @@ int main(int argc, char** argv) {
   if (pos.size() < 2) {
     std::fprintf(stderr,
                  "usage: %s [--cells] [--rec-local] [--plain] [--autotune] [--transform V] [--timing] [--no-output] [--no-cubes] "
-                 "[--c0 V] [--duration S] [--frame F] [--look-from V] [--copy-mib N] [--check-every N] [--info] [--ring N] [--pml-shape N] scene.txt sources.txt [recorders.txt] [steps] [h] [out_root]\n",
+                 "[--c0 V] [--duration S] [--frame F] [--look-from V] [--copy-mib N] [--check-every N] [--info] [--ring N] [--pml-shape N] [--pml-split N] scene.txt sources.txt [recorders.txt] [steps] [h] [out_root]\n",
                  argv[0]);
     return 2;
   }
@@ -129,6 +132,8 @@ int main(int argc, char** argv) {
     if (timing && ard_set_kernel_timing(sim.context(), 1) != ARD_OK) throw Error(ard_last_error());
     if (ring >= 0 && ard_set_batched_option(sim.context(), ARD_OPT_RING, ring) != ARD_OK) throw Error(ard_last_error());
     if (pml_shape >= 0 && ard_set_batched_option(sim.context(), ARD_OPT_PML_SHAPE, pml_shape) != ARD_OK)
+      throw Error(ard_last_error());
+    if (pml_split >= 0 && ard_set_batched_option(sim.context(), ARD_OPT_PML_SPLIT, pml_split) != ARD_OK)
       throw Error(ard_last_error());
     auto t1 = std::chrono::steady_clock::now();
     std::cout << "Initialization finished. (" << std::chrono::duration<double>(t1 - t0).count() << " s)" << std::endl;

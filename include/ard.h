@@ -207,7 +207,7 @@ int ard_synchronize(ard_ctx* ctx);
  * since timing was switched on. Phases: */
 #define ARD_PH_SOURCES 0
 #define ARD_PH_PML 1
-#define ARD_PH_FOLD 2
+#define ARD_PH_FOLD 2 /* batched form, no longer launched: the forward transforms read the raw force slabs */
 #define ARD_PH_FACE_FWD1 3
 #define ARD_PH_FACE_FWD2 4
 #define ARD_PH_MODAL 5
@@ -288,12 +288,17 @@ int ard_set_batched_tiling(ard_ctx* ctx, int rows_per_wave, int kz_per_block);
  *                     slab's normal, 32 x 8 cells for y- and z-normal slabs
  *                     and 16 x 16 for x-normal ones (0, default), or the
  *                     other way round (1).
+ *   ARD_OPT_PML_SPLIT 0..100: percent of the PML update's tile blocks
+ *                     (leading slabs, whole slabs) launched beside the
+ *                     forward face transforms; the rest run beside the
+ *                     unfold and the inverse face transforms (default 50).
  * ARD_ERR_STATE for a context that is not a finalized batched one,
  * ARD_ERR_INVALID for other options or values. No counterpart in the
  * reference: they schedule the step of ARD/simulation.cpp:300-317 on one
  * device. */
 #define ARD_OPT_RING 2
 #define ARD_OPT_PML_SHAPE 3
+#define ARD_OPT_PML_SPLIT 4
 int ard_set_batched_option(ard_ctx* ctx, int option, int value);
 int ard_batched_option(ard_ctx* ctx, int option, int* value);
 

@@ -52,7 +52,8 @@ def test_batched_schedule_options_bit_identical():
     steps = 300
     traces = {}
     for name, opts in (("default", {}), ("ring2", {A.OPT_RING: 2}), ("pml_shape1", {A.OPT_PML_SHAPE: 1}),
-                       ("both", {A.OPT_RING: 2, A.OPT_PML_SHAPE: 1})):
+                       ("both", {A.OPT_RING: 2, A.OPT_PML_SHAPE: 1}), ("split100", {A.OPT_PML_SPLIT: 100}),
+                       ("split50", {A.OPT_PML_SPLIT: 50}), ("split33_shape1", {A.OPT_PML_SPLIT: 33, A.OPT_PML_SHAPE: 1})):
         print("case options", name, flush=True)
         sim = A.Simulation(sc, form=A.FORM_BATCHED)
         sim.set_batched_tiling(4, 48)
@@ -129,6 +130,10 @@ def test_batched_option_values_checked():
         sim.set_batched_option(A.OPT_RING, 3)
     with pytest.raises(A.ArdError):
         sim.set_batched_option(A.OPT_PML_SHAPE, 2)
+    for bad in (-1, 101):
+        with pytest.raises(A.ArdError):
+            sim.set_batched_option(A.OPT_PML_SPLIT, bad)
+    assert sim.batched_option(A.OPT_PML_SPLIT) == 0
     with pytest.raises(A.ArdError):
         sim.set_batched_option(99, 1)
     sim.close()
