@@ -175,3 +175,30 @@ def test_ard_run_info_and_finite_checks(tmp_path, form):
         # Info prints start "xs,ys,zs" and end "xe,ye,ze"
         assert (ln[1], int(ln[2]), int(ln[3])) == (typ, b[1], b[2])
         assert tuple(int(v) for v in ln[4:]) == (b[3], b[5], b[7], b[4], b[6], b[8])
+
+
+def test_pml_placements_bit_identical_traces():
+    """The PML's launch placements (beside the forward transforms, beside the
+    unfold and inverse transforms, or split between them) change no
+    arithmetic: bit-identical traces and fields."""
+    sc = S.scene3()
+    steps = 200
+    out = {}
+    A.set_transform_variant(3)
+    try:
+        for name, opts in (("split0", {}), ("split100", {A.OPT_PML_SPLIT: 100}),
+                           ("split40", {A.OPT_PML_SPLIT: 40}), ("split70_shape1", {A.OPT_PML_SPLIT: 70, A.OPT_PML_SHAPE: 1})):
+            print("case", name, flush=True)
+            sim = A.Simulation(sc, form=A.FORM_BATCHED)
+            for k, v in opts.items():
+                sim.set_batched_option(k, v)
+                assert sim.batched_option(k) == v
+            sim.update(steps)
+            out[name] = (sim.recorders(0, steps), [sim.pressure_field(i) for i in range(len(sim.partitions()))])
+            sim.close()
+    finally:
+        A.set_transform_variant(1)
+    for name, (tr, fields) in out.items():
+        assert np.array_equal(tr, out["split0"][0]), name
+        for a, b in zip(fields, out["split0"][1]):
+            assert np.array_equal(a, b), name
