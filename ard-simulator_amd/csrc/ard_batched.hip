@@ -1024,17 +1024,22 @@ int enqueue_step(Ctx& c, int par, PhaseTimer* T) {
   }
   if (fork) {
     // PML group 1 beside the unfold and the inverse transforms (by default
-    // the whole PML: on the hall 0.622 ms per step, against 0.630 with the
+    // the whole PML: on the hall 0.606 ms per step, against 0.632 with the
     // whole PML beside the forward transforms and 0.641-0.645 with it split
     // between the two, 25-75 % beside the forward ones: there it outlasts
     // them and slows the memory-bound modal pass). The fork also keeps the
-    // side stream in the capture when group 1 is empty.
+    // side stream in the capture when group 1 is empty. The unfold is
+    // enqueued before the PML's launch (the PML still depends on the modal
+    // pass only), so the graph holds its node first and its blocks take the
+    // CUs first: 0.620 -> 0.606 ms.
     ok(hipEventRecord(B.ev_fork2, st));
+  }
+  phase(ARD_PH_UNFOLD, st, [&] { return launch_unfold(S.unfold, S.unfold_start, S.nunfold, S.unfold_total, st); });
+  if (fork) {
     ok(hipStreamWaitEvent(B.side, B.ev_fork2, 0));
     if (B.pml_n[1]) r |= launch_pml_group(c, B, 1, par, B.side);
     ok(hipEventRecord(B.ev_join, B.side));
   }
-  phase(ARD_PH_UNFOLD, st, [&] { return launch_unfold(S.unfold, S.unfold_start, S.nunfold, S.unfold_total, st); });
   face(2);
   face(3);
   // the PML stencil overlaps the forward face transforms and is joined
