@@ -291,7 +291,8 @@ int ard_set_batched_tiling(ard_ctx* ctx, int rows_per_wave, int kz_per_block);
  *   ARD_OPT_PML_SPLIT 0..100: percent of the PML update's tile blocks
  *                     (leading slabs, whole slabs) launched beside the
  *                     forward face transforms; the rest run beside the
- *                     unfold and the inverse face transforms (default 50).
+ *                     unfold and the inverse face transforms (default 0:
+ *                     the whole PML beside the unfold and inverse ones).
  * ARD_ERR_STATE for a context that is not a finalized batched one,
  * ARD_ERR_INVALID for other options or values. No counterpart in the
  * reference: they schedule the step of ARD/simulation.cpp:300-317 on one
