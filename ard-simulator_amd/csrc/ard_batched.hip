@@ -1166,15 +1166,17 @@ int autotune(Ctx& c) {
         e |= launch_pml_group(c, B, 0, 0, B.side);
       }
       e |= launch_face(B, S, 1, v, st);
+      // group 1 forked here and enqueued after the next launch on the main
+      // stream, as the step enqueues it after the unfold
+      if (fork && B.pml_n[1]) e |= hipEventRecord(B.ev_fork2, st) != hipSuccess;
+      e |= launch_face(B, S, 2, v, st);
       if (fork) {
         if (B.pml_n[1]) {
-          e |= hipEventRecord(B.ev_fork2, st) != hipSuccess;
           e |= hipStreamWaitEvent(B.side, B.ev_fork2, 0) != hipSuccess;
           e |= launch_pml_group(c, B, 1, 0, B.side);
         }
         e |= hipEventRecord(B.ev_join, B.side) != hipSuccess;
       }
-      e |= launch_face(B, S, 2, v, st);
       e |= launch_face(B, S, 3, v, st);
       if (fork) e |= hipStreamWaitEvent(st, B.ev_join, 0) != hipSuccess;
     };
